@@ -1,0 +1,178 @@
+/*
+ * fg_oracle.h -- CPU oracle for the flip-graph random walk.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker (or the timed CPU baseline).  The product path (libflipgraph.so,
+ * the CLIs) never links, loads or calls anything under oracle/.
+ *
+ * This is a plain-C restatement of the reference's walk logic
+ * (dronperminov/FlipGraphGPU).  Every function cites the reference
+ * file:line it follows.  Two deliberate, documented differences:
+ *
+ *  - RNG: cuRAND is replaced by Philox4x32-10 (Salmon et al., SC'11),
+ *    keyed per walker.  One Philox block feeds each walk iteration; rare
+ *    operators draw from a second (aux) stream of the same generator.
+ *  - Pair order: the reference samples `curand() % size` into a FlipSet
+ *    whose order depends on its add/remove history
+ *    (src/entities/flip_set.cu:7-31).  In CANONICAL mode the pair list is
+ *    always taken in the order a fresh initFlips() builds
+ *    (src/schemes/scheme_z2.cu:141-150: component, then index1, then
+ *    index2).  The set of pairs, and therefore the distribution of flips,
+ *    is the same.  REFERENCE mode keeps the history-ordered FlipSet
+ *    exactly as the reference does (used for the CPU baseline).
+ *
+ * Pins: see tests/test_oracle_*.py (Philox known-answer vectors, closed
+ * forms, Strassen, brute-force invariants, planted-mistake checks).
+ */
+#ifndef FG_ORACLE_H
+#define FG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FGO_MAX_RANK 350  /* src/config.cuh:10 */
+#define FGO_MAX_PAIRS 500 /* src/config.cuh:14 */
+#define FGO_RING_Z2 0
+#define FGO_RING_ZT 1
+#define FGO_SELECT_CANONICAL 0
+#define FGO_SELECT_REFERENCE 1
+#define FGO_PLUS_MAX_TRIES 1000 /* bound on tryPlus rejection (reference loops forever) */
+
+/* src/entities/flip_set.cuh:5-18 */
+typedef struct {
+    int size;
+    uint32_t pairs[FGO_MAX_PAIRS];
+} fgo_flipset;
+
+/* src/schemes/scheme_z2.cuh:21-27 and src/schemes/scheme_integer.cuh:16-21.
+ * Z2: v[c][t] is the factor bitmask.  ZT: (v, s) are Addition.values /
+ * Addition.signs (src/entities/addition.cuh:8-12), ok is Addition.valid. */
+typedef struct {
+    int ring;
+    int n[3];
+    int nn[3];
+    int m;
+    uint64_t v[3][FGO_MAX_RANK];
+    uint64_t s[3][FGO_MAX_RANK];
+    uint8_t ok[3][FGO_MAX_RANK];
+    fgo_flipset flips[3];
+} fgo_scheme;
+
+typedef struct {
+    uint32_t key[2];
+    uint32_t walker;
+    uint64_t it;  /* main-stream block counter: one block per walk iteration */
+    uint64_t aux; /* aux-stream block counter */
+    uint32_t abuf[4];
+    int aidx;     /* next unread aux word, 4 = empty */
+} fgo_rng;
+
+typedef struct {
+    int iterations;        /* walk-loop iterations per launch */
+    int max_iterations;    /* upper bound of the random iteration count */
+    int random_iterations; /* reference: resize probability > 0 */
+    int plus_iterations;
+    int use_reduce;
+    uint32_t thr_reduce;   /* P(reduce check) = thr / 2^32 */
+    int use_expand;
+    uint32_t thr_expand;
+    uint32_t thr_best_eq;  /* equal-rank best copy, 0.01 * 2^32 */
+    int select_mode;       /* FGO_SELECT_* */
+    int check_reduce;      /* tryFlip(checkReduce) */
+    int zt_visit_cap;      /* max candidates one integer tryFlip visits */
+} fgo_walk_params;
+
+typedef struct {
+    fgo_scheme cur;
+    fgo_scheme best;
+    int best_rank;
+    int flips_count;
+    fgo_rng rng;
+    uint64_t total_flips;
+    uint64_t sum_m;
+} fgo_walker;
+
+/* Walker record exchanged with tests; same layout as fg_walker_record in
+ * include/fg.h (written independently). */
+#define FGO_REC_MAX_M 256
+typedef struct {
+    int32_t n[3];
+    int32_t m;
+    int32_t best_m;
+    int32_t best_rank;
+    int32_t flips_count;
+    int32_t aux_idx;
+    uint64_t it;
+    uint64_t aux;
+    uint64_t total_flips;
+    uint64_t sum_m;
+    uint64_t cur_v[3][FGO_REC_MAX_M];
+    uint64_t cur_s[3][FGO_REC_MAX_M];
+    uint64_t best_v[3][FGO_REC_MAX_M];
+    uint64_t best_s[3][FGO_REC_MAX_M];
+} fgo_walker_record;
+
+/* ---- RNG ---- */
+void fgo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void fgo_rng_init(fgo_rng *r, uint64_t seed, uint32_t walker);
+void fgo_rng_main_block(fgo_rng *r, uint32_t out[4]);
+uint32_t fgo_rng_aux32(fgo_rng *r);
+
+/* ---- schemes ---- */
+int fgo_scheme_size(void);
+void fgo_init_naive(fgo_scheme *s, int ring, int n1, int n2, int n3);
+int fgo_validate(const fgo_scheme *s);
+int fgo_complexity(const fgo_scheme *s);
+void fgo_init_flips(fgo_scheme *s);
+int fgo_count_pairs(const fgo_scheme *s);
+/* blob: int32 [n1 n2 n3 m | U m*nn0 | V m*nn1 | W m*nn2] (text-input order) */
+int fgo_from_blob(fgo_scheme *s, int ring, const int32_t *blob, int check_validity);
+int fgo_to_blob(const fgo_scheme *s, int32_t *blob);
+int fgo_blob_len(const fgo_scheme *s);
+void fgo_copy(fgo_scheme *dst, const fgo_scheme *src, int with_flips);
+
+/* explicit operators (trace replay) */
+void fgo_flip(fgo_scheme *s, int i, int j, int k, int i1, int i2, int check_reduce);
+void fgo_plus(fgo_scheme *s, int i, int j, int k, int i1, int i2, int variant);
+void fgo_split_z2(fgo_scheme *s, int i, int j, int k, int index, uint64_t a1);
+void fgo_split_zt(fgo_scheme *s, int i, int j, int k, int index, uint64_t av, uint64_t as);
+void fgo_reduce(fgo_scheme *s, int i, int i1, int i2, int sub);
+int fgo_try_reduce(fgo_scheme *s);
+int fgo_fix_signs(fgo_scheme *s);
+int fgo_pick_canonical(const fgo_scheme *s, uint32_t r, int *c, int *t1, int *t2);
+
+/* ---- walk ---- */
+void fgo_walker_init(fgo_walker *w, const fgo_scheme *start, int best_rank, uint64_t seed, uint32_t walker);
+void fgo_walk(fgo_walker *w, const fgo_walk_params *p);
+void fgo_walker_to_record(const fgo_walker *w, fgo_walker_record *rec);
+int fgo_walker_from_record(fgo_walker *w, int ring, const fgo_walker_record *rec, uint64_t seed, uint32_t walker);
+
+/* Batch helpers for ctypes: walkers are passed as records.  Each runs
+ * `launches` launches of fgo_walk.  nthreads > 1 partitions walkers over
+ * pthreads (CPU baseline).  Returns 0 on success. */
+int fgo_walk_records(int ring, fgo_walker_record *recs, int count, uint32_t walker_base,
+                     uint64_t seed, const fgo_walk_params *p, int launches, int nthreads);
+int fgo_naive_records(int ring, int n1, int n2, int n3, fgo_walker_record *recs, int count);
+int fgo_record_validate(int ring, const fgo_walker_record *rec, int best);
+
+/* Blob-level helpers for ctypes. */
+int fgo_blob_validate(int ring, const int32_t *blob);
+int fgo_blob_complexity(int ring, const int32_t *blob);
+int fgo_blob_naive(int ring, int n1, int n2, int n3, int32_t *out);
+/* Replay an explicit operator trace.  ops: 8 int32 per op:
+ * [code, a0..a6].  code 0 flip(i,j,k,i1,i2,checkReduce); 1 plus(i,j,k,i1,i2,variant);
+ * 2 split(i,j,k,index, a_lo, a_hi[, s_lo]) (ZT: a = values, s = signs in a5/a6 low bits);
+ * 3 reduce(i,i1,i2,sub); 4 tryReduce; 5 fixSigns.  Returns new blob length or -1. */
+int fgo_blob_replay(int ring, const int32_t *in, const int32_t *ops, int nops, int32_t *out);
+int fgo_blob_pairs(int ring, const int32_t *blob, int mode, uint32_t *out, int cap);
+int fgo_check_flipset_walk(int ring, int n1, int n2, int n3, uint64_t seed, int iterations, int walkers,
+                           int plus_iterations);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
