@@ -1,0 +1,192 @@
+"""ctypes binding of libflipgraph.so (include/fg.h).
+
+The shared library holds the HIP kernels and the host runtime; this module
+only moves buffers across the C ABI.  It never falls back to CPU code: if
+the library is missing, importing the engine raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libflipgraph.so")
+
+RING_Z2 = 0
+RING_ZT = 1
+REC_MAX_M = 256
+
+RECORD_DTYPE = np.dtype([
+    ("n", np.int32, (3,)),
+    ("m", np.int32),
+    ("best_m", np.int32),
+    ("best_rank", np.int32),
+    ("flips_count", np.int32),
+    ("aux_idx", np.int32),
+    ("it", np.uint64),
+    ("aux", np.uint64),
+    ("total_flips", np.uint64),
+    ("sum_m", np.uint64),
+    ("cur_v", np.uint64, (3, REC_MAX_M)),
+    ("cur_s", np.uint64, (3, REC_MAX_M)),
+    ("best_v", np.uint64, (3, REC_MAX_M)),
+    ("best_s", np.uint64, (3, REC_MAX_M)),
+])
+
+
+class FgConfig(ctypes.Structure):
+    _fields_ = [
+        ("ring", ctypes.c_int32),
+        ("walkers", ctypes.c_int32),
+        ("max_iterations", ctypes.c_int32),
+        ("plus_iterations", ctypes.c_int32),
+        ("random_iterations", ctypes.c_int32),
+        ("check_reduce", ctypes.c_int32),
+        ("expand_probability", ctypes.c_double),
+        ("reduce_probability", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("max_rank", ctypes.c_int32),
+        ("zt_visit_cap", ctypes.c_int32),
+    ]
+
+
+class FgError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FgError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() (make -C flipgraphgpu_amd)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)
+    L.fg_last_error.restype = ctypes.c_char_p
+    L.fg_create.argtypes = [ctypes.POINTER(FgConfig), ctypes.POINTER(vp)]
+    L.fg_destroy.argtypes = [vp]
+    L.fg_init_naive.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.fg_init_from_schemes.argtypes = [vp, i32p, ctypes.c_int64]
+    L.fg_walk.argtypes = [vp, ctypes.c_int]
+    L.fg_best.argtypes = [vp, i32p, ctypes.c_int64, i32p, i32p]
+    L.fg_walker_scheme.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int64]
+    L.fg_verify.argtypes = [ctypes.c_int, i32p, ctypes.c_int64, i32p]
+    L.fg_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                              ctypes.POINTER(ctypes.c_uint64)]
+    L.fg_last_walk_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    L.fg_read_walkers.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.fg_write_walkers.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.fg_device_info.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                 ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_int]
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise FgError(f"{what}: {load_library().fg_last_error().decode()} (rc={rc})")
+    return rc
+
+
+def _i32(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def device_info():
+    L = load_library()
+    cu, clk, lds = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    name = ctypes.create_string_buffer(64)
+    _check(L.fg_device_info(ctypes.byref(cu), ctypes.byref(clk), ctypes.byref(lds), name, 64), "fg_device_info")
+    return {"cu": cu.value, "clock_khz": clk.value, "lds_per_block": lds.value, "arch": name.value.decode()}
+
+
+class Engine:
+    """One walker pool on the current GPU (fg_ctx)."""
+
+    def __init__(self, walkers, seed, ring=RING_Z2, max_iterations=10000, plus_iterations=8000,
+                 random_iterations=False, check_reduce=True, expand_probability=0.0,
+                 reduce_probability=0.0, max_rank=0, zt_visit_cap=64):
+        self.L = load_library()
+        self.ring = ring
+        self.cfg = FgConfig(ring, walkers, max_iterations, plus_iterations, int(random_iterations),
+                            int(check_reduce), expand_probability, reduce_probability, seed, max_rank,
+                            zt_visit_cap)
+        self.ctx = ctypes.c_void_p()
+        _check(self.L.fg_create(ctypes.byref(self.cfg), ctypes.byref(self.ctx)), "fg_create")
+        self.walkers = walkers
+
+    def close(self):
+        if self.ctx:
+            self.L.fg_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_naive(self, n1, n2, n3):
+        _check(self.L.fg_init_naive(self.ctx, n1, n2, n3), "fg_init_naive")
+
+    def init_from_schemes(self, blob):
+        b, p = _i32(blob)
+        _check(self.L.fg_init_from_schemes(self.ctx, p, len(b)), "fg_init_from_schemes")
+
+    def walk(self, iterations=0):
+        """One launch; returns the number of current schemes failing the GPU Brent check."""
+        return _check(self.L.fg_walk(self.ctx, iterations), "fg_walk")
+
+    def last_walk_ms(self):
+        x = ctypes.c_float()
+        _check(self.L.fg_last_walk_ms(self.ctx, ctypes.byref(x)), "fg_last_walk_ms")
+        return x.value
+
+    def counters(self):
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(self.L.fg_counters(self.ctx, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "fg_counters")
+        return {"flips": a.value, "sum_m": b.value, "iterations": c.value}
+
+    def read_walkers(self, begin=0, count=None):
+        count = self.walkers - begin if count is None else count
+        recs = np.zeros(count, dtype=RECORD_DTYPE)
+        _check(self.L.fg_read_walkers(self.ctx, begin, count, recs.ctypes.data), "fg_read_walkers")
+        return recs
+
+    def write_walkers(self, recs, begin=0):
+        recs = np.ascontiguousarray(recs, dtype=RECORD_DTYPE)
+        _check(self.L.fg_write_walkers(self.ctx, begin, len(recs), recs.ctypes.data), "fg_write_walkers")
+
+    def best(self, cap=1 << 16):
+        out = np.zeros(cap, dtype=np.int32)
+        rank, walker = ctypes.c_int32(), ctypes.c_int32()
+        n = _check(self.L.fg_best(self.ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cap,
+                                  ctypes.byref(rank), ctypes.byref(walker)), "fg_best")
+        return out[:n], rank.value, walker.value
+
+    def walker_scheme(self, walker, best=False, cap=1 << 16):
+        out = np.zeros(cap, dtype=np.int32)
+        n = _check(self.L.fg_walker_scheme(self.ctx, walker, int(best),
+                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cap),
+                   "fg_walker_scheme")
+        return out[:n]
+
+
+def verify(blobs, ring=RING_Z2):
+    """GPU Brent check of a list of single-scheme blobs; returns a bool array."""
+    L = load_library()
+    flat = np.concatenate([np.array([len(blobs)], dtype=np.int32)] + [np.asarray(b, dtype=np.int32) for b in blobs])
+    f, p = _i32(flat)
+    ok = np.zeros(len(blobs), dtype=np.int32)
+    _check(L.fg_verify(ring, p, len(f), ok.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "fg_verify")
+    return ok.astype(bool)

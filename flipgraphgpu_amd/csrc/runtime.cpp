@@ -1,0 +1,548 @@
+// Host runtime behind include/fg.h: owns the device walker pool of one
+// context, launches the walk and Brent-check kernels on the context's
+// stream and moves walker state between host records and the
+// structure-of-arrays device layout.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "fg.h"
+#include "kernels.h"
+#include "scheme_host.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string &msg, int code = -1) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return fail(std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, size_t n) {
+    return hipMalloc(reinterpret_cast<void **>(p), std::max<size_t>(n, 1) * sizeof(T));
+}
+
+uint32_t threshold(double p, int max_iterations) {
+    // P(u * maxIterations < p), u uniform in (0, 1]  ->  x < thr, x uniform 32-bit
+    if (p <= 0 || max_iterations <= 0) return 0;
+    const double x = p / (double)max_iterations * 4294967296.0;
+    if (x >= 4294967295.0) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
+constexpr uint32_t kThrBestEq = 42949673u;  // 0.01 * 2^32 (flip_graph.cu:545)
+
+}  // namespace
+
+struct fg_ctx {
+    fg_config cfg{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int W = 0;
+    int maxm = 0;
+    bool ready = false;
+    float last_ms = 0.f;
+    size_t lds_limit = 0;
+
+    uint32_t *cur = nullptr, *cur_s = nullptr, *best = nullptr, *best_s = nullptr;
+    int32_t *m = nullptr, *best_m = nullptr, *best_rank = nullptr, *flips_count = nullptr, *aux_idx = nullptr;
+    int32_t *fmt = nullptr, *err = nullptr, *ok = nullptr;
+    uint64_t *it = nullptr, *aux = nullptr, *total_flips = nullptr, *sum_m = nullptr;
+
+    void release() {
+        void *ptrs[] = {cur, cur_s, best, best_s, m, best_m, best_rank, flips_count, aux_idx,
+                        fmt, err, ok, it, aux, total_flips, sum_m};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+        cur = cur_s = best = best_s = nullptr;
+        m = best_m = best_rank = flips_count = aux_idx = fmt = err = ok = nullptr;
+        it = aux = total_flips = sum_m = nullptr;
+        ready = false;
+    }
+};
+
+extern "C" const char *fg_last_error(void) { return g_err.c_str(); }
+
+extern "C" int fg_device_info(int *cu_count, int *clock_khz, int *lds_per_block, char *name, int name_cap) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    if (clock_khz) *clock_khz = prop.clockRate;
+    if (lds_per_block) *lds_per_block = (int)prop.sharedMemPerBlock;
+    if (name && name_cap > 0) {
+        strncpy(name, prop.gcnArchName, (size_t)name_cap - 1);
+        name[name_cap - 1] = 0;
+    }
+    return 0;
+}
+
+extern "C" int fg_create(const fg_config *cfg, fg_ctx **out) {
+    if (!cfg || !out) return fail("fg_create: null argument");
+    *out = nullptr;
+    if (cfg->ring != FG_RING_Z2 && cfg->ring != FG_RING_ZT) return fail("fg_create: unknown ring");
+    if (cfg->walkers <= 0) return fail("fg_create: walkers must be positive");
+    if (cfg->max_iterations <= 0) return fail("fg_create: max_iterations must be positive");
+    if (cfg->plus_iterations < 0) return fail("fg_create: plus_iterations must be >= 0");
+    if (cfg->ring == FG_RING_ZT) return fail("fg_create: the integer ring walk kernel is not available in this build");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(std::string("fg_create: this build targets gfx950 (MI355X), found ") + prop.gcnArchName);
+    fg_ctx *c = new fg_ctx();
+    c->cfg = *cfg;
+    if (c->cfg.zt_visit_cap <= 0) c->cfg.zt_visit_cap = 64;
+    c->lds_limit = prop.sharedMemPerBlock;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return fail("fg_create: stream/event creation failed");
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" int fg_destroy(fg_ctx *ctx) {
+    if (!ctx) return 0;
+    ctx->release();
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+namespace {
+
+// Fill the pool from start schemes (walker w gets starts[w % count]) and
+// set bestRanks as FlipGraph::updateRanks does (flip_graph.cu:249-277): the
+// smallest rank per sorted format key, seeded with the naive rank of the
+// configured format.
+int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, int n1, int n2, int n3) {
+    if (starts.empty()) return fail("init: no schemes");
+    const int W = ctx->cfg.walkers;
+    int maxm = ctx->cfg.max_rank;
+    for (const auto &s : starts) {
+        maxm = std::max(maxm, std::max(s.m, s.naive_rank()));
+        if (s.ring != ctx->cfg.ring) return fail("init: ring mismatch");
+        for (int c = 0; c < 3; ++c)
+            if (s.nn[c] > 32) return fail("init: factors with more than 32 coefficients are not supported on the GPU path");
+    }
+    if (maxm > 255) return fail("init: more than 255 terms per scheme are not supported on the GPU path");
+    if (fgk::z2_lds_bytes(maxm) > ctx->lds_limit)
+        return fail("init: " + std::to_string(maxm) + " terms per walker exceed the LDS of one workgroup");
+
+    ctx->release();
+    ctx->W = W;
+    ctx->maxm = maxm;
+    const size_t plane = (size_t)3 * maxm * W;
+    std::vector<uint32_t> words(plane, 0);
+    std::vector<int32_t> m(W), best_m(W), best_rank(W), zeros(W, 0), aidx(W, 4), fmt((size_t)4 * W, 0);
+    std::vector<uint64_t> z64(W, 0);
+
+    std::map<std::string, int> best_by_key;
+    best_by_key[fg::format_key(n1, n2, n3)] = n1 * n2 * n3;
+    for (int w = 0; w < W; ++w) {
+        const fg::Scheme &s = starts[w % starts.size()];
+        m[w] = s.m;
+        best_m[w] = s.m;
+        fmt[4 * w] = s.n[0];
+        fmt[4 * w + 1] = s.n[1];
+        fmt[4 * w + 2] = s.n[2];
+        for (int c = 0; c < 3; ++c)
+            for (int t = 0; t < s.m; ++t) words[((size_t)c * maxm + t) * W + w] = (uint32_t)s.v[c][t];
+        const std::string key = fg::format_key(s.n[0], s.n[1], s.n[2]);
+        auto f = best_by_key.find(key);
+        if (f == best_by_key.end() || s.m < f->second) best_by_key[key] = s.m;
+    }
+    for (int w = 0; w < W; ++w) {
+        const fg::Scheme &s = starts[w % starts.size()];
+        best_rank[w] = best_by_key[fg::format_key(s.n[0], s.n[1], s.n[2])];
+    }
+
+    HIPCHK(dalloc(&ctx->cur, plane));
+    HIPCHK(dalloc(&ctx->best, plane));
+    HIPCHK(dalloc(&ctx->m, W));
+    HIPCHK(dalloc(&ctx->best_m, W));
+    HIPCHK(dalloc(&ctx->best_rank, W));
+    HIPCHK(dalloc(&ctx->flips_count, W));
+    HIPCHK(dalloc(&ctx->aux_idx, W));
+    HIPCHK(dalloc(&ctx->fmt, (size_t)4 * W));
+    HIPCHK(dalloc(&ctx->err, 1));
+    HIPCHK(dalloc(&ctx->ok, W));
+    HIPCHK(dalloc(&ctx->it, W));
+    HIPCHK(dalloc(&ctx->aux, W));
+    HIPCHK(dalloc(&ctx->total_flips, W));
+    HIPCHK(dalloc(&ctx->sum_m, W));
+    HIPCHK(hipMemcpy(ctx->cur, words.data(), plane * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->best, words.data(), plane * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->m, m.data(), W * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->best_m, best_m.data(), W * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->best_rank, best_rank.data(), W * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->flips_count, zeros.data(), W * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->aux_idx, aidx.data(), W * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->fmt, fmt.data(), (size_t)16 * W, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->it, z64.data(), W * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->aux, z64.data(), W * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->total_flips, z64.data(), W * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->sum_m, z64.data(), W * 8, hipMemcpyHostToDevice));
+    ctx->ready = true;
+    return 0;
+}
+
+int verify_planes(fg_ctx *ctx, const uint32_t *words, const int32_t *m, const int32_t *fmt, int count, int maxm,
+                  int32_t *ok_dev) {
+    fgk::VerifyArgs v{};
+    v.words = words;
+    v.signs = nullptr;
+    v.m = m;
+    v.fmt = fmt;
+    v.count = count;
+    v.max_m = maxm;
+    v.ok = ok_dev;
+    HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(ok_dev), 1, count, ctx->stream));
+    HIPCHK(fgk::launch_verify_z2(v, ctx->stream));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int fg_init_naive(fg_ctx *ctx, int n1, int n2, int n3) {
+    if (!ctx) return fail("fg_init_naive: null context");
+    if (n1 < 1 || n2 < 1 || n3 < 1) return fail("fg_init_naive: sizes must be positive");
+    if (n1 * n2 > 64 || n2 * n3 > 64 || n3 * n1 > 64) return fail("fg_init_naive: a factor exceeds 64 coefficients");
+    std::vector<fg::Scheme> starts{fg::naive_scheme(ctx->cfg.ring, n1, n2, n3)};
+    return init_pool(ctx, starts, n1, n2, n3);
+}
+
+extern "C" int fg_init_from_schemes(fg_ctx *ctx, const int32_t *blob, int64_t len) {
+    if (!ctx || !blob || len < 1) return fail("fg_init_from_schemes: bad arguments");
+    const int count = blob[0];
+    if (count < 1) return fail("fg_init_from_schemes: scheme count must be positive");
+    std::vector<fg::Scheme> starts;
+    int64_t pos = 1;
+    for (int q = 0; q < count && q < ctx->cfg.walkers; ++q) {
+        fg::Scheme s;
+        std::string err;
+        if (!fg::scheme_from_blob(ctx->cfg.ring, blob, len, &pos, &s, &err))
+            return fail("fg_init_from_schemes: scheme " + std::to_string(q) + ": " + err);
+        while (fg::try_reduce(&s)) {
+        }
+        if (!fg::validate(s)) return fail("fg_init_from_schemes: scheme " + std::to_string(q) + " fails the Brent check", -2);
+        starts.push_back(std::move(s));
+    }
+    const fg::Scheme &s0 = starts[0];
+    return init_pool(ctx, starts, s0.n[0], s0.n[1], s0.n[2]);
+}
+
+extern "C" int fg_walk(fg_ctx *ctx, int iterations) {
+    if (!ctx || !ctx->ready) return fail("fg_walk: context not initialised");
+    fgk::WalkArgs a{};
+    a.cur = ctx->cur;
+    a.best = ctx->best;
+    a.m = ctx->m;
+    a.best_m = ctx->best_m;
+    a.best_rank = ctx->best_rank;
+    a.flips_count = ctx->flips_count;
+    a.aux_idx = ctx->aux_idx;
+    a.it = ctx->it;
+    a.aux = ctx->aux;
+    a.total_flips = ctx->total_flips;
+    a.sum_m = ctx->sum_m;
+    a.fmt = ctx->fmt;
+    a.walkers = ctx->W;
+    a.max_m = ctx->maxm;
+    a.key0 = (uint32_t)ctx->cfg.seed;
+    a.key1 = (uint32_t)(ctx->cfg.seed >> 32);
+    a.iterations = iterations > 0 ? iterations : ctx->cfg.max_iterations;
+    a.max_iterations = iterations > 0 ? iterations : ctx->cfg.max_iterations;
+    a.random_iterations = ctx->cfg.random_iterations;
+    a.plus_iterations = ctx->cfg.plus_iterations;
+    a.use_reduce = ctx->cfg.reduce_probability > 0;
+    a.thr_reduce = threshold(ctx->cfg.reduce_probability, ctx->cfg.max_iterations);
+    a.use_expand = ctx->cfg.expand_probability > 0;
+    a.thr_expand = threshold(ctx->cfg.expand_probability, ctx->cfg.max_iterations);
+    a.thr_best_eq = kThrBestEq;
+    a.check_reduce = ctx->cfg.check_reduce;
+    a.zt_visit_cap = ctx->cfg.zt_visit_cap;
+    a.err = ctx->err;
+    HIPCHK(hipMemsetAsync(ctx->err, 0, sizeof(int32_t), ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(fgk::launch_walk_z2(a, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    int rc = verify_planes(ctx, ctx->cur, ctx->m, ctx->fmt, ctx->W, ctx->maxm, ctx->ok);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
+    int32_t err = 0;
+    HIPCHK(hipMemcpy(&err, ctx->err, sizeof err, hipMemcpyDeviceToHost));
+    if (err) return fail("fg_walk: kernel invariant violation, flags " + std::to_string(err), -3);
+    std::vector<int32_t> ok(ctx->W);
+    HIPCHK(hipMemcpy(ok.data(), ctx->ok, ok.size() * 4, hipMemcpyDeviceToHost));
+    int bad = 0;
+    for (int x : ok) bad += x == 0;
+    return bad;
+}
+
+extern "C" int fg_last_walk_ms(fg_ctx *ctx, float *ms) {
+    if (!ctx || !ms) return fail("fg_last_walk_ms: null argument");
+    *ms = ctx->last_ms;
+    return 0;
+}
+
+extern "C" int fg_counters(fg_ctx *ctx, uint64_t *flips, uint64_t *sum_m, uint64_t *iterations) {
+    if (!ctx || !ctx->ready) return fail("fg_counters: context not initialised");
+    std::vector<uint64_t> a(ctx->W), b(ctx->W), c(ctx->W);
+    HIPCHK(hipMemcpy(a.data(), ctx->total_flips, ctx->W * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(b.data(), ctx->sum_m, ctx->W * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(c.data(), ctx->it, ctx->W * 8, hipMemcpyDeviceToHost));
+    uint64_t sa = 0, sb = 0, sc = 0;
+    for (int w = 0; w < ctx->W; ++w) {
+        sa += a[w];
+        sb += b[w];
+        sc += c[w];
+    }
+    if (flips) *flips = sa;
+    if (sum_m) *sum_m = sb;
+    if (iterations) *iterations = sc;
+    return 0;
+}
+
+extern "C" int fg_read_walkers(fg_ctx *ctx, int begin, int count, fg_walker_record *out) {
+    if (!ctx || !ctx->ready || !out) return fail("fg_read_walkers: bad arguments");
+    if (begin < 0 || count < 0 || begin + count > ctx->W) return fail("fg_read_walkers: range out of bounds");
+    if (ctx->maxm > FG_REC_MAX_M) return fail("fg_read_walkers: scheme capacity exceeds the record");
+    const int W = ctx->W, maxm = ctx->maxm;
+    const size_t plane = (size_t)3 * maxm * W;
+    std::vector<uint32_t> cur(plane), best(plane);
+    std::vector<int32_t> m(W), bm(W), br(W), fc(W), ai(W), fmt((size_t)4 * W);
+    std::vector<uint64_t> it(W), aux(W), tf(W), sm(W);
+    HIPCHK(hipMemcpy(cur.data(), ctx->cur, plane * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(best.data(), ctx->best, plane * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(m.data(), ctx->m, W * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(bm.data(), ctx->best_m, W * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(br.data(), ctx->best_rank, W * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fc.data(), ctx->flips_count, W * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ai.data(), ctx->aux_idx, W * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fmt.data(), ctx->fmt, (size_t)16 * W, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(it.data(), ctx->it, W * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(aux.data(), ctx->aux, W * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tf.data(), ctx->total_flips, W * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sm.data(), ctx->sum_m, W * 8, hipMemcpyDeviceToHost));
+    for (int q = 0; q < count; ++q) {
+        const int w = begin + q;
+        fg_walker_record &r = out[q];
+        memset(&r, 0, sizeof r);
+        for (int c = 0; c < 3; ++c) r.n[c] = fmt[4 * w + c];
+        r.m = m[w];
+        r.best_m = bm[w];
+        r.best_rank = br[w];
+        r.flips_count = fc[w];
+        r.aux_idx = ai[w];
+        r.it = it[w];
+        r.aux = aux[w];
+        r.total_flips = tf[w];
+        r.sum_m = sm[w];
+        for (int c = 0; c < 3; ++c) {
+            for (int t = 0; t < r.m; ++t) r.cur_v[c][t] = cur[((size_t)c * maxm + t) * W + w];
+            for (int t = 0; t < r.best_m; ++t) r.best_v[c][t] = best[((size_t)c * maxm + t) * W + w];
+        }
+    }
+    return 0;
+}
+
+extern "C" int fg_write_walkers(fg_ctx *ctx, int begin, int count, const fg_walker_record *in) {
+    if (!ctx || !ctx->ready || !in) return fail("fg_write_walkers: bad arguments");
+    if (begin < 0 || count < 0 || begin + count > ctx->W) return fail("fg_write_walkers: range out of bounds");
+    const int W = ctx->W, maxm = ctx->maxm;
+    for (int q = 0; q < count; ++q) {
+        const fg_walker_record &r = in[q];
+        if (r.m < 0 || r.m > maxm || r.best_m < 0 || r.best_m > maxm)
+            return fail("fg_write_walkers: record rank exceeds the pool capacity");
+        if (r.n[0] * r.n[1] * r.n[2] > maxm) return fail("fg_write_walkers: format exceeds the pool capacity");
+        for (int c = 0; c < 3; ++c)
+            if (r.n[c] * r.n[(c + 1) % 3] > 32) return fail("fg_write_walkers: factor wider than 32 coefficients");
+    }
+    const size_t plane = (size_t)3 * maxm * W;
+    std::vector<uint32_t> cur(plane), best(plane);
+    std::vector<int32_t> m(W), bm(W), br(W), fc(W), ai(W), fmt((size_t)4 * W);
+    std::vector<uint64_t> it(W), aux(W), tf(W), sm(W);
+    HIPCHK(hipMemcpy(cur.data(), ctx->cur, plane * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(best.data(), ctx->best, plane * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(m.data(), ctx->m, W * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(bm.data(), ctx->best_m, W * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(br.data(), ctx->best_rank, W * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fc.data(), ctx->flips_count, W * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ai.data(), ctx->aux_idx, W * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fmt.data(), ctx->fmt, (size_t)16 * W, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(it.data(), ctx->it, W * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(aux.data(), ctx->aux, W * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tf.data(), ctx->total_flips, W * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sm.data(), ctx->sum_m, W * 8, hipMemcpyDeviceToHost));
+    for (int q = 0; q < count; ++q) {
+        const int w = begin + q;
+        const fg_walker_record &r = in[q];
+        for (int c = 0; c < 3; ++c) fmt[4 * w + c] = r.n[c];
+        m[w] = r.m;
+        bm[w] = r.best_m;
+        br[w] = r.best_rank;
+        fc[w] = r.flips_count;
+        ai[w] = r.aux_idx;
+        it[w] = r.it;
+        aux[w] = r.aux;
+        tf[w] = r.total_flips;
+        sm[w] = r.sum_m;
+        for (int c = 0; c < 3; ++c) {
+            for (int t = 0; t < r.m; ++t) cur[((size_t)c * maxm + t) * W + w] = (uint32_t)r.cur_v[c][t];
+            for (int t = 0; t < r.best_m; ++t) best[((size_t)c * maxm + t) * W + w] = (uint32_t)r.best_v[c][t];
+        }
+    }
+    HIPCHK(hipMemcpy(ctx->cur, cur.data(), plane * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->best, best.data(), plane * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->m, m.data(), W * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->best_m, bm.data(), W * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->best_rank, br.data(), W * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->flips_count, fc.data(), W * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->aux_idx, ai.data(), W * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->fmt, fmt.data(), (size_t)16 * W, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->it, it.data(), W * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->aux, aux.data(), W * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->total_flips, tf.data(), W * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->sum_m, sm.data(), W * 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
+namespace {
+
+int walker_scheme(fg_ctx *ctx, int w, int which, fg::Scheme *out) {
+    const int W = ctx->W, maxm = ctx->maxm;
+    int32_t m = 0, fmt[4];
+    HIPCHK(hipMemcpy(&m, (which ? ctx->best_m : ctx->m) + w, 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fmt, ctx->fmt + 4 * w, 16, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> col((size_t)3 * maxm);
+    // one strided column of the plane: 3 * maxm words, pitch W words
+    HIPCHK(hipMemcpy2D(col.data(), 4, (which ? ctx->best : ctx->cur) + w, (size_t)W * 4, 4, (size_t)3 * maxm,
+                       hipMemcpyDeviceToHost));
+    fg::Scheme s;
+    s.ring = ctx->cfg.ring;
+    for (int c = 0; c < 3; ++c) s.n[c] = fmt[c];
+    for (int c = 0; c < 3; ++c) s.nn[c] = s.n[c] * s.n[(c + 1) % 3];
+    s.m = m;
+    s.resize(m);
+    for (int c = 0; c < 3; ++c)
+        for (int t = 0; t < m; ++t) s.v[c][t] = col[(size_t)c * maxm + t];
+    *out = std::move(s);
+    return 0;
+}
+
+int emit_blob(const fg::Scheme &s, int32_t *out, int64_t cap) {
+    std::vector<int32_t> b;
+    fg::scheme_to_blob(s, &b);
+    if ((int64_t)b.size() > cap) return fail("output blob capacity too small");
+    std::copy(b.begin(), b.end(), out);
+    return (int)b.size();
+}
+
+}  // namespace
+
+extern "C" int fg_walker_scheme(fg_ctx *ctx, int walker, int which, int32_t *out_blob, int64_t cap) {
+    if (!ctx || !ctx->ready || !out_blob) return fail("fg_walker_scheme: bad arguments");
+    if (walker < 0 || walker >= ctx->W) return fail("fg_walker_scheme: walker out of range");
+    fg::Scheme s;
+    int rc = walker_scheme(ctx, walker, which, &s);
+    if (rc) return rc;
+    return emit_blob(s, out_blob, cap);
+}
+
+extern "C" int fg_best(fg_ctx *ctx, int32_t *out_blob, int64_t cap, int32_t *out_rank, int32_t *out_walker) {
+    if (!ctx || !ctx->ready) return fail("fg_best: context not initialised");
+    std::vector<int32_t> bm(ctx->W);
+    HIPCHK(hipMemcpy(bm.data(), ctx->best_m, ctx->W * 4, hipMemcpyDeviceToHost));
+    int bw = 0;
+    for (int w = 1; w < ctx->W; ++w)
+        if (bm[w] < bm[bw]) bw = w;
+    if (out_rank) *out_rank = bm[bw];
+    if (out_walker) *out_walker = bw;
+    if (!out_blob) return 0;
+    fg::Scheme s;
+    int rc = walker_scheme(ctx, bw, 1, &s);
+    if (rc) return rc;
+    return emit_blob(s, out_blob, cap);
+}
+
+extern "C" int fg_verify(int ring, const int32_t *blob, int64_t len, int32_t *out_ok) {
+    if (!blob || len < 1 || !out_ok) return fail("fg_verify: bad arguments");
+    if (ring != FG_RING_Z2) return fail("fg_verify: only the Z2 check runs on the GPU in this build");
+    const int count = blob[0];
+    if (count < 0) return fail("fg_verify: negative count");
+    std::vector<fg::Scheme> ss(count);
+    int64_t pos = 1;
+    int maxm = 1;
+    for (int q = 0; q < count; ++q) {
+        std::string err;
+        if (!fg::scheme_from_blob(ring, blob, len, &pos, &ss[q], &err)) return fail("fg_verify: " + err);
+        for (int c = 0; c < 3; ++c)
+            if (ss[q].nn[c] > 32) return fail("fg_verify: factor wider than 32 coefficients");
+        maxm = std::max(maxm, ss[q].m);
+    }
+    if (count == 0) return 0;
+    std::vector<uint32_t> words((size_t)3 * maxm * count, 0);
+    std::vector<int32_t> m(count), fmt((size_t)4 * count, 0);
+    for (int q = 0; q < count; ++q) {
+        m[q] = ss[q].m;
+        for (int c = 0; c < 3; ++c) fmt[4 * q + c] = ss[q].n[c];
+        for (int c = 0; c < 3; ++c)
+            for (int t = 0; t < ss[q].m; ++t) words[((size_t)c * maxm + t) * count + q] = (uint32_t)ss[q].v[c][t];
+    }
+    uint32_t *dw = nullptr;
+    int32_t *dm = nullptr, *df = nullptr, *dok = nullptr;
+    hipStream_t st = nullptr;
+    int rc = 0;
+    auto cleanup = [&]() {
+        if (dw) (void)hipFree(dw);
+        if (dm) (void)hipFree(dm);
+        if (df) (void)hipFree(df);
+        if (dok) (void)hipFree(dok);
+        if (st) (void)hipStreamDestroy(st);
+    };
+    if (dalloc(&dw, words.size()) != hipSuccess || dalloc(&dm, count) != hipSuccess ||
+        dalloc(&df, (size_t)4 * count) != hipSuccess || dalloc(&dok, count) != hipSuccess ||
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        cleanup();
+        return fail("fg_verify: device allocation failed");
+    }
+    fg_ctx tmp;
+    tmp.stream = st;
+    if (hipMemcpy(dw, words.data(), words.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dm, m.data(), (size_t)count * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(df, fmt.data(), fmt.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        cleanup();
+        return fail("fg_verify: copy failed");
+    }
+    rc = verify_planes(&tmp, dw, dm, df, count, maxm, dok);
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail("fg_verify: kernel failed");
+    if (!rc && hipMemcpy(out_ok, dok, (size_t)count * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = fail("fg_verify: copy back failed");
+    tmp.stream = nullptr;
+    cleanup();
+    if (rc) return rc;
+    int good = 0;
+    for (int q = 0; q < count; ++q) good += out_ok[q] != 0;
+    return good;
+}

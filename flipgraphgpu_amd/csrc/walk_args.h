@@ -1,0 +1,75 @@
+// Kernel argument block shared by the host runtime and the HIP walk kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace fgk {
+
+// Walkers per workgroup: one wave64, one walker per lane.
+constexpr int kWalkersPerGroup = 64;
+
+// Device-resident walker pool.  Factor planes are structure-of-arrays,
+// word(c, t, w) = words[(c * max_m + t) * walkers + w], so a wave touching
+// term t of 64 consecutive walkers reads 64 consecutive words.
+struct WalkArgs {
+    uint32_t *cur;        // Z2 factor masks / ZT coefficient magnitudes
+    uint32_t *cur_s;      // ZT signs (null for Z2)
+    uint32_t *best;
+    uint32_t *best_s;
+    int32_t *m;
+    int32_t *best_m;
+    int32_t *best_rank;
+    int32_t *flips_count;
+    int32_t *aux_idx;
+    uint64_t *it;
+    uint64_t *aux;
+    uint64_t *total_flips;
+    uint64_t *sum_m;
+    const int32_t *fmt;   // [walkers][4]: n1 n2 n3 unused
+    int32_t walkers;
+    int32_t max_m;
+    uint32_t key0;
+    uint32_t key1;
+    int32_t iterations;
+    int32_t max_iterations;
+    int32_t random_iterations;
+    int32_t plus_iterations;
+    int32_t use_reduce;
+    uint32_t thr_reduce;
+    int32_t use_expand;
+    uint32_t thr_expand;
+    uint32_t thr_best_eq;
+    int32_t check_reduce;
+    int32_t zt_visit_cap;
+    int32_t *err;         // invariant violations (bit flags), host checks after each launch
+};
+
+// Error bits written by the walk kernels.
+constexpr int32_t kErrSelect = 1;   // pair draw found no term (partner counts inconsistent)
+constexpr int32_t kErrWalk = 2;     // class list walk exceeded the term count
+constexpr int32_t kErrCapacity = 4; // a term would exceed the LDS capacity
+
+// Terms per packed H word and the padding of the H plane (a multiple of 16
+// terms so the selection scan reads whole 4-word chunks).
+constexpr int kHTermsPad = 16;
+__host__ __device__ inline int h_words(int max_m) { return ((max_m + kHTermsPad - 1) / kHTermsPad) * (kHTermsPad / 4); }
+
+// LDS bytes per workgroup for the Z2 kernel with 32-bit factor words.
+__host__ __device__ inline size_t z2_lds_bytes(int max_m) {
+    return (size_t)kWalkersPerGroup * (3 * (size_t)max_m * 4 + 3 * (size_t)h_words(max_m) * 4 + 3 * (size_t)max_m);
+}
+
+// Brent check of schemes held in the same plane layout.
+struct VerifyArgs {
+    const uint32_t *words;
+    const uint32_t *signs;  // null for Z2
+    const int32_t *m;
+    const int32_t *fmt;     // [count][4]
+    int32_t count;
+    int32_t max_m;
+    int32_t *ok;            // [count], host sets 1, kernel clears on failure
+};
+
+}  // namespace fgk

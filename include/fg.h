@@ -1,0 +1,119 @@
+/*
+ * fg.h -- C ABI of the MI355X flip-graph search engine (libflipgraph.so).
+ *
+ * The reference (dronperminov/FlipGraphGPU) has no library API: its
+ * boundary is the flip_graph / complexity_minimizer / additions_reducer
+ * processes, whose host classes own managed walker pools and launch
+ * one-thread-per-walker kernels.  These entry points replace those host
+ * classes and kernels; the CLIs in flipgraphgpu_amd/csrc call only this
+ * ABI.  Conventions: int return codes (0 = ok, < 0 = error, message via
+ * fg_last_error()), never exit(); the context owns device memory, callers
+ * own every buffer passed in; one host thread and one HIP stream per
+ * context.
+ */
+#ifndef FG_H
+#define FG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FG_RING_Z2 0 /* src/config.cuh:43-49, SchemeZ2 */
+#define FG_RING_ZT 1 /* SchemeInteger, coefficients in {-1,0,1} */
+
+#define FG_REC_MAX_M 256 /* term capacity of fg_walker_record */
+
+typedef struct fg_ctx fg_ctx;
+
+/* Flags of src/main_flip_graph.cu:10-25 that reach the walk. */
+typedef struct {
+    int32_t ring;               /* FG_RING_Z2 / FG_RING_ZT (compile-time in the reference, src/config.cuh:6) */
+    int32_t walkers;            /* --schemes */
+    int32_t max_iterations;     /* --max-iterations */
+    int32_t plus_iterations;    /* --plus-iterations */
+    int32_t random_iterations;  /* 1 when --resize-probability > 0 (src/entities/flip_graph.cu:530) */
+    int32_t check_reduce;       /* tryFlip(checkReduce): 1 flip_graph, 0 complexity_minimizer */
+    double expand_probability;  /* --expand-probability (divided by max iterations) */
+    double reduce_probability;  /* --reduce-probability (divided by max iterations) */
+    uint64_t seed;              /* --seed (0 is replaced by time(0) in the CLI, not here) */
+    int32_t max_rank;           /* term capacity per walker; 0 = max(naive rank, input ranks) */
+    int32_t zt_visit_cap;       /* integer ring: candidates one tryFlip may visit (0 = 64) */
+} fg_config;
+
+/* Per-walker state, for tests and checkpointing. */
+typedef struct {
+    int32_t n[3];
+    int32_t m;
+    int32_t best_m;
+    int32_t best_rank;
+    int32_t flips_count; /* flips since the last rank drop (the reference's flips[idx]) */
+    int32_t aux_idx;
+    uint64_t it;          /* main RNG stream position = walk iterations done */
+    uint64_t aux;         /* aux RNG stream position */
+    uint64_t total_flips; /* applied flips */
+    uint64_t sum_m;       /* sum over applied flips of the rank at that flip */
+    uint64_t cur_v[3][FG_REC_MAX_M];  /* Z2 masks / ZT coefficient magnitudes */
+    uint64_t cur_s[3][FG_REC_MAX_M];  /* ZT signs (0 for Z2) */
+    uint64_t best_v[3][FG_REC_MAX_M];
+    uint64_t best_s[3][FG_REC_MAX_M];
+} fg_walker_record;
+
+const char *fg_last_error(void);
+
+/* FlipGraph::FlipGraph (src/entities/flip_graph.cu:4-188): allocate the
+ * walker pool and seed the per-walker RNG streams. */
+int fg_create(const fg_config *cfg, fg_ctx **out);
+/* FlipGraph::~FlipGraph (src/entities/flip_graph.cu:417-442). */
+int fg_destroy(fg_ctx *ctx);
+
+/* FlipGraph::initializeNaive + initialize (src/entities/flip_graph.cu:226-233,
+ * 190-199): every walker starts from the naive n1 x n2 x n3 scheme. */
+int fg_init_naive(fg_ctx *ctx, int n1, int n2, int n3);
+
+/* FlipGraph::initializeFromFile (src/entities/flip_graph.cu:201-224): blob is
+ * int32 [count, then count x (n1 n2 n3 m, U m*n1n2, V m*n2n3, W m*n3n1)] --
+ * the reference's --input-path text format as integers.  Each scheme is
+ * reduced with tryReduce until it no longer shrinks, then walkers are filled
+ * round-robin.  Returns -2 if a scheme fails the Brent check. */
+int fg_init_from_schemes(fg_ctx *ctx, const int32_t *blob, int64_t len);
+
+/* FlipGraph::randomWalk / randomWalkKernel (src/entities/flip_graph.cu:235-240,
+ * 520-573): one launch.  iterations <= 0 uses max_iterations (random in
+ * [1, max_iterations] per walker when random_iterations is set).  Runs the
+ * GPU Brent check of every walker's current scheme afterwards; returns the
+ * number of invalid schemes found (0 expected) or < 0 on error. */
+int fg_walk(fg_ctx *ctx, int iterations);
+
+/* Best scheme over all walkers (smallest best rank; lowest walker index
+ * among ties), as a single-scheme blob (no count prefix). */
+int fg_best(fg_ctx *ctx, int32_t *out_blob, int64_t cap, int32_t *out_rank, int32_t *out_walker);
+
+/* Blob of walker `walker`'s best (which=1) or current (which=0) scheme. */
+int fg_walker_scheme(fg_ctx *ctx, int walker, int which, int32_t *out_blob, int64_t cap);
+
+/* Scheme::validate (src/schemes/scheme_z2.cu:3-29, scheme_integer.cu:3-38) on
+ * the GPU for every scheme of a count-prefixed blob; out_ok[i] = 1 if valid.
+ * Returns the number of valid schemes or < 0. */
+int fg_verify(int ring, const int32_t *blob, int64_t len, int32_t *out_ok);
+
+/* Aggregate counters since fg_init_*: applied flips, sum of ranks at those
+ * flips, walk iterations. */
+int fg_counters(fg_ctx *ctx, uint64_t *flips, uint64_t *sum_m, uint64_t *iterations);
+
+/* Duration of the last fg_walk kernel launch, measured with HIP events on
+ * the context's stream. */
+int fg_last_walk_ms(fg_ctx *ctx, float *ms);
+
+/* Read / overwrite walker state (tests, checkpoints). */
+int fg_read_walkers(fg_ctx *ctx, int begin, int count, fg_walker_record *out);
+int fg_write_walkers(fg_ctx *ctx, int begin, int count, const fg_walker_record *in);
+
+/* Device properties used for roofline reporting. */
+int fg_device_info(int *cu_count, int *clock_khz, int *lds_per_block, char *name, int name_cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,65 @@
+// Host emulation of the Z2 walk kernel for the CPU test suite: runs the
+// kernel's own per-lane code (flipgraphgpu_amd/csrc/walk_z2_lane.h) one lane
+// at a time over a host LDS buffer.  Lanes of the kernel never communicate,
+// so this reproduces the GPU result exactly.  Test infrastructure only.
+#include <cstring>
+#include <vector>
+
+#include "fg.h"
+#include "walk_args.h"
+#include "walk_z2_lane.h"
+
+extern "C" int emu_walk_z2_records(fg_walker_record *recs, int count, uint64_t seed, int iterations,
+                                   int max_iterations, int random_iterations, int plus_iterations,
+                                   int use_reduce, uint32_t thr_reduce, int use_expand, uint32_t thr_expand,
+                                   uint32_t thr_best_eq, int check_reduce, int launches, int32_t *err_out) {
+    int maxm = 1;
+    for (int q = 0; q < count; ++q) {
+        maxm = std::max(maxm, std::max(recs[q].m, recs[q].best_m));
+        maxm = std::max(maxm, recs[q].n[0] * recs[q].n[1] * recs[q].n[2]);
+    }
+    const int W = count;
+    const size_t plane = (size_t)3 * maxm * W;
+    std::vector<uint32_t> cur(plane), best(plane);
+    std::vector<int32_t> m(W), bm(W), br(W), fc(W), ai(W), fmt((size_t)4 * W);
+    std::vector<uint64_t> it(W), aux(W), tf(W), sm(W);
+    for (int w = 0; w < W; ++w) {
+        const fg_walker_record &r = recs[w];
+        for (int c = 0; c < 3; ++c) fmt[4 * w + c] = r.n[c];
+        m[w] = r.m; bm[w] = r.best_m; br[w] = r.best_rank; fc[w] = r.flips_count; ai[w] = r.aux_idx;
+        it[w] = r.it; aux[w] = r.aux; tf[w] = r.total_flips; sm[w] = r.sum_m;
+        for (int c = 0; c < 3; ++c) {
+            for (int t = 0; t < r.m; ++t) cur[((size_t)c * maxm + t) * W + w] = (uint32_t)r.cur_v[c][t];
+            for (int t = 0; t < r.best_m; ++t) best[((size_t)c * maxm + t) * W + w] = (uint32_t)r.best_v[c][t];
+        }
+    }
+    int32_t err = 0;
+    fgk::WalkArgs a{};
+    a.cur = cur.data(); a.best = best.data(); a.m = m.data(); a.best_m = bm.data(); a.best_rank = br.data();
+    a.flips_count = fc.data(); a.aux_idx = ai.data(); a.it = it.data(); a.aux = aux.data();
+    a.total_flips = tf.data(); a.sum_m = sm.data(); a.fmt = fmt.data();
+    a.walkers = W; a.max_m = maxm; a.key0 = (uint32_t)seed; a.key1 = (uint32_t)(seed >> 32);
+    a.iterations = iterations; a.max_iterations = max_iterations; a.random_iterations = random_iterations;
+    a.plus_iterations = plus_iterations; a.use_reduce = use_reduce; a.thr_reduce = thr_reduce;
+    a.use_expand = use_expand; a.thr_expand = thr_expand; a.thr_best_eq = thr_best_eq;
+    a.check_reduce = check_reduce; a.zt_visit_cap = 64; a.err = &err;
+    std::vector<uint8_t> lds(fgk::z2_lds_bytes(maxm), 0xCD);
+    const int G = (W + fgk::kWalkersPerGroup - 1) / fgk::kWalkersPerGroup;
+    for (int l = 0; l < launches; ++l)
+        for (int g = 0; g < G; ++g)
+            for (int lane = 0; lane < fgk::kWalkersPerGroup; ++lane)
+                fgk::z2::walk_lane(a, lds.data(), lane, g * fgk::kWalkersPerGroup + lane);
+    for (int w = 0; w < W; ++w) {
+        fg_walker_record &r = recs[w];
+        std::memset(r.cur_v, 0, sizeof r.cur_v);
+        std::memset(r.best_v, 0, sizeof r.best_v);
+        r.m = m[w]; r.best_m = bm[w]; r.best_rank = br[w]; r.flips_count = fc[w]; r.aux_idx = ai[w];
+        r.it = it[w]; r.aux = aux[w]; r.total_flips = tf[w]; r.sum_m = sm[w];
+        for (int c = 0; c < 3; ++c) {
+            for (int t = 0; t < r.m; ++t) r.cur_v[c][t] = cur[((size_t)c * maxm + t) * W + w];
+            for (int t = 0; t < r.best_m; ++t) r.best_v[c][t] = best[((size_t)c * maxm + t) * W + w];
+        }
+    }
+    *err_out = err;
+    return 0;
+}

@@ -1,0 +1,36 @@
+"""ctypes access to the host emulation of the walk kernels (tests/emu)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+import oracle_ffi as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "emu", "libemu.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        subprocess.run(["make", "-s", "-C", os.path.join(HERE, "emu")], check=True)
+        L = ctypes.CDLL(LIB)
+        L.emu_walk_z2_records.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64] + [ctypes.c_int] * 4 + \
+            [ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
+             ctypes.POINTER(ctypes.c_int32)]
+        _lib = L
+    return _lib
+
+
+def walk_z2(recs, params, seed, launches=1):
+    recs = np.ascontiguousarray(recs.copy())
+    err = ctypes.c_int32()
+    p = params
+    rc = lib().emu_walk_z2_records(recs.ctypes.data, len(recs), seed, p.iterations, p.max_iterations,
+                                   p.random_iterations, p.plus_iterations, p.use_reduce, p.thr_reduce,
+                                   p.use_expand, p.thr_expand, p.thr_best_eq, p.check_reduce, launches,
+                                   ctypes.byref(err))
+    assert rc == 0
+    return recs, err.value

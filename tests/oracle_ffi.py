@@ -162,7 +162,7 @@ def naive_records(n1, n2, n3, count, ring=RING_Z2):
 
 
 def walk_records(recs, params, seed, ring=RING_Z2, walker_base=0, launches=1, nthreads=1):
-    recs = np.ascontiguousarray(recs)
+    recs = np.array(recs, dtype=RECORD_DTYPE, copy=True, order="C")
     rc = lib().fgo_walk_records(ring, recs.ctypes.data, len(recs), walker_base, seed,
                                 ctypes.byref(params), launches, nthreads)
     assert rc == 0

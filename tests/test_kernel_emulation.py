@@ -1,0 +1,46 @@
+"""The walk kernel's own lane code, run on the host (tests/emu), must match
+the oracle bit-for-bit.  This is the CPU-side gate before any GPU run: it
+exercises the partner-class bookkeeping, selection and rare paths of the
+exact source the GPU compiles."""
+import pytest
+
+import emu_ffi as E
+import oracle_ffi as O
+from walk_compare import first_mismatch
+
+
+def _check(n, walkers, iterations, seed, launches=1, plus=8000, p_expand=0.0, random_iterations=False,
+           check_reduce=True):
+    start = O.naive_records(*n, walkers)
+    p = O.make_params(iterations=iterations, plus_iterations=plus, p_expand=p_expand,
+                      random_iterations=random_iterations, check_reduce=check_reduce)
+    emu, err = E.walk_z2(start, p, seed, launches=launches)
+    ora = O.walk_records(start, p, seed, launches=launches, nthreads=8)
+    assert err == 0
+    mism = first_mismatch(emu, ora)
+    assert mism is None, mism
+    return ora
+
+
+@pytest.mark.parametrize("n,walkers,iterations", [
+    ((2, 2, 2), 64, 2000), ((2, 2, 3), 64, 2000), ((3, 3, 3), 64, 3000), ((2, 3, 4), 64, 2000),
+    ((4, 4, 4), 64, 1500), ((3, 4, 5), 32, 1000),
+])
+def test_emulated_kernel_matches_oracle(n, walkers, iterations):
+    ora = _check(n, walkers, iterations, seed=12345)
+    assert (ora["total_flips"] > 0).all()
+
+
+@pytest.mark.parametrize("n", [(2, 2, 2), (3, 3, 3), (3, 3, 4)])
+def test_emulated_kernel_plus_expand_paths(n):
+    _check(n, 64, 2000, seed=7, launches=2, plus=150, p_expand=200.0)
+
+
+def test_emulated_kernel_random_iterations():
+    _check((3, 3, 3), 64, 1200, seed=99, launches=3, random_iterations=True, p_expand=20.0)
+
+
+def test_emulated_kernel_without_reductions():
+    # complexity-minimizer mode: tryFlip(checkReduce = false)
+    ora = _check((3, 3, 3), 64, 2000, seed=3, check_reduce=False)
+    assert (ora["m"] >= 27).all() or (ora["m"] < 27).any()
