@@ -325,6 +325,34 @@ extern "C" int fg_counters(fg_ctx *ctx, uint64_t *flips, uint64_t *sum_m, uint64
     return 0;
 }
 
+extern "C" int fg_summary(fg_ctx *ctx, int32_t *m, int32_t *best_m, int32_t *best_rank, int32_t *flips_count,
+                          int32_t *n) {
+    if (!ctx || !ctx->ready) return fail("fg_summary: context not initialised");
+    const size_t bytes = (size_t)ctx->W * 4;
+    if (n) {
+        std::vector<int32_t> fmt((size_t)4 * ctx->W);
+        HIPCHK(hipMemcpy(fmt.data(), ctx->fmt, fmt.size() * 4, hipMemcpyDeviceToHost));
+        for (int w = 0; w < ctx->W; ++w)
+            for (int c = 0; c < 3; ++c) n[3 * w + c] = fmt[4 * w + c];
+    }
+    if (m) HIPCHK(hipMemcpy(m, ctx->m, bytes, hipMemcpyDeviceToHost));
+    if (best_m) HIPCHK(hipMemcpy(best_m, ctx->best_m, bytes, hipMemcpyDeviceToHost));
+    if (best_rank) HIPCHK(hipMemcpy(best_rank, ctx->best_rank, bytes, hipMemcpyDeviceToHost));
+    if (flips_count) HIPCHK(hipMemcpy(flips_count, ctx->flips_count, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int fg_shape(fg_ctx *ctx, int32_t *walkers, int32_t *n1, int32_t *n2, int32_t *n3) {
+    if (!ctx || !ctx->ready) return fail("fg_shape: context not initialised");
+    int32_t fmt[4];
+    HIPCHK(hipMemcpy(fmt, ctx->fmt, 16, hipMemcpyDeviceToHost));
+    if (walkers) *walkers = ctx->W;
+    if (n1) *n1 = fmt[0];
+    if (n2) *n2 = fmt[1];
+    if (n3) *n3 = fmt[2];
+    return 0;
+}
+
 extern "C" int fg_read_walkers(fg_ctx *ctx, int begin, int count, fg_walker_record *out) {
     if (!ctx || !ctx->ready || !out) return fail("fg_read_walkers: bad arguments");
     if (begin < 0 || count < 0 || begin + count > ctx->W) return fail("fg_read_walkers: range out of bounds");

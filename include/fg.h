@@ -20,6 +20,8 @@
 extern "C" {
 #endif
 
+#define FG_API __attribute__((visibility("default")))
+
 #define FG_RING_Z2 0 /* src/config.cuh:43-49, SchemeZ2 */
 #define FG_RING_ZT 1 /* SchemeInteger, coefficients in {-1,0,1} */
 
@@ -60,58 +62,67 @@ typedef struct {
     uint64_t best_s[3][FG_REC_MAX_M];
 } fg_walker_record;
 
-const char *fg_last_error(void);
+FG_API const char *fg_last_error(void);
 
 /* FlipGraph::FlipGraph (src/entities/flip_graph.cu:4-188): allocate the
  * walker pool and seed the per-walker RNG streams. */
-int fg_create(const fg_config *cfg, fg_ctx **out);
+FG_API int fg_create(const fg_config *cfg, fg_ctx **out);
 /* FlipGraph::~FlipGraph (src/entities/flip_graph.cu:417-442). */
-int fg_destroy(fg_ctx *ctx);
+FG_API int fg_destroy(fg_ctx *ctx);
 
 /* FlipGraph::initializeNaive + initialize (src/entities/flip_graph.cu:226-233,
  * 190-199): every walker starts from the naive n1 x n2 x n3 scheme. */
-int fg_init_naive(fg_ctx *ctx, int n1, int n2, int n3);
+FG_API int fg_init_naive(fg_ctx *ctx, int n1, int n2, int n3);
 
 /* FlipGraph::initializeFromFile (src/entities/flip_graph.cu:201-224): blob is
  * int32 [count, then count x (n1 n2 n3 m, U m*n1n2, V m*n2n3, W m*n3n1)] --
  * the reference's --input-path text format as integers.  Each scheme is
  * reduced with tryReduce until it no longer shrinks, then walkers are filled
  * round-robin.  Returns -2 if a scheme fails the Brent check. */
-int fg_init_from_schemes(fg_ctx *ctx, const int32_t *blob, int64_t len);
+FG_API int fg_init_from_schemes(fg_ctx *ctx, const int32_t *blob, int64_t len);
 
 /* FlipGraph::randomWalk / randomWalkKernel (src/entities/flip_graph.cu:235-240,
  * 520-573): one launch.  iterations <= 0 uses max_iterations (random in
  * [1, max_iterations] per walker when random_iterations is set).  Runs the
  * GPU Brent check of every walker's current scheme afterwards; returns the
  * number of invalid schemes found (0 expected) or < 0 on error. */
-int fg_walk(fg_ctx *ctx, int iterations);
+FG_API int fg_walk(fg_ctx *ctx, int iterations);
 
 /* Best scheme over all walkers (smallest best rank; lowest walker index
  * among ties), as a single-scheme blob (no count prefix). */
-int fg_best(fg_ctx *ctx, int32_t *out_blob, int64_t cap, int32_t *out_rank, int32_t *out_walker);
+FG_API int fg_best(fg_ctx *ctx, int32_t *out_blob, int64_t cap, int32_t *out_rank, int32_t *out_walker);
 
 /* Blob of walker `walker`'s best (which=1) or current (which=0) scheme. */
-int fg_walker_scheme(fg_ctx *ctx, int walker, int which, int32_t *out_blob, int64_t cap);
+FG_API int fg_walker_scheme(fg_ctx *ctx, int walker, int which, int32_t *out_blob, int64_t cap);
 
 /* Scheme::validate (src/schemes/scheme_z2.cu:3-29, scheme_integer.cu:3-38) on
  * the GPU for every scheme of a count-prefixed blob; out_ok[i] = 1 if valid.
  * Returns the number of valid schemes or < 0. */
-int fg_verify(int ring, const int32_t *blob, int64_t len, int32_t *out_ok);
+FG_API int fg_verify(int ring, const int32_t *blob, int64_t len, int32_t *out_ok);
 
 /* Aggregate counters since fg_init_*: applied flips, sum of ranks at those
  * flips, walk iterations. */
-int fg_counters(fg_ctx *ctx, uint64_t *flips, uint64_t *sum_m, uint64_t *iterations);
+FG_API int fg_counters(fg_ctx *ctx, uint64_t *flips, uint64_t *sum_m, uint64_t *iterations);
 
 /* Duration of the last fg_walk kernel launch, measured with HIP events on
  * the context's stream. */
-int fg_last_walk_ms(fg_ctx *ctx, float *ms);
+FG_API int fg_last_walk_ms(fg_ctx *ctx, float *ms);
+
+/* Per-walker summary for reports (FlipGraph::report, src/entities/flip_graph.cu:303-380):
+ * current rank, best-scheme rank, bestRanks[], flips-since-last-drop and the
+ * format (3 values per walker).  Any output pointer may be null; each
+ * non-null one holds `walkers` values (3 * walkers for n). */
+FG_API int fg_summary(fg_ctx *ctx, int32_t *m, int32_t *best_m, int32_t *best_rank, int32_t *flips_count, int32_t *n);
+
+/* Number of walkers and the format of walker 0. */
+FG_API int fg_shape(fg_ctx *ctx, int32_t *walkers, int32_t *n1, int32_t *n2, int32_t *n3);
 
 /* Read / overwrite walker state (tests, checkpoints). */
-int fg_read_walkers(fg_ctx *ctx, int begin, int count, fg_walker_record *out);
-int fg_write_walkers(fg_ctx *ctx, int begin, int count, const fg_walker_record *in);
+FG_API int fg_read_walkers(fg_ctx *ctx, int begin, int count, fg_walker_record *out);
+FG_API int fg_write_walkers(fg_ctx *ctx, int begin, int count, const fg_walker_record *in);
 
 /* Device properties used for roofline reporting. */
-int fg_device_info(int *cu_count, int *clock_khz, int *lds_per_block, char *name, int name_cap);
+FG_API int fg_device_info(int *cu_count, int *clock_khz, int *lds_per_block, char *name, int name_cap);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,7 @@ def declared_functions():
     for h in os.listdir(os.path.join(ROOT, "include")):
         if h.endswith(".h"):
             text = open(os.path.join(ROOT, "include", h)).read()
-            names += re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(fg_\w+)\s*\(", text, re.M)
+            names += re.findall(r"^\s*(?:FG_API\s+)?(?:const\s+)?\w+\s*\*?\s*(fg_\w+)\s*\(", text, re.M)
     return sorted(set(names))
 
 
