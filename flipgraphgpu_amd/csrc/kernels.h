@@ -1,4 +1,4 @@
-// Host launchers of the HIP kernels (defined in the .hip files).
+// Host launchers of the HIP kernels (defined in walk.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,7 +7,8 @@
 
 namespace fgk {
 
-hipError_t launch_walk_z2(const WalkArgs &a, hipStream_t stream);
-hipError_t launch_verify_z2(const VerifyArgs &v, hipStream_t stream);
+// ring 0 = Z2, 1 = integer ring with packed ternary words
+hipError_t launch_walk(int ring, const WalkArgs &a, hipStream_t stream);
+hipError_t launch_verify(int ring, const VerifyArgs &v, hipStream_t stream);
 
 }  // namespace fgk

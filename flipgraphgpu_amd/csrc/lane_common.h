@@ -1,0 +1,419 @@
+// Ring-independent per-lane machinery of the walk kernels, shared by the
+// HIP kernels and the host emulation used by the tests.  Everything here is
+// __host__ __device__; lanes never communicate, so running lanes one after
+// another on the host reproduces a kernel exactly.
+//
+// One lane owns one walker (the reference's randomWalkKernel also runs one
+// thread per scheme, src/entities/flip_graph.cu:520-573).  The walker's
+// scheme lives in LDS for the whole launch as three factor planes
+// F[c][t][lane] (one 32-bit word per factor: a Z2 mask, or for the integer
+// ring the coefficient magnitudes in bits 0-15 and their signs in bits
+// 16-31); two byte planes describe the flip partners:
+//
+//   N[c][t]  next member (cyclic, ascending term index) of the class of terms
+//            whose component-c factors are equal to term t's,
+//   H[c][t]  number of class members with an index above t.
+//
+// The reference keeps the partner pairs as an explicit list per component
+// (FlipSet, src/entities/flip_set.cu) and rebuilds or scans it on every
+// flip.  Here the unordered pair (t1 < t2) at position r of the fresh-list
+// order (component, t1, t2 -- the order initFlips() produces,
+// src/schemes/scheme_z2.cu:141-150) is found by a prefix scan of H followed
+// by r' + 1 steps along N, and a flip updates only the two classes it
+// touches.  Rare events (reductions, zero factors, plus / split) rebuild
+// N and H from scratch.
+//
+// Randomness: Philox4x32-10 keyed by the seed, counter (iteration, walker,
+// stream).  One block per walk iteration supplies the pair draw, the two
+// orientation coins, the 1% equal-rank best-copy draw and the expand
+// draw; rare operators use the aux stream.  The CPU oracle consumes the
+// same stream in the same order, so a walker's trajectory is bit-identical.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "walk_args.h"
+
+#define FGK_HD __host__ __device__ __forceinline__
+
+namespace fgk {
+
+FGK_HD void fgk_atomic_or(int32_t *p, int32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicOr(p, v);
+#else
+    *p |= v;
+#endif
+}
+
+// Sum of the four bytes of w (v_sad_u8 on the GPU).
+FGK_HD uint32_t byte_sum(uint32_t w) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sad_u8(w, 0u, 0u);
+#else
+    return (w & 0xFFu) + ((w >> 8) & 0xFFu) + ((w >> 16) & 0xFFu) + (w >> 24);
+#endif
+}
+
+namespace lane {
+
+constexpr int WG = kWalkersPerGroup;
+constexpr uint32_t kPM0 = 0xD2511F53u, kPM1 = 0xCD9E8D57u;
+constexpr uint32_t kPW0 = 0x9E3779B9u, kPW1 = 0xBB67AE85u;
+constexpr int kMaxRank = 350;        // src/config.cuh:10
+constexpr int kPlusMaxTries = 1000;  // tryPlus / trySplitExisted rejection bound
+
+FGK_HD void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t &o0,
+                     uint32_t &o1, uint32_t &o2, uint32_t &o3) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += kPW0;
+            k1 += kPW1;
+        }
+        const uint64_t p0 = (uint64_t)kPM0 * c0;
+        const uint64_t p1 = (uint64_t)kPM1 * c2;
+        const uint32_t y0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t y2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = y0;
+        c1 = (uint32_t)p1;
+        c2 = y2;
+        c3 = (uint32_t)p0;
+    }
+    o0 = c0; o1 = c1; o2 = c2; o3 = c3;
+}
+
+struct LaneBase {
+    uint32_t *F;  // lane-offset plane bases
+    uint8_t *H;   // lane-offset byte base of the packed H plane
+    uint8_t *N;
+    int32_t *err;
+    int maxm;
+    int hw;       // H words per component
+    int n0, n1, n2;
+    int nn0, nn1, nn2;
+    int m;
+    uint32_t S0, S1, S2;  // unordered pair count per component
+    uint32_t k0, k1, wid;
+    uint64_t aux;
+    uint32_t a0, a1, a2, a3;
+    int aidx;
+    int visit_cap;  // integer ring: candidates one tryFlip may visit
+
+    // Extra per-lane LDS beyond the common planes (none by default).
+    FGK_HD void lane_extra(uint8_t *, int) {}
+
+    FGK_HD uint32_t &f(int c, int t) { return F[(c * maxm + t) * WG]; }
+    // H packs terms 4q..4q+3 of a lane into one dword: byte (t & 3) of word q.
+    FGK_HD uint8_t &h(int c, int t) { return H[((c * hw + (t >> 2)) * WG) * 4 + (t & 3)]; }
+    FGK_HD uint32_t &hword(int c, int q) { return reinterpret_cast<uint32_t *>(H)[(c * hw + q) * WG]; }
+    FGK_HD uint8_t &nx(int c, int t) { return N[(c * maxm + t) * WG]; }
+    FGK_HD void fail(int32_t code) { fgk_atomic_or(err, code); }
+
+    // Component-indexed scalars are selected arithmetically: a select
+    // between member addresses would keep the lane state out of registers.
+    FGK_HD int nn(int c) const {
+        return (c == 0 ? 1 : 0) * nn0 + (c == 1 ? 1 : 0) * nn1 + (c == 2 ? 1 : 0) * nn2;
+    }
+    FGK_HD void addS(int c, int d) {
+        S0 += (uint32_t)((c == 0 ? 1 : 0) * d);
+        S1 += (uint32_t)((c == 1 ? 1 : 0) * d);
+        S2 += (uint32_t)((c == 2 ? 1 : 0) * d);
+    }
+    FGK_HD int naive_rank() const { return n0 * n1 * n2; }
+
+    // ---- aux RNG stream ----
+    FGK_HD void aux_fill(uint64_t block) {
+        philox10((uint32_t)block, (uint32_t)(block >> 32), wid, 1u, k0, k1, a0, a1, a2, a3);
+    }
+    // a0..a3 hold the unread words of the current aux block, shifted down
+    // as they are consumed; aidx counts the words already taken (4 = empty).
+    FGK_HD uint32_t aux32() {
+        if (aidx >= 4) {
+            aux_fill(aux);
+            ++aux;
+            aidx = 0;
+        }
+        const uint32_t r = a0;
+        a0 = a1;
+        a1 = a2;
+        a2 = a3;
+        ++aidx;
+        return r;
+    }
+    // Reload the unread tail of block aux - 1 after a launch boundary.
+    FGK_HD void aux_resume() {
+        aux_fill(aux - 1);
+        for (int q = 0; q < aidx; ++q) {
+            a0 = a1;
+            a1 = a2;
+            a2 = a3;
+        }
+    }
+    // randomPermutation(p, 3), src/utils/random.cuh:9-22: step i = 2 swaps
+    // p[2] with p[rand % 3], step i = 1 swaps p[1] with p[rand % 2].
+    FGK_HD void perm3(int &p0, int &p1, int &p2) {
+        const uint32_t j2 = aux32() % 3u;
+        const int q0 = j2 == 0 ? 2 : 0;
+        const int q1 = j2 == 1 ? 2 : 1;
+        const int q2 = j2 == 0 ? 0 : (j2 == 1 ? 1 : 2);
+        const uint32_t j1 = aux32() % 2u;
+        p0 = j1 == 0 ? q1 : q0;
+        p1 = j1 == 0 ? q0 : q1;
+        p2 = q2;
+    }
+
+    // ---- partner classes ----
+    // initFlips (src/schemes/scheme_z2.cu:141-150) for the N/H representation.
+    FGK_HD void rebuild() {
+        S0 = S1 = S2 = 0;
+        for (int c = 0; c < 3; ++c) {
+            for (int q = 0; q < hw; ++q) hword(c, q) = 0;
+            for (int t = 0; t < m; ++t) nx(c, t) = 0xFF;
+            uint32_t S = 0;
+            for (int t = 0; t < m; ++t) {
+                if (nx(c, t) != 0xFF) continue;
+                const uint32_t v = f(c, t);
+                int prev = t, cnt = 0;
+                for (int u = t + 1; u < m; ++u) {
+                    if (f(c, u) == v) {
+                        nx(c, prev) = (uint8_t)u;
+                        prev = u;
+                        ++cnt;
+                    }
+                }
+                nx(c, prev) = (uint8_t)t;
+                int y = t, r = cnt;
+                do {
+                    h(c, y) = (uint8_t)r;
+                    --r;
+                    y = nx(c, y);
+                } while (y != t && r >= 0);
+                S += (uint32_t)(cnt * (cnt + 1) / 2);
+            }
+            addS(c, (int)S);
+        }
+    }
+
+    // Term x leaves its component-c class (its factor is about to change).
+    FGK_HD void leave(int c, int x) {
+        int y = nx(c, x);
+        if (y == x) return;
+        int p = x, cnt = 0;
+        while (y != x) {
+            if (y < x) h(c, y) = (uint8_t)(h(c, y) - 1);
+            ++cnt;
+            p = y;
+            y = nx(c, y);
+            if (cnt > m) {
+                fail(kErrWalk);
+                return;
+            }
+        }
+        nx(c, p) = nx(c, x);
+        nx(c, x) = (uint8_t)x;
+        h(c, x) = 0;
+        addS(c, -cnt);
+    }
+
+    // Singleton term x joins the component-c class containing y0.
+    FGK_HD void join(int c, int x, int y0) {
+        int y = y0, cnt = 0, gt = 0, p = -1, mx = -1;
+        do {
+            ++cnt;
+            if (y < x) {
+                h(c, y) = (uint8_t)(h(c, y) + 1);
+                if (y > p) p = y;
+            } else {
+                ++gt;
+            }
+            if (y > mx) mx = y;
+            y = nx(c, y);
+            if (cnt > m) {
+                fail(kErrWalk);
+                return;
+            }
+        } while (y != y0);
+        if (p < 0) p = mx;
+        h(c, x) = (uint8_t)gt;
+        nx(c, x) = nx(c, p);
+        nx(c, p) = (uint8_t)x;
+        addS(c, cnt);
+    }
+
+    // The unordered pair at position r (< S0 + S1 + S2) of the fresh-list
+    // order: component, then t1, then t2.  H is scanned in chunks of 4 words
+    // (16 terms) with byte sums, then t2 is r' + 1 steps along N from t1.
+    FGK_HD bool select_pair(uint32_t r, int &c, int &t1, int &t2) {
+        c = 0;
+        if (r >= S0) {
+            r -= S0;
+            c = 1;
+            if (r >= S1) {
+                r -= S1;
+                c = 2;
+            }
+        }
+        t1 = -1;
+        for (int q = 0; q < hw; q += 4) {
+            const uint32_t w0 = hword(c, q), w1 = hword(c, q + 1), w2 = hword(c, q + 2), w3 = hword(c, q + 3);
+            const uint32_t s0 = byte_sum(w0), s1 = byte_sum(w1), s2 = byte_sum(w2), s3 = byte_sum(w3);
+            const uint32_t tot = s0 + s1 + s2 + s3;
+            if (r >= tot) {
+                r -= tot;
+                continue;
+            }
+            uint32_t word = w0;
+            int base = q * 4;
+            if (r >= s0) {
+                r -= s0; word = w1; base += 4;
+                if (r >= s1) {
+                    r -= s1; word = w2; base += 4;
+                    if (r >= s2) { r -= s2; word = w3; base += 4; }
+                }
+            }
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t hv = (word >> (8 * b)) & 0xFFu;
+                if (r < hv) {
+                    t1 = base + b;
+                    break;
+                }
+                r -= hv;
+            }
+            break;
+        }
+        if (t1 < 0 || t1 >= m) {
+            fail(kErrSelect);
+            return false;
+        }
+        t2 = t1;
+        for (uint32_t q = 0; q <= r; ++q) t2 = nx(c, t2);
+        return true;
+    }
+
+    // ---- term edits (callers rebuild the classes afterwards) ----
+    FGK_HD void copy_term(int dst, int src) {
+        f(0, dst) = f(0, src);
+        f(1, dst) = f(1, src);
+        f(2, dst) = f(2, src);
+    }
+    FGK_HD bool nonzero_term(int t) { return f(0, t) && f(1, t) && f(2, t); }
+    // removeAt (src/schemes/scheme_z2.cu:166-175)
+    FGK_HD void remove_at(int t) {
+        --m;
+        if (t != m) copy_term(t, m);
+    }
+};
+
+// Orientation of a flip in component c: i = c, (j, k) = the other two in
+// increasing order, swapped by coin bit 0 (scheme_z2.cu:988-1016).
+FGK_HD void orientation(int c, uint32_t coin, int &j, int &k) {
+    j = c == 0 ? 1 : 0;
+    k = c == 2 ? 1 : 2;
+    if (coin & 1u) {
+        const int t = j;
+        j = k;
+        k = t;
+    }
+}
+
+// The walk of one lane (randomWalkKernel body, src/entities/flip_graph.cu:520-573).
+// `lds` is the workgroup's LDS block, `w` the walker index of this lane.
+template <class Lane>
+FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w) {
+    const bool active = w < A.walkers;
+    const int W = A.walkers;
+    const int maxm = A.max_m;
+
+    Lane s;
+    const int hw = h_words(maxm);
+    s.F = reinterpret_cast<uint32_t *>(lds) + lane;
+    s.H = lds + (size_t)3 * maxm * WG * sizeof(uint32_t) + (size_t)lane * 4;
+    s.N = lds + (size_t)3 * maxm * WG * sizeof(uint32_t) + (size_t)3 * hw * WG * 4 + lane;
+    s.lane_extra(lds + walk_lds_base_bytes(maxm), lane);
+    s.err = A.err;
+    s.maxm = maxm;
+    s.hw = hw;
+    s.k0 = A.key0;
+    s.k1 = A.key1;
+    s.wid = (uint32_t)w;
+    s.m = 0;
+    s.n0 = s.n1 = s.n2 = 1;
+    s.visit_cap = A.zt_visit_cap;
+    int best_rank = 0, best_m = 0, flips_count = 0;
+    uint64_t it = 0, total_flips = 0, sum_m = 0;
+    s.aux = 0;
+    s.aidx = 4;
+    if (active) {
+        s.n0 = A.fmt[4 * w];
+        s.n1 = A.fmt[4 * w + 1];
+        s.n2 = A.fmt[4 * w + 2];
+        s.m = A.m[w];
+        best_rank = A.best_rank[w];
+        best_m = A.best_m[w];
+        flips_count = A.flips_count[w];
+        it = A.it[w];
+        s.aux = A.aux[w];
+        s.aidx = A.aux_idx[w];
+        total_flips = A.total_flips[w];
+        sum_m = A.sum_m[w];
+        for (int c = 0; c < 3; ++c)
+            for (int t = 0; t < s.m; ++t) s.f(c, t) = A.cur[((size_t)c * maxm + t) * W + w];
+        if (s.aidx < 4 && s.aux > 0) s.aux_resume();
+    }
+    s.nn0 = s.n0 * s.n1;
+    s.nn1 = s.n1 * s.n2;
+    s.nn2 = s.n2 * s.n0;
+    if (active) s.rebuild();
+
+    int iters = 0;
+    if (active) {
+        iters = A.iterations;
+        if (A.random_iterations) iters = 1 + (int)(s.aux32() % (uint32_t)A.max_iterations);
+    }
+    const bool check_reduce = A.check_reduce != 0;
+
+    for (int iter = 0; iter < iters; ++iter) {
+        uint32_t b0, b1, b2, b3;
+        philox10((uint32_t)it, (uint32_t)(it >> 32), (uint32_t)w, 0u, s.k0, s.k1, b0, b1, b2, b3);
+        ++it;
+        const int rank = s.m;
+        if (!s.try_flip(b0, b1, check_reduce)) {  // flip_graph.cu:535-537
+            s.try_expand(1 + (int)(b1 % 2u));
+            continue;
+        }
+        ++total_flips;
+        sum_m += (uint64_t)rank;
+        if (s.m < rank) flips_count = 0;
+        ++flips_count;
+        if (s.m < best_rank || (s.m == best_rank && b2 < A.thr_best_eq)) {  // :545-548
+            best_rank = s.m;
+            best_m = s.m;
+            for (int c = 0; c < 3; ++c)
+                for (int t = 0; t < s.m; ++t) A.best[((size_t)c * maxm + t) * W + w] = s.f(c, t);
+        }
+        if (A.use_reduce && s.aux32() < A.thr_reduce) s.try_reduce();  // :550-551
+        if (A.use_expand && b3 < A.thr_expand && s.m <= best_m + 2) {   // :553-556
+            if (s.try_expand(1 + (int)(s.aux32() % 2u))) flips_count = 0;
+        }
+        if (flips_count >= A.plus_iterations && s.try_plus()) flips_count = 0;  // :564-565
+    }
+
+    if (active) {
+        A.m[w] = s.m;
+        A.best_m[w] = best_m;
+        A.best_rank[w] = best_rank;
+        A.flips_count[w] = flips_count;
+        A.it[w] = it;
+        A.aux[w] = s.aux;
+        A.aux_idx[w] = s.aidx;
+        A.total_flips[w] = total_flips;
+        A.sum_m[w] = sum_m;
+        for (int c = 0; c < 3; ++c)
+            for (int t = 0; t < s.m; ++t) A.cur[((size_t)c * maxm + t) * W + w] = s.f(c, t);
+    }
+}
+
+}  // namespace lane
+}  // namespace fgk

@@ -46,6 +46,15 @@ uint32_t threshold(double p, int max_iterations) {
 
 constexpr uint32_t kThrBestEq = 42949673u;  // 0.01 * 2^32 (flip_graph.cu:545)
 
+// Device word of one factor: the Z2 mask, or for the integer ring the
+// magnitudes in bits 0-15 and the signs in bits 16-31.
+uint32_t pack_word(int ring, uint64_t v, uint64_t s) {
+    return ring ? (uint32_t)((v & 0xFFFFu) | ((s & v & 0xFFFFu) << 16)) : (uint32_t)v;
+}
+uint64_t word_values(int ring, uint32_t w) { return ring ? (w & 0xFFFFu) : w; }
+uint64_t word_signs(int ring, uint32_t w) { return ring ? (w >> 16) : 0; }
+int max_coefficients(int ring) { return ring ? 16 : 32; }
+
 }  // namespace
 
 struct fg_ctx {
@@ -99,7 +108,6 @@ extern "C" int fg_create(const fg_config *cfg, fg_ctx **out) {
     if (cfg->walkers <= 0) return fail("fg_create: walkers must be positive");
     if (cfg->max_iterations <= 0) return fail("fg_create: max_iterations must be positive");
     if (cfg->plus_iterations < 0) return fail("fg_create: plus_iterations must be >= 0");
-    if (cfg->ring == FG_RING_ZT) return fail("fg_create: the integer ring walk kernel is not available in this build");
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
@@ -139,14 +147,18 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, int n1, int n2
     if (starts.empty()) return fail("init: no schemes");
     const int W = ctx->cfg.walkers;
     int maxm = ctx->cfg.max_rank;
+    const int ring = ctx->cfg.ring;
     for (const auto &s : starts) {
         maxm = std::max(maxm, std::max(s.m, s.naive_rank()));
-        if (s.ring != ctx->cfg.ring) return fail("init: ring mismatch");
+        if (s.ring != ring) return fail("init: ring mismatch");
         for (int c = 0; c < 3; ++c)
-            if (s.nn[c] > 32) return fail("init: factors with more than 32 coefficients are not supported on the GPU path");
+            if (s.nn[c] > max_coefficients(ring))
+                return fail("init: factors with more than " + std::to_string(max_coefficients(ring)) +
+                            " coefficients are not supported on the GPU path for this ring");
     }
     if (maxm > 255) return fail("init: more than 255 terms per scheme are not supported on the GPU path");
-    if (fgk::z2_lds_bytes(maxm) > ctx->lds_limit)
+    const size_t lds = ring ? fgk::zt_lds_bytes(maxm) : fgk::z2_lds_bytes(maxm);
+    if (lds > ctx->lds_limit)
         return fail("init: " + std::to_string(maxm) + " terms per walker exceed the LDS of one workgroup");
 
     ctx->release();
@@ -167,7 +179,7 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, int n1, int n2
         fmt[4 * w + 1] = s.n[1];
         fmt[4 * w + 2] = s.n[2];
         for (int c = 0; c < 3; ++c)
-            for (int t = 0; t < s.m; ++t) words[((size_t)c * maxm + t) * W + w] = (uint32_t)s.v[c][t];
+            for (int t = 0; t < s.m; ++t) words[((size_t)c * maxm + t) * W + w] = pack_word(ring, s.v[c][t], s.s[c][t]);
         const std::string key = fg::format_key(s.n[0], s.n[1], s.n[2]);
         auto f = best_by_key.find(key);
         if (f == best_by_key.end() || s.m < f->second) best_by_key[key] = s.m;
@@ -207,8 +219,8 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, int n1, int n2
     return 0;
 }
 
-int verify_planes(fg_ctx *ctx, const uint32_t *words, const int32_t *m, const int32_t *fmt, int count, int maxm,
-                  int32_t *ok_dev) {
+int verify_planes(fg_ctx *ctx, int ring, const uint32_t *words, const int32_t *m, const int32_t *fmt, int count,
+                  int maxm, int32_t *ok_dev) {
     fgk::VerifyArgs v{};
     v.words = words;
     v.signs = nullptr;
@@ -218,7 +230,7 @@ int verify_planes(fg_ctx *ctx, const uint32_t *words, const int32_t *m, const in
     v.max_m = maxm;
     v.ok = ok_dev;
     HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(ok_dev), 1, count, ctx->stream));
-    HIPCHK(fgk::launch_verify_z2(v, ctx->stream));
+    HIPCHK(fgk::launch_verify(ring, v, ctx->stream));
     return 0;
 }
 
@@ -285,9 +297,9 @@ extern "C" int fg_walk(fg_ctx *ctx, int iterations) {
     a.err = ctx->err;
     HIPCHK(hipMemsetAsync(ctx->err, 0, sizeof(int32_t), ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(fgk::launch_walk_z2(a, ctx->stream));
+    HIPCHK(fgk::launch_walk(ctx->cfg.ring, a, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-    int rc = verify_planes(ctx, ctx->cur, ctx->m, ctx->fmt, ctx->W, ctx->maxm, ctx->ok);
+    int rc = verify_planes(ctx, ctx->cfg.ring, ctx->cur, ctx->m, ctx->fmt, ctx->W, ctx->maxm, ctx->ok);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
@@ -388,9 +400,18 @@ extern "C" int fg_read_walkers(fg_ctx *ctx, int begin, int count, fg_walker_reco
         r.aux = aux[w];
         r.total_flips = tf[w];
         r.sum_m = sm[w];
+        const int ring = ctx->cfg.ring;
         for (int c = 0; c < 3; ++c) {
-            for (int t = 0; t < r.m; ++t) r.cur_v[c][t] = cur[((size_t)c * maxm + t) * W + w];
-            for (int t = 0; t < r.best_m; ++t) r.best_v[c][t] = best[((size_t)c * maxm + t) * W + w];
+            for (int t = 0; t < r.m; ++t) {
+                const uint32_t x = cur[((size_t)c * maxm + t) * W + w];
+                r.cur_v[c][t] = word_values(ring, x);
+                r.cur_s[c][t] = word_signs(ring, x);
+            }
+            for (int t = 0; t < r.best_m; ++t) {
+                const uint32_t x = best[((size_t)c * maxm + t) * W + w];
+                r.best_v[c][t] = word_values(ring, x);
+                r.best_s[c][t] = word_signs(ring, x);
+            }
         }
     }
     return 0;
@@ -406,7 +427,8 @@ extern "C" int fg_write_walkers(fg_ctx *ctx, int begin, int count, const fg_walk
             return fail("fg_write_walkers: record rank exceeds the pool capacity");
         if (r.n[0] * r.n[1] * r.n[2] > maxm) return fail("fg_write_walkers: format exceeds the pool capacity");
         for (int c = 0; c < 3; ++c)
-            if (r.n[c] * r.n[(c + 1) % 3] > 32) return fail("fg_write_walkers: factor wider than 32 coefficients");
+            if (r.n[c] * r.n[(c + 1) % 3] > max_coefficients(ctx->cfg.ring))
+                return fail("fg_write_walkers: factor wider than the GPU word of this ring");
     }
     const size_t plane = (size_t)3 * maxm * W;
     std::vector<uint32_t> cur(plane), best(plane);
@@ -437,9 +459,11 @@ extern "C" int fg_write_walkers(fg_ctx *ctx, int begin, int count, const fg_walk
         aux[w] = r.aux;
         tf[w] = r.total_flips;
         sm[w] = r.sum_m;
+        const int ring = ctx->cfg.ring;
         for (int c = 0; c < 3; ++c) {
-            for (int t = 0; t < r.m; ++t) cur[((size_t)c * maxm + t) * W + w] = (uint32_t)r.cur_v[c][t];
-            for (int t = 0; t < r.best_m; ++t) best[((size_t)c * maxm + t) * W + w] = (uint32_t)r.best_v[c][t];
+            for (int t = 0; t < r.m; ++t) cur[((size_t)c * maxm + t) * W + w] = pack_word(ring, r.cur_v[c][t], r.cur_s[c][t]);
+            for (int t = 0; t < r.best_m; ++t)
+                best[((size_t)c * maxm + t) * W + w] = pack_word(ring, r.best_v[c][t], r.best_s[c][t]);
         }
     }
     HIPCHK(hipMemcpy(ctx->cur, cur.data(), plane * 4, hipMemcpyHostToDevice));
@@ -475,7 +499,10 @@ int walker_scheme(fg_ctx *ctx, int w, int which, fg::Scheme *out) {
     s.m = m;
     s.resize(m);
     for (int c = 0; c < 3; ++c)
-        for (int t = 0; t < m; ++t) s.v[c][t] = col[(size_t)c * maxm + t];
+        for (int t = 0; t < m; ++t) {
+            s.v[c][t] = word_values(s.ring, col[(size_t)c * maxm + t]);
+            s.s[c][t] = word_signs(s.ring, col[(size_t)c * maxm + t]);
+        }
     *out = std::move(s);
     return 0;
 }
@@ -517,7 +544,7 @@ extern "C" int fg_best(fg_ctx *ctx, int32_t *out_blob, int64_t cap, int32_t *out
 
 extern "C" int fg_verify(int ring, const int32_t *blob, int64_t len, int32_t *out_ok) {
     if (!blob || len < 1 || !out_ok) return fail("fg_verify: bad arguments");
-    if (ring != FG_RING_Z2) return fail("fg_verify: only the Z2 check runs on the GPU in this build");
+    if (ring != FG_RING_Z2 && ring != FG_RING_ZT) return fail("fg_verify: unknown ring");
     const int count = blob[0];
     if (count < 0) return fail("fg_verify: negative count");
     std::vector<fg::Scheme> ss(count);
@@ -527,7 +554,7 @@ extern "C" int fg_verify(int ring, const int32_t *blob, int64_t len, int32_t *ou
         std::string err;
         if (!fg::scheme_from_blob(ring, blob, len, &pos, &ss[q], &err)) return fail("fg_verify: " + err);
         for (int c = 0; c < 3; ++c)
-            if (ss[q].nn[c] > 32) return fail("fg_verify: factor wider than 32 coefficients");
+            if (ss[q].nn[c] > max_coefficients(ring)) return fail("fg_verify: factor wider than the GPU word of this ring");
         maxm = std::max(maxm, ss[q].m);
     }
     if (count == 0) return 0;
@@ -537,7 +564,8 @@ extern "C" int fg_verify(int ring, const int32_t *blob, int64_t len, int32_t *ou
         m[q] = ss[q].m;
         for (int c = 0; c < 3; ++c) fmt[4 * q + c] = ss[q].n[c];
         for (int c = 0; c < 3; ++c)
-            for (int t = 0; t < ss[q].m; ++t) words[((size_t)c * maxm + t) * count + q] = (uint32_t)ss[q].v[c][t];
+            for (int t = 0; t < ss[q].m; ++t)
+                words[((size_t)c * maxm + t) * count + q] = pack_word(ring, ss[q].v[c][t], ss[q].s[c][t]);
     }
     uint32_t *dw = nullptr;
     int32_t *dm = nullptr, *df = nullptr, *dok = nullptr;
@@ -564,7 +592,7 @@ extern "C" int fg_verify(int ring, const int32_t *blob, int64_t len, int32_t *ou
         cleanup();
         return fail("fg_verify: copy failed");
     }
-    rc = verify_planes(&tmp, dw, dm, df, count, maxm, dok);
+    rc = verify_planes(&tmp, ring, dw, dm, df, count, maxm, dok);
     if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail("fg_verify: kernel failed");
     if (!rc && hipMemcpy(out_ok, dok, (size_t)count * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = fail("fg_verify: copy back failed");
     tmp.stream = nullptr;

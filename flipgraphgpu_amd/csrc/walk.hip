@@ -1,0 +1,109 @@
+// Flip-graph random walk kernels for CDNA4 (gfx950): the Z2 and integer
+// ring kernels around the per-lane logic of walk_z2_lane.h /
+// walk_zt_lane.h, the bit-sliced Brent checks, and their host launchers.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "walk_args.h"
+#include "walk_z2_lane.h"
+#include "walk_zt_lane.h"
+
+namespace fgk {
+
+__global__ void __launch_bounds__(kWalkersPerGroup) walk_z2_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    const int lane = threadIdx.x;
+    lane::walk_lane<lane::Z2Lane>(A, lds, lane, blockIdx.x * kWalkersPerGroup + lane);
+}
+
+__global__ void __launch_bounds__(kWalkersPerGroup) walk_zt_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    const int lane = threadIdx.x;
+    lane::walk_lane<lane::ZTLane>(A, lds, lane, blockIdx.x * kWalkersPerGroup + lane);
+}
+
+// Brent equations (src/schemes/scheme_z2.cu:3-29), bit-sliced: one thread
+// per (scheme, U index i); for each V index j the XOR over terms of the W
+// masks with u_t[i] v_t[j] = 1 must equal the single W bit of c[i/n2][j%n3].
+__global__ void verify_z2_kernel(VerifyArgs V) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = g / 64;
+    const int i = g % 64;
+    if (s >= V.count) return;
+    const int n0 = V.fmt[4 * s], n1 = V.fmt[4 * s + 1], n2 = V.fmt[4 * s + 2];
+    const int nn0 = n0 * n1, nn1 = n1 * n2;
+    if (i >= nn0) return;
+    const int m = V.m[s];
+    const size_t stride = (size_t)V.count;
+    bool ok = true;
+    for (int j = 0; j < nn1 && ok; ++j) {
+        uint32_t acc = 0;
+        for (int t = 0; t < m; ++t) {
+            const uint32_t u = V.words[((size_t)0 * V.max_m + t) * stride + s];
+            const uint32_t v = V.words[((size_t)1 * V.max_m + t) * stride + s];
+            if (((u >> i) & (v >> j) & 1u) != 0) acc ^= V.words[((size_t)2 * V.max_m + t) * stride + s];
+        }
+        const int i1 = i / n1, i2 = i % n1, j1 = j / n2, j2 = j % n2;
+        const uint32_t want = (i2 == j1) ? (1u << (j2 * n0 + i1)) : 0u;
+        ok = acc == want;
+    }
+    if (!ok) V.ok[s] = 0;
+}
+
+// Brent equations over the integers (src/schemes/scheme_integer.cu:3-38) for
+// packed ternary words (magnitudes bits 0-15, signs bits 16-31): one thread
+// per (scheme, U index i) accumulates sum_t u_t[i] v_t[j] w_t[k] for every
+// (j, k).
+__global__ void verify_zt_kernel(VerifyArgs V) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = g / 64;
+    const int i = g % 64;
+    if (s >= V.count) return;
+    const int n0 = V.fmt[4 * s], n1 = V.fmt[4 * s + 1], n2 = V.fmt[4 * s + 2];
+    const int nn0 = n0 * n1, nn1 = n1 * n2, nn2 = n2 * n0;
+    if (i >= nn0) return;
+    const int m = V.m[s];
+    const size_t stride = (size_t)V.count;
+    bool ok = true;
+    for (int j = 0; j < nn1 && ok; ++j) {
+        for (int k = 0; k < nn2 && ok; ++k) {
+            int e = 0;
+            for (int t = 0; t < m; ++t) {
+                const uint32_t u = V.words[((size_t)0 * V.max_m + t) * stride + s];
+                const uint32_t v = V.words[((size_t)1 * V.max_m + t) * stride + s];
+                const uint32_t w = V.words[((size_t)2 * V.max_m + t) * stride + s];
+                if ((u >> i) & (v >> j) & (w >> k) & 1u) {
+                    const uint32_t neg = ((u >> (16 + i)) ^ (v >> (16 + j)) ^ (w >> (16 + k))) & 1u;
+                    e += neg ? -1 : 1;
+                }
+            }
+            const int i1 = i / n1, i2 = i % n1, j1 = j / n2, j2 = j % n2, k1 = k / n0, k2 = k % n0;
+            const int target = (i2 == j1) && (i1 == k2) && (j2 == k1);
+            ok = e == target;
+        }
+    }
+    if (!ok) V.ok[s] = 0;
+}
+
+hipError_t launch_walk(int ring, const WalkArgs &a, hipStream_t stream) {
+    const size_t lds = ring ? zt_lds_bytes(a.max_m) : z2_lds_bytes(a.max_m);
+    const void *fn = ring ? reinterpret_cast<const void *>(walk_zt_kernel) : reinterpret_cast<const void *>(walk_z2_kernel);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    const int grid = (a.walkers + kWalkersPerGroup - 1) / kWalkersPerGroup;
+    if (ring) hipLaunchKernelGGL(walk_zt_kernel, dim3(grid), dim3(kWalkersPerGroup), lds, stream, a);
+    else hipLaunchKernelGGL(walk_z2_kernel, dim3(grid), dim3(kWalkersPerGroup), lds, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_verify(int ring, const VerifyArgs &v, hipStream_t stream) {
+    const long threads = (long)v.count * 64;
+    const int block = 256;
+    const int grid = (int)((threads + block - 1) / block);
+    if (grid == 0) return hipSuccess;
+    if (ring) hipLaunchKernelGGL(verify_zt_kernel, dim3(grid), dim3(block), 0, stream, v);
+    else hipLaunchKernelGGL(verify_z2_kernel, dim3(grid), dim3(block), 0, stream, v);
+    return hipGetLastError();
+}
+
+}  // namespace fgk

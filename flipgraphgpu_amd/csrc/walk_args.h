@@ -56,9 +56,18 @@ constexpr int32_t kErrCapacity = 4; // a term would exceed the LDS capacity
 constexpr int kHTermsPad = 16;
 __host__ __device__ inline int h_words(int max_m) { return ((max_m + kHTermsPad - 1) / kHTermsPad) * (kHTermsPad / 4); }
 
-// LDS bytes per workgroup for the Z2 kernel with 32-bit factor words.
-__host__ __device__ inline size_t z2_lds_bytes(int max_m) {
+// LDS bytes per workgroup of the common planes: 32-bit factor words F,
+// packed H words and N bytes, for every lane.
+__host__ __device__ inline size_t walk_lds_base_bytes(int max_m) {
     return (size_t)kWalkersPerGroup * (3 * (size_t)max_m * 4 + 3 * (size_t)h_words(max_m) * 4 + 3 * (size_t)max_m);
+}
+__host__ __device__ inline size_t z2_lds_bytes(int max_m) { return walk_lds_base_bytes(max_m); }
+
+// Integer ring: the candidate pairs one tryFlip visits are remembered in
+// LDS (16-bit indices, kZtMaxVisits per lane).
+constexpr int kZtMaxVisits = 16;
+__host__ __device__ inline size_t zt_lds_bytes(int max_m) {
+    return walk_lds_base_bytes(max_m) + (size_t)kWalkersPerGroup * kZtMaxVisits * 2;
 }
 
 // Brent check of schemes held in the same plane layout.

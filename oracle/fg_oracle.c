@@ -558,34 +558,35 @@ static int try_flip_z2(fgo_scheme *s, const uint32_t b[4], const fgo_walk_params
 }
 
 /* Integer tryFlip (scheme_integer.cu:907-975).  The reference visits every
- * candidate in a uniformly random order (a full Fisher-Yates permutation
- * drawn up front) and applies the first admissible one.  Here the same
- * visiting order is drawn lazily (forward Fisher-Yates: step q swaps
- * position q with q + rand % (S - q)), so only visited candidates cost
- * draws; at most zt_visit_cap candidates are visited.  Visit 0 uses the
- * main block (b[0] position, b[1] coins), later visits the aux stream. */
+ * candidate pair in a uniformly random order (a Fisher-Yates permutation of
+ * all of them, drawn up front) and applies the first admissible one.  Here
+ * the same visiting order is drawn lazily: each visit draws a candidate
+ * uniformly and redraws while it was already visited (at most
+ * FGO_ZT_REDRAWS redraws), so only visited candidates cost draws; at most
+ * min(zt_visit_cap, FGO_ZT_MAX_VISITS) candidates are visited.  Visit 0
+ * takes the main block (b[0] candidate, b[1] coins); later visits draw the
+ * candidate and then the coins from the aux stream. */
+#define FGO_ZT_MAX_VISITS 16
+#define FGO_ZT_REDRAWS 64
 static int try_flip_zt(fgo_scheme *s, const uint32_t b[4], fgo_rng *r, const fgo_walk_params *p) {
     int size = total_pairs(s, p->select_mode);
     if (!size) return 0;
     int cap = p->zt_visit_cap < size ? p->zt_visit_cap : size;
-    /* sparse virtual permutation: keys/vals overrides of identity */
-    int keys[2 * 1024], vals[2 * 1024], nov = 0;
-    if (cap > 1024) cap = 1024;
+    if (cap > FGO_ZT_MAX_VISITS) cap = FGO_ZT_MAX_VISITS;
+    int visited[FGO_ZT_MAX_VISITS];
+    int nv = 0;
     for (int q = 0; q < cap; q++) {
-        uint32_t pick = q == 0 ? b[0] : fgo_rng_aux32(r);
-        uint32_t coin = q == 0 ? b[1] : fgo_rng_aux32(r);
-        int jpos = q + (int)(pick % (uint32_t)(size - q));
-        int vq = q, vj = jpos;
-        for (int o = 0; o < nov; o++) { if (keys[o] == q) vq = vals[o]; if (keys[o] == jpos) vj = vals[o]; }
-        /* swap a[q] <-> a[jpos] */
-        int found_q = 0, found_j = 0;
-        for (int o = 0; o < nov; o++) {
-            if (keys[o] == q) { vals[o] = vj; found_q = 1; }
-            else if (keys[o] == jpos) { vals[o] = vq; found_j = 1; }
+        int cand = (int)((q == 0 ? b[0] : fgo_rng_aux32(r)) % (uint32_t)size);
+        int redraws = 0;
+        for (;;) {
+            int seen = 0;
+            for (int o = 0; o < nv; o++) seen |= visited[o] == cand;
+            if (!seen) break;
+            if (++redraws > FGO_ZT_REDRAWS) return 0;
+            cand = (int)(fgo_rng_aux32(r) % (uint32_t)size);
         }
-        if (!found_q) { keys[nov] = q; vals[nov] = vj; nov++; }
-        if (!found_j && jpos != q) { keys[nov] = jpos; vals[nov] = vq; nov++; }
-        int cand = vj;
+        visited[nv++] = cand;
+        uint32_t coin = q == 0 ? b[1] : fgo_rng_aux32(r);
         int c, t1, t2, i, j, k;
         select_pair(s, p->select_mode, (uint32_t)cand, &c, &t1, &t2);
         pick_orientation(c, (int)coin, &i, &j, &k);

@@ -83,7 +83,7 @@ typedef struct {
     uint32_t thr_best_eq;  /* equal-rank best copy, 0.01 * 2^32 */
     int select_mode;       /* FGO_SELECT_* */
     int check_reduce;      /* tryFlip(checkReduce) */
-    int zt_visit_cap;      /* max candidates one integer tryFlip visits */
+    int zt_visit_cap;      /* max candidates one integer tryFlip visits (at most 16) */
 } fgo_walk_params;
 
 typedef struct {

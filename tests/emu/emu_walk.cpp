@@ -1,4 +1,4 @@
-// Host emulation of the Z2 walk kernel for the CPU test suite: runs the
+// Host emulation of the walk kernels for the CPU test suite: runs the
 // kernel's own per-lane code (flipgraphgpu_amd/csrc/walk_z2_lane.h) one lane
 // at a time over a host LDS buffer.  Lanes of the kernel never communicate,
 // so this reproduces the GPU result exactly.  Test infrastructure only.
@@ -8,11 +8,17 @@
 #include "fg.h"
 #include "walk_args.h"
 #include "walk_z2_lane.h"
+#include "walk_zt_lane.h"
 
-extern "C" int emu_walk_z2_records(fg_walker_record *recs, int count, uint64_t seed, int iterations,
+static uint32_t pack(int ring, uint64_t v, uint64_t s) {
+    return ring ? (uint32_t)((v & 0xFFFFu) | ((s & v & 0xFFFFu) << 16)) : (uint32_t)v;
+}
+
+extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uint64_t seed, int iterations,
                                    int max_iterations, int random_iterations, int plus_iterations,
                                    int use_reduce, uint32_t thr_reduce, int use_expand, uint32_t thr_expand,
-                                   uint32_t thr_best_eq, int check_reduce, int launches, int32_t *err_out) {
+                                   uint32_t thr_best_eq, int check_reduce, int launches, int visit_cap,
+                                   int32_t *err_out) {
     int maxm = 1;
     for (int q = 0; q < count; ++q) {
         maxm = std::max(maxm, std::max(recs[q].m, recs[q].best_m));
@@ -29,8 +35,8 @@ extern "C" int emu_walk_z2_records(fg_walker_record *recs, int count, uint64_t s
         m[w] = r.m; bm[w] = r.best_m; br[w] = r.best_rank; fc[w] = r.flips_count; ai[w] = r.aux_idx;
         it[w] = r.it; aux[w] = r.aux; tf[w] = r.total_flips; sm[w] = r.sum_m;
         for (int c = 0; c < 3; ++c) {
-            for (int t = 0; t < r.m; ++t) cur[((size_t)c * maxm + t) * W + w] = (uint32_t)r.cur_v[c][t];
-            for (int t = 0; t < r.best_m; ++t) best[((size_t)c * maxm + t) * W + w] = (uint32_t)r.best_v[c][t];
+            for (int t = 0; t < r.m; ++t) cur[((size_t)c * maxm + t) * W + w] = pack(ring, r.cur_v[c][t], r.cur_s[c][t]);
+            for (int t = 0; t < r.best_m; ++t) best[((size_t)c * maxm + t) * W + w] = pack(ring, r.best_v[c][t], r.best_s[c][t]);
         }
     }
     int32_t err = 0;
@@ -42,22 +48,34 @@ extern "C" int emu_walk_z2_records(fg_walker_record *recs, int count, uint64_t s
     a.iterations = iterations; a.max_iterations = max_iterations; a.random_iterations = random_iterations;
     a.plus_iterations = plus_iterations; a.use_reduce = use_reduce; a.thr_reduce = thr_reduce;
     a.use_expand = use_expand; a.thr_expand = thr_expand; a.thr_best_eq = thr_best_eq;
-    a.check_reduce = check_reduce; a.zt_visit_cap = 64; a.err = &err;
-    std::vector<uint8_t> lds(fgk::z2_lds_bytes(maxm), 0xCD);
+    a.check_reduce = check_reduce; a.zt_visit_cap = visit_cap; a.err = &err;
+    std::vector<uint8_t> lds(ring ? fgk::zt_lds_bytes(maxm) : fgk::z2_lds_bytes(maxm), 0xCD);
     const int G = (W + fgk::kWalkersPerGroup - 1) / fgk::kWalkersPerGroup;
     for (int l = 0; l < launches; ++l)
         for (int g = 0; g < G; ++g)
-            for (int lane = 0; lane < fgk::kWalkersPerGroup; ++lane)
-                fgk::z2::walk_lane(a, lds.data(), lane, g * fgk::kWalkersPerGroup + lane);
+            for (int lane = 0; lane < fgk::kWalkersPerGroup; ++lane) {
+                if (ring) fgk::lane::walk_lane<fgk::lane::ZTLane>(a, lds.data(), lane, g * fgk::kWalkersPerGroup + lane);
+                else fgk::lane::walk_lane<fgk::lane::Z2Lane>(a, lds.data(), lane, g * fgk::kWalkersPerGroup + lane);
+            }
     for (int w = 0; w < W; ++w) {
         fg_walker_record &r = recs[w];
         std::memset(r.cur_v, 0, sizeof r.cur_v);
         std::memset(r.best_v, 0, sizeof r.best_v);
+        std::memset(r.cur_s, 0, sizeof r.cur_s);
+        std::memset(r.best_s, 0, sizeof r.best_s);
         r.m = m[w]; r.best_m = bm[w]; r.best_rank = br[w]; r.flips_count = fc[w]; r.aux_idx = ai[w];
         r.it = it[w]; r.aux = aux[w]; r.total_flips = tf[w]; r.sum_m = sm[w];
         for (int c = 0; c < 3; ++c) {
-            for (int t = 0; t < r.m; ++t) r.cur_v[c][t] = cur[((size_t)c * maxm + t) * W + w];
-            for (int t = 0; t < r.best_m; ++t) r.best_v[c][t] = best[((size_t)c * maxm + t) * W + w];
+            for (int t = 0; t < r.m; ++t) {
+                const uint32_t x = cur[((size_t)c * maxm + t) * W + w];
+                r.cur_v[c][t] = ring ? (x & 0xFFFFu) : x;
+                r.cur_s[c][t] = ring ? (x >> 16) : 0;
+            }
+            for (int t = 0; t < r.best_m; ++t) {
+                const uint32_t x = best[((size_t)c * maxm + t) * W + w];
+                r.best_v[c][t] = ring ? (x & 0xFFFFu) : x;
+                r.best_s[c][t] = ring ? (x >> 16) : 0;
+            }
         }
     }
     *err_out = err;

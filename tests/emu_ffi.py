@@ -17,20 +17,20 @@ def lib():
     if _lib is None:
         subprocess.run(["make", "-s", "-C", os.path.join(HERE, "emu")], check=True)
         L = ctypes.CDLL(LIB)
-        L.emu_walk_z2_records.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64] + [ctypes.c_int] * 4 + \
-            [ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
-             ctypes.POINTER(ctypes.c_int32)]
+        L.emu_walk_records.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64] + \
+            [ctypes.c_int] * 4 + [ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
 
 
-def walk_z2(recs, params, seed, launches=1):
+def walk(recs, params, seed, launches=1, ring=O.RING_Z2):
     recs = np.ascontiguousarray(recs.copy())
     err = ctypes.c_int32()
     p = params
-    rc = lib().emu_walk_z2_records(recs.ctypes.data, len(recs), seed, p.iterations, p.max_iterations,
-                                   p.random_iterations, p.plus_iterations, p.use_reduce, p.thr_reduce,
-                                   p.use_expand, p.thr_expand, p.thr_best_eq, p.check_reduce, launches,
-                                   ctypes.byref(err))
+    rc = lib().emu_walk_records(ring, recs.ctypes.data, len(recs), seed, p.iterations, p.max_iterations,
+                                p.random_iterations, p.plus_iterations, p.use_reduce, p.thr_reduce,
+                                p.use_expand, p.thr_expand, p.thr_best_eq, p.check_reduce, launches,
+                                p.zt_visit_cap, ctypes.byref(err))
     assert rc == 0
     return recs, err.value

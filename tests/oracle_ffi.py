@@ -63,7 +63,7 @@ THR_BEST_EQ = 42949673  # 0.01 * 2^32, rounded up
 
 def make_params(iterations=1000, max_iterations=None, plus_iterations=8000, p_expand=0.0,
                 p_reduce=0.0, random_iterations=False, select_mode=SELECT_CANONICAL,
-                check_reduce=True, zt_visit_cap=64):
+                check_reduce=True, zt_visit_cap=16):
     mi = max_iterations if max_iterations is not None else iterations
     return WalkParams(iterations, mi, int(random_iterations), plus_iterations,
                       int(p_reduce > 0), threshold(p_reduce, mi),

@@ -10,12 +10,12 @@ from walk_compare import first_mismatch
 
 
 def _check(n, walkers, iterations, seed, launches=1, plus=8000, p_expand=0.0, random_iterations=False,
-           check_reduce=True):
-    start = O.naive_records(*n, walkers)
-    p = O.make_params(iterations=iterations, plus_iterations=plus, p_expand=p_expand,
+           check_reduce=True, ring=O.RING_Z2, p_reduce=0.0):
+    start = O.naive_records(*n, walkers, ring=ring)
+    p = O.make_params(iterations=iterations, plus_iterations=plus, p_expand=p_expand, p_reduce=p_reduce,
                       random_iterations=random_iterations, check_reduce=check_reduce)
-    emu, err = E.walk_z2(start, p, seed, launches=launches)
-    ora = O.walk_records(start, p, seed, launches=launches, nthreads=8)
+    emu, err = E.walk(start, p, seed, launches=launches, ring=ring)
+    ora = O.walk_records(start, p, seed, launches=launches, nthreads=8, ring=ring)
     assert err == 0
     mism = first_mismatch(emu, ora)
     assert mism is None, mism
@@ -44,3 +44,20 @@ def test_emulated_kernel_without_reductions():
     # complexity-minimizer mode: tryFlip(checkReduce = false)
     ora = _check((3, 3, 3), 64, 2000, seed=3, check_reduce=False)
     assert (ora["m"] >= 27).all() or (ora["m"] < 27).any()
+
+
+def test_emulated_kernel_reduce_probability():
+    _check((3, 3, 3), 64, 1500, seed=11, p_reduce=300.0, plus=200, p_expand=50.0)
+
+
+@pytest.mark.parametrize("n,walkers,iterations", [
+    ((2, 2, 2), 64, 2000), ((2, 2, 3), 64, 2000), ((3, 3, 3), 64, 2500), ((4, 4, 4), 64, 1500),
+])
+def test_emulated_integer_kernel_matches_oracle(n, walkers, iterations):
+    ora = _check(n, walkers, iterations, seed=4242, ring=O.RING_ZT)
+    assert (ora["total_flips"] > 0).all()
+
+
+@pytest.mark.parametrize("n", [(2, 2, 2), (3, 3, 3)])
+def test_emulated_integer_kernel_plus_expand_paths(n):
+    _check(n, 64, 2000, seed=8, launches=2, plus=150, p_expand=200.0, ring=O.RING_ZT, p_reduce=50.0)

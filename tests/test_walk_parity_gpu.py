@@ -12,21 +12,23 @@ from walk_compare import first_mismatch
 pytestmark = pytest.mark.gpu
 
 
-def _run_both(n, walkers, iterations, seed, launches=1, plus=8000, p_expand=0.0, random_iterations=False):
-    print(f"[parity] n={n} walkers={walkers} iterations={iterations} launches={launches} seed={seed} "
-          f"plus={plus} p_expand={p_expand} random={random_iterations}", flush=True)
-    with fg.Engine(walkers, seed, max_iterations=iterations, plus_iterations=plus,
-                   expand_probability=p_expand, random_iterations=random_iterations) as e:
+def _run_both(n, walkers, iterations, seed, launches=1, plus=8000, p_expand=0.0, random_iterations=False,
+              ring=O.RING_Z2, p_reduce=0.0):
+    print(f"[parity] ring={ring} n={n} walkers={walkers} iterations={iterations} launches={launches} seed={seed} "
+          f"plus={plus} p_expand={p_expand} p_reduce={p_reduce} random={random_iterations}", flush=True)
+    with fg.Engine(walkers, seed, ring=ring, max_iterations=iterations, plus_iterations=plus,
+                   expand_probability=p_expand, reduce_probability=p_reduce,
+                   random_iterations=random_iterations, zt_visit_cap=16) as e:
         e.init_naive(*n)
         init = e.read_walkers()
-        ora0 = O.naive_records(*n, walkers)
+        ora0 = O.naive_records(*n, walkers, ring=ring)
         assert first_mismatch(init, ora0) is None
         for _ in range(launches):
             assert e.walk(iterations) == 0
         gpu = e.read_walkers()
-    p = O.make_params(iterations=iterations, plus_iterations=plus, p_expand=p_expand,
+    p = O.make_params(iterations=iterations, plus_iterations=plus, p_expand=p_expand, p_reduce=p_reduce,
                       random_iterations=random_iterations)
-    ora = O.walk_records(ora0, p, seed, launches=launches, nthreads=8)
+    ora = O.walk_records(ora0, p, seed, launches=launches, nthreads=8, ring=ring)
     return gpu, ora
 
 
@@ -54,6 +56,20 @@ def test_random_iterations_bit_identical():
     gpu, ora = _run_both((3, 3, 3), 96, 1500, seed=99, launches=3, random_iterations=True, p_expand=20.0)
     assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
     assert len(set(int(x) for x in ora["it"])) > 10  # iteration counts really differ per walker
+
+
+@pytest.mark.parametrize("n,walkers,iterations", [((2, 2, 2), 128, 2500), ((3, 3, 3), 128, 3000),
+                                                 ((4, 4, 4), 64, 1500)])
+def test_integer_walk_bit_identical_to_oracle(n, walkers, iterations):
+    gpu, ora = _run_both(n, walkers, iterations, seed=4242, ring=O.RING_ZT)
+    assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
+
+
+def test_integer_walk_rare_paths_bit_identical():
+    gpu, ora = _run_both((3, 3, 3), 128, 2000, seed=8, launches=2, plus=150, p_expand=200.0, p_reduce=50.0,
+                         ring=O.RING_ZT)
+    assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
+    assert (gpu["cur_s"] != 0).any()  # negative coefficients really occur
 
 
 def test_walk_from_loaded_states_bit_identical():
@@ -90,6 +106,23 @@ def test_gpu_brent_check_matches_oracle():
         blobs.append(O.naive_blob(*n))
         want.append(True)
     got = fg.verify(blobs)
+    assert list(got) == want
+
+
+def test_gpu_integer_brent_check_matches_oracle():
+    rng = np.random.default_rng(1)
+    blobs, want = [], []
+    for sch in (S.strassen(), S.strassen_squared()):
+        b = S.to_blob(*sch)
+        blobs.append(b)
+        want.append(True)
+        for _ in range(20):
+            c = b.copy()
+            pos = int(rng.integers(4, len(c)))
+            c[pos] = int(rng.choice([x for x in (-1, 0, 1) if x != int(c[pos])]))
+            blobs.append(c)
+            want.append(O.validate(c, O.RING_ZT) == 1)
+    got = fg.verify(blobs, ring=fg.RING_ZT)
     assert list(got) == want
 
 
