@@ -38,7 +38,24 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """A plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE readings
+    convert the walk kernel's counters into bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}))
 
 
 def cpu_baseline(n, plus_iterations, target_seconds):
@@ -66,6 +83,9 @@ def cpu_baseline(n, plus_iterations, target_seconds):
 
 def main():
     a = parse()
+    if a.calibrate_copy:
+        calibrate_copy(a.calibrate_copy)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
