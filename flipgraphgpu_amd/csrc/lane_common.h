@@ -14,6 +14,11 @@
 //            whose component-c factors are equal to term t's,
 //   H[c][t]  number of class members with an index above t.
 //
+// Both byte planes hold terms 4q..4q+3 of a lane in one dword, dwords
+// lane-interleaved, so a wave touching any term of each lane's walker hits
+// 64 different banks.  The best scheme found so far is copied to HBM by the
+// whole wave at once (one walker per pass, contiguous per walker).
+//
 // The reference keeps the partner pairs as an explicit list per component
 // (FlipSet, src/entities/flip_set.cu) and rebuilds or scans it on every
 // flip.  Here the unordered pair (t1 < t2) at position r of the fresh-list
@@ -108,7 +113,7 @@ struct LaneBase {
     // H packs terms 4q..4q+3 of a lane into one dword: byte (t & 3) of word q.
     FGK_HD uint8_t &h(int c, int t) { return H[((c * hw + (t >> 2)) * WG) * 4 + (t & 3)]; }
     FGK_HD uint32_t &hword(int c, int q) { return reinterpret_cast<uint32_t *>(H)[(c * hw + q) * WG]; }
-    FGK_HD uint8_t &nx(int c, int t) { return N[(c * maxm + t) * WG]; }
+    FGK_HD uint8_t &nx(int c, int t) { return N[((c * hw + (t >> 2)) * WG) * 4 + (t & 3)]; }
     FGK_HD void fail(int32_t code) { fgk_atomic_or(err, code); }
 
     // Component-indexed scalars are selected arithmetically: a select
@@ -217,29 +222,50 @@ struct LaneBase {
         addS(c, -cnt);
     }
 
-    // Singleton term x joins the component-c class containing y0.
-    FGK_HD void join(int c, int x, int y0) {
-        int y = y0, cnt = 0, gt = 0, p = -1, mx = -1;
+    // Walk the component-c class containing y0 that singleton term x is about
+    // to join: raise H of the members below x, collect what the join needs,
+    // and let `rule(y)` test each member for a reduction (0 = none); the
+    // lowest member with a rule is returned in (best_t, best_rule).  H is
+    // touched before the caller knows whether a reduction follows, which is
+    // harmless: every reduction rebuilds the classes.
+    struct JoinInfo {
+        int cnt, gt, p, mx;
+    };
+    template <class Rule>
+    FGK_HD void class_walk(int c, int x, int y0, JoinInfo &ji, Rule rule, int &best_t, int &best_rule) {
+        int y = y0;
+        ji.cnt = 0;
+        ji.gt = 0;
+        ji.p = -1;
+        ji.mx = -1;
         do {
-            ++cnt;
+            ++ji.cnt;
             if (y < x) {
                 h(c, y) = (uint8_t)(h(c, y) + 1);
-                if (y > p) p = y;
+                if (y > ji.p) ji.p = y;
             } else {
-                ++gt;
+                ++ji.gt;
             }
-            if (y > mx) mx = y;
+            if (y > ji.mx) ji.mx = y;
+            const int r = rule(y);
+            if (r && y < best_t) {
+                best_t = y;
+                best_rule = r;
+            }
             y = nx(c, y);
-            if (cnt > m) {
+            if (ji.cnt > m) {
                 fail(kErrWalk);
                 return;
             }
         } while (y != y0);
-        if (p < 0) p = mx;
-        h(c, x) = (uint8_t)gt;
+    }
+    // Link singleton x into the class walked by class_walk.
+    FGK_HD void link_join(int c, int x, const JoinInfo &ji) {
+        const int p = ji.p < 0 ? ji.mx : ji.p;
+        h(c, x) = (uint8_t)ji.gt;
         nx(c, x) = nx(c, p);
         nx(c, p) = (uint8_t)x;
-        addS(c, cnt);
+        addS(c, ji.cnt);
     }
 
     // The unordered pair at position r (< S0 + S1 + S2) of the fresh-list
@@ -306,6 +332,10 @@ struct LaneBase {
     }
 };
 
+// A uniform draw in [0, size): floor(x * size / 2^32) (one v_mul_hi_u32; the
+// reference's curand() % size needs an integer division).
+FGK_HD uint32_t draw_below(uint32_t x, uint32_t size) { return (uint32_t)(((uint64_t)x * size) >> 32); }
+
 // Orientation of a flip in component c: i = c, (j, k) = the other two in
 // increasing order, swapped by coin bit 0 (scheme_z2.cu:988-1016).
 FGK_HD void orientation(int c, uint32_t coin, int &j, int &k) {
@@ -316,6 +346,44 @@ FGK_HD void orientation(int c, uint32_t coin, int &j, int &k) {
         j = k;
         k = t;
     }
+}
+
+// Maximum of v over the wave (the lane's own value on the host).
+FGK_HD int wave_max(int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    for (int o = 32; o > 0; o >>= 1) {
+        const int u = __shfl_xor(v, o);
+        v = u > v ? u : v;
+    }
+#endif
+    return v;
+}
+
+// copyTo(schemesBest[idx]) (flip_graph.cu:547): the current scheme of every
+// lane with `want` goes to its walker's slot of A.best, laid out per walker
+// as [c][t] (max_m words per component).  On the GPU the whole wave copies
+// one walker per pass with coalesced stores; the host emulation copies its
+// own lane.
+template <class Lane>
+FGK_HD void copy_best(Lane &s, const WalkArgs &A, bool want, int lane, int w) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t mask = __ballot(want);
+    while (mask) {
+        const int src = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const int ms = __shfl(s.m, src);
+        const uint32_t *Fs = s.F - lane + src;
+        uint32_t *dst = A.best + (size_t)(w - lane + src) * 3 * A.max_m;
+        for (int c = 0; c < 3; ++c)
+            for (int t = lane; t < ms; t += WG) dst[c * A.max_m + t] = Fs[(c * A.max_m + t) * WG];
+    }
+#else
+    if (want) {
+        uint32_t *dst = A.best + (size_t)w * 3 * A.max_m;
+        for (int c = 0; c < 3; ++c)
+            for (int t = 0; t < s.m; ++t) dst[c * A.max_m + t] = s.f(c, t);
+    }
+#endif
 }
 
 // The walk of one lane (randomWalkKernel body, src/entities/flip_graph.cu:520-573).
@@ -330,7 +398,7 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w) {
     const int hw = h_words(maxm);
     s.F = reinterpret_cast<uint32_t *>(lds) + lane;
     s.H = lds + (size_t)3 * maxm * WG * sizeof(uint32_t) + (size_t)lane * 4;
-    s.N = lds + (size_t)3 * maxm * WG * sizeof(uint32_t) + (size_t)3 * hw * WG * 4 + lane;
+    s.N = lds + (size_t)3 * maxm * WG * sizeof(uint32_t) + (size_t)3 * hw * WG * 4 + (size_t)lane * 4;
     s.lane_extra(lds + walk_lds_base_bytes(maxm), lane);
     s.err = A.err;
     s.maxm = maxm;
@@ -373,31 +441,40 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w) {
         if (A.random_iterations) iters = 1 + (int)(s.aux32() % (uint32_t)A.max_iterations);
     }
     const bool check_reduce = A.check_reduce != 0;
+    // every lane runs the same number of loop trips so the wave stays
+    // converged at the cooperative best copy
+    const int trips = wave_max(iters);
 
-    for (int iter = 0; iter < iters; ++iter) {
-        uint32_t b0, b1, b2, b3;
-        philox10((uint32_t)it, (uint32_t)(it >> 32), (uint32_t)w, 0u, s.k0, s.k1, b0, b1, b2, b3);
-        ++it;
-        const int rank = s.m;
-        if (!s.try_flip(b0, b1, check_reduce)) {  // flip_graph.cu:535-537
-            s.try_expand(1 + (int)(b1 % 2u));
-            continue;
+    for (int iter = 0; iter < trips; ++iter) {
+        bool flipped = false, copy = false;
+        uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+        if (iter < iters) {
+            philox10((uint32_t)it, (uint32_t)(it >> 32), (uint32_t)w, 0u, s.k0, s.k1, b0, b1, b2, b3);
+            ++it;
+            const int rank = s.m;
+            if (!s.try_flip(b0, b1, check_reduce)) {  // flip_graph.cu:535-537
+                s.try_expand(1 + (int)(b1 % 2u));
+            } else {
+                flipped = true;
+                ++total_flips;
+                sum_m += (uint64_t)rank;
+                if (s.m < rank) flips_count = 0;
+                ++flips_count;
+                if (s.m < best_rank || (s.m == best_rank && b2 < A.thr_best_eq)) {  // :545-548
+                    best_rank = s.m;
+                    best_m = s.m;
+                    copy = true;
+                }
+            }
         }
-        ++total_flips;
-        sum_m += (uint64_t)rank;
-        if (s.m < rank) flips_count = 0;
-        ++flips_count;
-        if (s.m < best_rank || (s.m == best_rank && b2 < A.thr_best_eq)) {  // :545-548
-            best_rank = s.m;
-            best_m = s.m;
-            for (int c = 0; c < 3; ++c)
-                for (int t = 0; t < s.m; ++t) A.best[((size_t)c * maxm + t) * W + w] = s.f(c, t);
+        copy_best(s, A, copy, lane, w);
+        if (flipped) {
+            if (A.use_reduce && s.aux32() < A.thr_reduce) s.try_reduce();  // :550-551
+            if (A.use_expand && b3 < A.thr_expand && s.m <= best_m + 2) {   // :553-556
+                if (s.try_expand(1 + (int)(s.aux32() % 2u))) flips_count = 0;
+            }
+            if (flips_count >= A.plus_iterations && s.try_plus()) flips_count = 0;  // :564-565
         }
-        if (A.use_reduce && s.aux32() < A.thr_reduce) s.try_reduce();  // :550-551
-        if (A.use_expand && b3 < A.thr_expand && s.m <= best_m + 2) {   // :553-556
-            if (s.try_expand(1 + (int)(s.aux32() % 2u))) flips_count = 0;
-        }
-        if (flips_count >= A.plus_iterations && s.try_plus()) flips_count = 0;  // :564-565
     }
 
     if (active) {

@@ -165,7 +165,7 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, int n1, int n2
     ctx->W = W;
     ctx->maxm = maxm;
     const size_t plane = (size_t)3 * maxm * W;
-    std::vector<uint32_t> words(plane, 0);
+    std::vector<uint32_t> words(plane, 0), best_words(plane, 0);
     std::vector<int32_t> m(W), best_m(W), best_rank(W), zeros(W, 0), aidx(W, 4), fmt((size_t)4 * W, 0);
     std::vector<uint64_t> z64(W, 0);
 
@@ -179,7 +179,11 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, int n1, int n2
         fmt[4 * w + 1] = s.n[1];
         fmt[4 * w + 2] = s.n[2];
         for (int c = 0; c < 3; ++c)
-            for (int t = 0; t < s.m; ++t) words[((size_t)c * maxm + t) * W + w] = pack_word(ring, s.v[c][t], s.s[c][t]);
+            for (int t = 0; t < s.m; ++t) {
+                const uint32_t x = pack_word(ring, s.v[c][t], s.s[c][t]);
+                words[((size_t)c * maxm + t) * W + w] = x;
+                best_words[((size_t)w * 3 + c) * maxm + t] = x;
+            }
         const std::string key = fg::format_key(s.n[0], s.n[1], s.n[2]);
         auto f = best_by_key.find(key);
         if (f == best_by_key.end() || s.m < f->second) best_by_key[key] = s.m;
@@ -204,7 +208,7 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, int n1, int n2
     HIPCHK(dalloc(&ctx->total_flips, W));
     HIPCHK(dalloc(&ctx->sum_m, W));
     HIPCHK(hipMemcpy(ctx->cur, words.data(), plane * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->best, words.data(), plane * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->best, best_words.data(), plane * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->m, m.data(), W * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->best_m, best_m.data(), W * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->best_rank, best_rank.data(), W * 4, hipMemcpyHostToDevice));
@@ -408,7 +412,7 @@ extern "C" int fg_read_walkers(fg_ctx *ctx, int begin, int count, fg_walker_reco
                 r.cur_s[c][t] = word_signs(ring, x);
             }
             for (int t = 0; t < r.best_m; ++t) {
-                const uint32_t x = best[((size_t)c * maxm + t) * W + w];
+                const uint32_t x = best[((size_t)w * 3 + c) * maxm + t];
                 r.best_v[c][t] = word_values(ring, x);
                 r.best_s[c][t] = word_signs(ring, x);
             }
@@ -463,7 +467,7 @@ extern "C" int fg_write_walkers(fg_ctx *ctx, int begin, int count, const fg_walk
         for (int c = 0; c < 3; ++c) {
             for (int t = 0; t < r.m; ++t) cur[((size_t)c * maxm + t) * W + w] = pack_word(ring, r.cur_v[c][t], r.cur_s[c][t]);
             for (int t = 0; t < r.best_m; ++t)
-                best[((size_t)c * maxm + t) * W + w] = pack_word(ring, r.best_v[c][t], r.best_s[c][t]);
+                best[((size_t)w * 3 + c) * maxm + t] = pack_word(ring, r.best_v[c][t], r.best_s[c][t]);
         }
     }
     HIPCHK(hipMemcpy(ctx->cur, cur.data(), plane * 4, hipMemcpyHostToDevice));
@@ -489,9 +493,11 @@ int walker_scheme(fg_ctx *ctx, int w, int which, fg::Scheme *out) {
     HIPCHK(hipMemcpy(&m, (which ? ctx->best_m : ctx->m) + w, 4, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(fmt, ctx->fmt + 4 * w, 16, hipMemcpyDeviceToHost));
     std::vector<uint32_t> col((size_t)3 * maxm);
-    // one strided column of the plane: 3 * maxm words, pitch W words
-    HIPCHK(hipMemcpy2D(col.data(), 4, (which ? ctx->best : ctx->cur) + w, (size_t)W * 4, 4, (size_t)3 * maxm,
-                       hipMemcpyDeviceToHost));
+    if (which) {  // best schemes are contiguous per walker
+        HIPCHK(hipMemcpy(col.data(), ctx->best + (size_t)w * 3 * maxm, col.size() * 4, hipMemcpyDeviceToHost));
+    } else {  // one strided column of the current-scheme plane: pitch W words
+        HIPCHK(hipMemcpy2D(col.data(), 4, ctx->cur + w, (size_t)W * 4, 4, (size_t)3 * maxm, hipMemcpyDeviceToHost));
+    }
     fg::Scheme s;
     s.ring = ctx->cfg.ring;
     for (int c = 0; c < 3; ++c) s.n[c] = fmt[c];

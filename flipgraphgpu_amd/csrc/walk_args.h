@@ -16,7 +16,7 @@ constexpr int kWalkersPerGroup = 64;
 struct WalkArgs {
     uint32_t *cur;        // Z2 factor masks / ZT coefficient magnitudes
     uint32_t *cur_s;      // ZT signs (null for Z2)
-    uint32_t *best;
+    uint32_t *best;       // best schemes, per walker [c][t]: best[(w * 3 + c) * max_m + t]
     uint32_t *best_s;
     int32_t *m;
     int32_t *best_m;
@@ -56,10 +56,10 @@ constexpr int32_t kErrCapacity = 4; // a term would exceed the LDS capacity
 constexpr int kHTermsPad = 16;
 __host__ __device__ inline int h_words(int max_m) { return ((max_m + kHTermsPad - 1) / kHTermsPad) * (kHTermsPad / 4); }
 
-// LDS bytes per workgroup of the common planes: 32-bit factor words F,
-// packed H words and N bytes, for every lane.
+// LDS bytes per workgroup of the common planes: 32-bit factor words F and
+// the packed H and N words, for every lane.
 __host__ __device__ inline size_t walk_lds_base_bytes(int max_m) {
-    return (size_t)kWalkersPerGroup * (3 * (size_t)max_m * 4 + 3 * (size_t)h_words(max_m) * 4 + 3 * (size_t)max_m);
+    return (size_t)kWalkersPerGroup * (3 * (size_t)max_m * 4 + 2 * 3 * (size_t)h_words(max_m) * 4);
 }
 __host__ __device__ inline size_t z2_lds_bytes(int max_m) { return walk_lds_base_bytes(max_m); }
 

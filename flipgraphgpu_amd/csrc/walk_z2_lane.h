@@ -74,37 +74,30 @@ struct Z2Lane : LaneBase {
             anyA = a == A ? t : anyA;
             anyB = b == B ? t : anyB;
         }
-        if (check_reduce && (anyA >= 0 || anyB >= 0)) {
-            const uint32_t fi = f(i, i1);
-            int tA = 1 << 30, cA = 0, tB = 1 << 30, cB = 0;
-            if (anyA >= 0) {
-                int y = anyA, g = 0;
-                do {
-                    const int cnd = f(i, y) == fi ? 1 : (f(k, y) == fk1 ? 2 : 0);
-                    if (cnd && y < tA) { tA = y; cA = cnd; }
-                    y = nx(j, y);
-                } while (y != anyA && ++g <= m);
-            }
-            if (anyB >= 0) {
-                int y = anyB, g = 0;
-                do {
-                    const int cnd = f(i, y) == fi ? 1 : (f(j, y) == fj2 ? 2 : 0);
-                    if (cnd && y < tB) { tB = y; cB = cnd; }
-                    y = nx(k, y);
-                } while (y != anyB && ++g <= m);
-            }
-            if (cA || cB) {
-                f(j, i1) = A;
-                f(k, i2) = B;
-                if (cA && tA <= tB) reduce(cA == 1 ? k : i, tA, i1);
-                else reduce(cB == 1 ? j : i, tB, i2);
-                return;
-            }
+        // one walk per new class: reduction candidates (reference order:
+        // lowest member first, index1's side before index2's) and join data
+        const uint32_t fi = f(i, i1);
+        int tA = 1 << 30, cA = 0, tB = 1 << 30, cB = 0;
+        JoinInfo jA, jB;
+        if (anyA >= 0)
+            class_walk(j, i1, anyA, jA,
+                       [&](int y) { return !check_reduce ? 0 : (f(i, y) == fi ? 1 : (f(k, y) == fk1 ? 2 : 0)); },
+                       tA, cA);
+        if (anyB >= 0)
+            class_walk(k, i2, anyB, jB,
+                       [&](int y) { return !check_reduce ? 0 : (f(i, y) == fi ? 1 : (f(j, y) == fj2 ? 2 : 0)); },
+                       tB, cB);
+        if (cA || cB) {
+            f(j, i1) = A;
+            f(k, i2) = B;
+            if (cA && tA <= tB) reduce(cA == 1 ? k : i, tA, i1);
+            else reduce(cB == 1 ? j : i, tB, i2);
+            return;
         }
         leave(j, i1);
-        if (anyA >= 0) join(j, i1, anyA);
+        if (anyA >= 0) link_join(j, i1, jA);
         leave(k, i2);
-        if (anyB >= 0) join(k, i2, anyB);
+        if (anyB >= 0) link_join(k, i2, jB);
         f(j, i1) = A;
         f(k, i2) = B;
     }
@@ -237,7 +230,7 @@ struct Z2Lane : LaneBase {
         const uint32_t total = S0 + S1 + S2;
         if (!total) return false;
         int c, t1, t2, j, k;
-        if (!select_pair(r % total, c, t1, t2)) return false;
+        if (!select_pair(draw_below(r, total), c, t1, t2)) return false;
         orientation(c, coin, j, k);
         if (coin & 2u) {
             const int t = t1;

@@ -169,37 +169,28 @@ struct ZTLane : LaneBase {
             anyA = a == A ? t : anyA;
             anyB = b == B ? t : anyB;
         }
-        if (check_reduce && (anyA >= 0 || anyB >= 0)) {
-            const uint32_t fi = f(i, i1);  // == f(i, i2): the pair shares component i
-            int tA = 1 << 30, rA = 0, tB = 1 << 30, rB = 0;
-            if (anyA >= 0) {
-                int y = anyA, g = 0;
-                do {
-                    const int rule = reduce_rule(i, j, k, y, fi, fk1);
-                    if (rule && y < tA) { tA = y; rA = rule; }
-                    y = nx(j, y);
-                } while (y != anyA && ++g <= m);
-            }
-            if (anyB >= 0) {
-                int y = anyB, g = 0;
-                do {
-                    const int rule = reduce_rule(i, k, j, y, fi, fj2);
-                    if (rule && y < tB) { tB = y; rB = rule; }
-                    y = nx(k, y);
-                } while (y != anyB && ++g <= m);
-            }
-            if (rA || rB) {
-                f(j, i1) = A;
-                f(k, i2) = B;
-                if (rA && tA <= tB) apply_rule(rA, i, k, tA, i1);
-                else apply_rule(rB, i, j, tB, i2);
-                return;
-            }
+        // one walk per new class: reduction candidates (reference order:
+        // lowest member first, index1's side before index2's) and join data
+        const uint32_t fi = f(i, i1);  // == f(i, i2): the pair shares component i
+        int tA = 1 << 30, rA = 0, tB = 1 << 30, rB = 0;
+        JoinInfo jA, jB;
+        if (anyA >= 0)
+            class_walk(j, i1, anyA, jA, [&](int y) { return check_reduce ? reduce_rule(i, j, k, y, fi, fk1) : 0; },
+                       tA, rA);
+        if (anyB >= 0)
+            class_walk(k, i2, anyB, jB, [&](int y) { return check_reduce ? reduce_rule(i, k, j, y, fi, fj2) : 0; },
+                       tB, rB);
+        if (rA || rB) {
+            f(j, i1) = A;
+            f(k, i2) = B;
+            if (rA && tA <= tB) apply_rule(rA, i, k, tA, i1);
+            else apply_rule(rB, i, j, tB, i2);
+            return;
         }
         leave(j, i1);
-        if (anyA >= 0) join(j, i1, anyA);
+        if (anyA >= 0) link_join(j, i1, jA);
         leave(k, i2);
-        if (anyB >= 0) join(k, i2, anyB);
+        if (anyB >= 0) link_join(k, i2, jB);
         f(j, i1) = A;
         f(k, i2) = B;
     }
@@ -346,13 +337,13 @@ struct ZTLane : LaneBase {
         int cap = visit_cap < (int)total ? visit_cap : (int)total;
         if (cap > kZtMaxVisits) cap = kZtMaxVisits;
         for (int q = 0; q < cap; ++q) {
-            uint32_t cand = (q == 0 ? r : aux32()) % total;
+            uint32_t cand = draw_below(q == 0 ? r : aux32(), total);
             for (int redraws = 0;;) {
                 bool seen = false;
                 for (int o = 0; o < q; ++o) seen |= visited(o) == cand;
                 if (!seen) break;
                 if (++redraws > kZtRedraws) return false;
-                cand = aux32() % total;
+                cand = draw_below(aux32(), total);
             }
             visited(q) = (uint16_t)cand;
             const uint32_t cn = q == 0 ? coin : aux32();

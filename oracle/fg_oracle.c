@@ -71,6 +71,11 @@ uint32_t fgo_rng_aux32(fgo_rng *r) {
     return r->abuf[r->aidx++];
 }
 
+/* Pair draws map a 32-bit word x onto [0, size) as floor(x * size / 2^32)
+ * (multiply-high) where the reference takes curand() % size
+ * (src/schemes/scheme_z2.cu:988); both are uniform up to size / 2^32. */
+static uint32_t draw_below(uint32_t x, uint32_t size) { return (uint32_t)(((uint64_t)x * size) >> 32); }
+
 /* randint(a, b) = a + rand % (b - a + 1)   (src/utils/random.cuh:5-7) */
 static int aux_randint(fgo_rng *r, int a, int b) {
     return a + (int)(fgo_rng_aux32(r) % (uint32_t)(b - a + 1));
@@ -346,37 +351,52 @@ void fgo_reduce(fgo_scheme *s, int i, int i1, int i2, int sub) {
     fgo_init_flips(s);
 }
 
-/* tryReduce (scheme_z2.cu:1127-1154, scheme_integer.cu:1062-1101).  The
- * lists are walked in stored order; they are freshly built whenever this is
- * called from the walk's integer flip and from input reading. */
-int fgo_try_reduce(fgo_scheme *s) {
-    if (s->ring == FGO_RING_Z2) {
-        for (int q = 0; q < s->flips[0].size; q++) {
-            int a = (int)(s->flips[0].pairs[q] >> 16), b = (int)(s->flips[0].pairs[q] & 0xFFFF);
-            if (s->v[1][a] == s->v[1][b]) { fgo_reduce(s, 2, a, b, 0); return 1; }
-            if (s->v[2][a] == s->v[2][b]) { fgo_reduce(s, 1, a, b, 0); return 1; }
-        }
-        for (int q = 0; q < s->flips[1].size; q++) {
-            int a = (int)(s->flips[1].pairs[q] >> 16), b = (int)(s->flips[1].pairs[q] & 0xFFFF);
-            if (s->v[2][a] == s->v[2][b]) { fgo_reduce(s, 0, a, b, 0); return 1; }
-        }
+/* tryReduce (scheme_z2.cu:1127-1154, scheme_integer.cu:1062-1101) over the
+ * component-0 pairs, then the component-1 pairs.  The pair lists are taken
+ * as stored (REFERENCE order) or in fresh-list order, index1 < index2
+ * (CANONICAL).  For the integer ring the orientation of a stored pair matters:
+ * limitSub / limitSum with the sign-canonical test are not symmetric. */
+static int try_reduce_pair_z2(fgo_scheme *s, int comp, int a, int b) {
+    if (comp == 0) {
+        if (s->v[1][a] == s->v[1][b]) { fgo_reduce(s, 2, a, b, 0); return 1; }
+        if (s->v[2][a] == s->v[2][b]) { fgo_reduce(s, 1, a, b, 0); return 1; }
         return 0;
     }
-    for (int q = 0; q < s->flips[0].size; q++) {
-        int a = (int)(s->flips[0].pairs[q] >> 16), b = (int)(s->flips[0].pairs[q] & 0xFFFF);
+    if (s->v[2][a] == s->v[2][b]) { fgo_reduce(s, 0, a, b, 0); return 1; }
+    return 0;
+}
+
+static int try_reduce_pair_zt(fgo_scheme *s, int comp, int a, int b) {
+    if (comp == 0) {
         if (tv_eq(tv_get(s, 1, a), tv_get(s, 1, b)) && tv_limit_sum(tv_get(s, 2, a), tv_get(s, 2, b), 0)) { fgo_reduce(s, 2, a, b, 0); return 1; }
         int c2 = tv_cmp(tv_get(s, 2, a), tv_get(s, 2, b));
         if (c2 == 1 && tv_limit_sum(tv_get(s, 1, a), tv_get(s, 1, b), 1)) { fgo_reduce(s, 1, a, b, 0); return 1; }
         if (c2 == -1 && tv_limit_sub(tv_get(s, 1, a), tv_get(s, 1, b), 1)) { fgo_reduce(s, 1, a, b, 1); return 1; }
+        return 0;
     }
-    for (int q = 0; q < s->flips[1].size; q++) {
-        int a = (int)(s->flips[1].pairs[q] >> 16), b = (int)(s->flips[1].pairs[q] & 0xFFFF);
-        int c2 = tv_cmp(tv_get(s, 2, a), tv_get(s, 2, b));
-        if (c2 == 1 && tv_limit_sum(tv_get(s, 0, a), tv_get(s, 0, b), 1)) { fgo_reduce(s, 0, a, b, 0); return 1; }
-        if (c2 == -1 && tv_limit_sub(tv_get(s, 0, a), tv_get(s, 0, b), 1)) { fgo_reduce(s, 0, a, b, 1); return 1; }
+    int c2 = tv_cmp(tv_get(s, 2, a), tv_get(s, 2, b));
+    if (c2 == 1 && tv_limit_sum(tv_get(s, 0, a), tv_get(s, 0, b), 1)) { fgo_reduce(s, 0, a, b, 0); return 1; }
+    if (c2 == -1 && tv_limit_sub(tv_get(s, 0, a), tv_get(s, 0, b), 1)) { fgo_reduce(s, 0, a, b, 1); return 1; }
+    return 0;
+}
+
+static int try_reduce_mode(fgo_scheme *s, int mode) {
+    int (*pair)(fgo_scheme *, int, int, int) = s->ring == FGO_RING_Z2 ? try_reduce_pair_z2 : try_reduce_pair_zt;
+    for (int comp = 0; comp < 2; comp++) {
+        if (mode == FGO_SELECT_REFERENCE) {
+            for (int q = 0; q < s->flips[comp].size; q++)
+                if (pair(s, comp, (int)(s->flips[comp].pairs[q] >> 16), (int)(s->flips[comp].pairs[q] & 0xFFFF))) return 1;
+        } else {
+            for (int a = 0; a < s->m; a++)
+                for (int b = a + 1; b < s->m; b++)
+                    if (fac_eq(s, comp, a, b) && pair(s, comp, a, b)) return 1;
+        }
     }
     return 0;
 }
+
+/* On a freshly built list both orders coincide. */
+int fgo_try_reduce(fgo_scheme *s) { return try_reduce_mode(s, FGO_SELECT_REFERENCE); }
 
 /* ------------------------------------------------------------------ */
 /* flip                                                                */
@@ -544,13 +564,13 @@ static void select_pair(const fgo_scheme *s, int mode, uint32_t r, int *c, int *
     else fgo_pick_canonical(s, r, c, t1, t2);
 }
 
-/* Z2 tryFlip (scheme_z2.cu:982-1027): b[0] picks the pair, bit 0 of b[1]
+/* Z2 tryFlip (scheme_z2.cu:982-1027): b[0] picks the pair (multiply-high), bit 0 of b[1]
  * swaps j/k, bit 1 swaps index1/index2. */
 static int try_flip_z2(fgo_scheme *s, const uint32_t b[4], const fgo_walk_params *p) {
     int size = total_pairs(s, p->select_mode);
     if (!size) return 0;
     int c, t1, t2, i, j, k;
-    select_pair(s, p->select_mode, b[0] % (uint32_t)size, &c, &t1, &t2);
+    select_pair(s, p->select_mode, draw_below(b[0], (uint32_t)size), &c, &t1, &t2);
     pick_orientation(c, (int)b[1], &i, &j, &k);
     if (b[1] & 2) { int t = t1; t1 = t2; t2 = t; }
     flip_z2(s, i, j, k, t1, t2, p->check_reduce);
@@ -576,14 +596,14 @@ static int try_flip_zt(fgo_scheme *s, const uint32_t b[4], fgo_rng *r, const fgo
     int visited[FGO_ZT_MAX_VISITS];
     int nv = 0;
     for (int q = 0; q < cap; q++) {
-        int cand = (int)((q == 0 ? b[0] : fgo_rng_aux32(r)) % (uint32_t)size);
+        int cand = (int)draw_below(q == 0 ? b[0] : fgo_rng_aux32(r), (uint32_t)size);
         int redraws = 0;
         for (;;) {
             int seen = 0;
             for (int o = 0; o < nv; o++) seen |= visited[o] == cand;
             if (!seen) break;
             if (++redraws > FGO_ZT_REDRAWS) return 0;
-            cand = (int)(fgo_rng_aux32(r) % (uint32_t)size);
+            cand = (int)draw_below(fgo_rng_aux32(r), (uint32_t)size);
         }
         visited[nv++] = cand;
         uint32_t coin = q == 0 ? b[1] : fgo_rng_aux32(r);
@@ -783,7 +803,7 @@ void fgo_walk(fgo_walker *w, const fgo_walk_params *p) {
             w->best_rank = s->m;
             fgo_copy(&w->best, s, 0);
         }
-        if (p->use_reduce && fgo_rng_aux32(&w->rng) < p->thr_reduce) fgo_try_reduce(s); /* :550-551 */
+        if (p->use_reduce && fgo_rng_aux32(&w->rng) < p->thr_reduce) try_reduce_mode(s, p->select_mode); /* :550-551 */
         if (p->use_expand && b[3] < p->thr_expand && s->m <= w->best.m + 2) { /* :553-556 */
             if (try_expand(s, aux_randint(&w->rng, 1, 2), &w->rng)) w->flips_count = 0;
         }

@@ -36,7 +36,7 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
         it[w] = r.it; aux[w] = r.aux; tf[w] = r.total_flips; sm[w] = r.sum_m;
         for (int c = 0; c < 3; ++c) {
             for (int t = 0; t < r.m; ++t) cur[((size_t)c * maxm + t) * W + w] = pack(ring, r.cur_v[c][t], r.cur_s[c][t]);
-            for (int t = 0; t < r.best_m; ++t) best[((size_t)c * maxm + t) * W + w] = pack(ring, r.best_v[c][t], r.best_s[c][t]);
+            for (int t = 0; t < r.best_m; ++t) best[((size_t)w * 3 + c) * maxm + t] = pack(ring, r.best_v[c][t], r.best_s[c][t]);
         }
     }
     int32_t err = 0;
@@ -72,7 +72,7 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                 r.cur_s[c][t] = ring ? (x >> 16) : 0;
             }
             for (int t = 0; t < r.best_m; ++t) {
-                const uint32_t x = best[((size_t)c * maxm + t) * W + w];
+                const uint32_t x = best[((size_t)w * 3 + c) * maxm + t];
                 r.best_v[c][t] = ring ? (x & 0xFFFFu) : x;
                 r.best_s[c][t] = ring ? (x >> 16) : 0;
             }
