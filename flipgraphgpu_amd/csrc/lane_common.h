@@ -3,10 +3,14 @@
 // __host__ __device__; lanes never communicate, so running lanes one after
 // another on the host reproduces a kernel exactly.
 //
-// One lane owns one walker (the reference's randomWalkKernel also runs one
-// thread per scheme, src/entities/flip_graph.cu:520-573).  The walker's
-// scheme lives in LDS for the whole launch as three factor planes
-// F[c][t][lane] (one 32-bit word per factor: a Z2 mask, or for the integer
+// An adjacent lane pair owns one walker (the reference's randomWalkKernel runs
+// one thread per scheme, src/entities/flip_graph.cu:520-573).  Both lanes of
+// the pair run the same instruction stream on the same walker state; the
+// only work they split is the two halves of a flip (the index1 side with
+// component j, the index2 side with component k), which exchange their
+// results with one lane shuffle.  The walker's scheme lives in LDS for the
+// whole launch as three factor planes F[c][t][slot] (one 32-bit word per
+// factor: a Z2 mask, or for the integer
 // ring the coefficient magnitudes in bits 0-15 and their signs in bits
 // 16-31); two byte planes describe the flip partners:
 //
@@ -14,9 +18,9 @@
 //            whose component-c factors are equal to term t's,
 //   H[c][t]  number of class members with an index above t.
 //
-// Both byte planes hold terms 4q..4q+3 of a lane in one dword, dwords
-// lane-interleaved, so a wave touching any term of each lane's walker hits
-// 64 different banks.  The best scheme found so far is copied to HBM by the
+// Both byte planes hold terms 4q..4q+3 of a walker in one dword, dwords
+// slot-interleaved, so a wave touching any term of each walker hits 32
+// different banks (the two lanes of a pair share an address).  The best scheme found so far is copied to HBM by the
 // whole wave at once (one walker per pass, contiguous per walker).
 //
 // The reference keeps the partner pairs as an explicit list per component
@@ -63,7 +67,7 @@ FGK_HD uint32_t byte_sum(uint32_t w) {
 
 namespace lane {
 
-constexpr int WG = kWalkersPerGroup;
+constexpr int WS = kWalkersPerGroup;  // LDS stride between consecutive terms of one walker
 constexpr uint32_t kPM0 = 0xD2511F53u, kPM1 = 0xCD9E8D57u;
 constexpr uint32_t kPW0 = 0x9E3779B9u, kPW1 = 0xBB67AE85u;
 constexpr int kMaxRank = 350;        // src/config.cuh:10
@@ -90,9 +94,10 @@ FGK_HD void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_
 }
 
 struct LaneBase {
-    uint32_t *F;  // lane-offset plane bases
-    uint8_t *H;   // lane-offset byte base of the packed H plane
+    uint32_t *F;  // walker-slot-offset plane bases
+    uint8_t *H;   // slot-offset byte base of the packed H plane
     uint8_t *N;
+    int side;     // 0: index1 / component-j half of a flip, 1: index2 / component-k half
     int32_t *err;
     int maxm;
     int hw;       // H words per component
@@ -109,11 +114,11 @@ struct LaneBase {
     // Extra per-lane LDS beyond the common planes (none by default).
     FGK_HD void lane_extra(uint8_t *, int) {}
 
-    FGK_HD uint32_t &f(int c, int t) { return F[(c * maxm + t) * WG]; }
-    // H packs terms 4q..4q+3 of a lane into one dword: byte (t & 3) of word q.
-    FGK_HD uint8_t &h(int c, int t) { return H[((c * hw + (t >> 2)) * WG) * 4 + (t & 3)]; }
-    FGK_HD uint32_t &hword(int c, int q) { return reinterpret_cast<uint32_t *>(H)[(c * hw + q) * WG]; }
-    FGK_HD uint8_t &nx(int c, int t) { return N[((c * hw + (t >> 2)) * WG) * 4 + (t & 3)]; }
+    FGK_HD uint32_t &f(int c, int t) { return F[(c * maxm + t) * WS]; }
+    // H packs terms 4q..4q+3 of a walker into one dword: byte (t & 3) of word q.
+    FGK_HD uint8_t &h(int c, int t) { return H[((c * hw + (t >> 2)) * WS) * 4 + (t & 3)]; }
+    FGK_HD uint32_t &hword(int c, int q) { return reinterpret_cast<uint32_t *>(H)[(c * hw + q) * WS]; }
+    FGK_HD uint8_t &nx(int c, int t) { return N[((c * hw + (t >> 2)) * WS) * 4 + (t & 3)]; }
     FGK_HD void fail(int32_t code) { fgk_atomic_or(err, code); }
 
     // Component-indexed scalars are selected arithmetically: a select
@@ -126,6 +131,7 @@ struct LaneBase {
         S1 += (uint32_t)((c == 1 ? 1 : 0) * d);
         S2 += (uint32_t)((c == 2 ? 1 : 0) * d);
     }
+    FGK_HD uint32_t Sc(int c) const { return (c == 0 ? S0 : 0u) + (c == 1 ? S1 : 0u) + (c == 2 ? S2 : 0u); }
     FGK_HD int naive_rank() const { return n0 * n1 * n2; }
 
     // ---- aux RNG stream ----
@@ -348,6 +354,15 @@ FGK_HD void orientation(int c, uint32_t coin, int &j, int &k) {
     }
 }
 
+// The value held by the other lane of this lane's pair (GPU only).
+FGK_HD int pair_exchange(int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __shfl_xor(v, 1);
+#else
+    return v;
+#endif
+}
+
 // Maximum of v over the wave (the lane's own value on the host).
 FGK_HD int wave_max(int v) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -360,22 +375,23 @@ FGK_HD int wave_max(int v) {
 }
 
 // copyTo(schemesBest[idx]) (flip_graph.cu:547): the current scheme of every
-// lane with `want` goes to its walker's slot of A.best, laid out per walker
-// as [c][t] (max_m words per component).  On the GPU the whole wave copies
-// one walker per pass with coalesced stores; the host emulation copies its
-// own lane.
+// walker with `want` goes to its slot of A.best, laid out per walker as
+// [c][t] (max_m words per component).  On the GPU the whole wave copies one
+// walker per pass with coalesced stores; the host emulation copies its own
+// walker.
 template <class Lane>
 FGK_HD void copy_best(Lane &s, const WalkArgs &A, bool want, int lane, int w) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    uint64_t mask = __ballot(want);
+    uint64_t mask = __ballot(want && s.side == 0);
     while (mask) {
         const int src = __builtin_ctzll(mask);
         mask &= mask - 1;
         const int ms = __shfl(s.m, src);
-        const uint32_t *Fs = s.F - lane + src;
-        uint32_t *dst = A.best + (size_t)(w - lane + src) * 3 * A.max_m;
+        const int slot = lane / kLanesPerWalker, src_slot = src / kLanesPerWalker;
+        const uint32_t *Fs = s.F - slot + src_slot;
+        uint32_t *dst = A.best + (size_t)(w - slot + src_slot) * 3 * A.max_m;
         for (int c = 0; c < 3; ++c)
-            for (int t = lane; t < ms; t += WG) dst[c * A.max_m + t] = Fs[(c * A.max_m + t) * WG];
+            for (int t = lane; t < ms; t += kThreadsPerGroup) dst[c * A.max_m + t] = Fs[(c * A.max_m + t) * WS];
     }
 #else
     if (want) {
@@ -387,7 +403,9 @@ FGK_HD void copy_best(Lane &s, const WalkArgs &A, bool want, int lane, int w) {
 }
 
 // The walk of one lane (randomWalkKernel body, src/entities/flip_graph.cu:520-573).
-// `lds` is the workgroup's LDS block, `w` the walker index of this lane.
+// `lds` is the workgroup's LDS block, `lane` the lane in the workgroup
+// (walker slot lane / 2, flip side lane % 2), `w` the walker index.  The host
+// emulation calls it once per walker with an even lane and does both sides.
 template <class Lane>
 FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w) {
     const bool active = w < A.walkers;
@@ -396,10 +414,12 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w) {
 
     Lane s;
     const int hw = h_words(maxm);
-    s.F = reinterpret_cast<uint32_t *>(lds) + lane;
-    s.H = lds + (size_t)3 * maxm * WG * sizeof(uint32_t) + (size_t)lane * 4;
-    s.N = lds + (size_t)3 * maxm * WG * sizeof(uint32_t) + (size_t)3 * hw * WG * 4 + (size_t)lane * 4;
-    s.lane_extra(lds + walk_lds_base_bytes(maxm), lane);
+    const int slot = lane / kLanesPerWalker;
+    s.side = lane % kLanesPerWalker;
+    s.F = reinterpret_cast<uint32_t *>(lds) + slot;
+    s.H = lds + (size_t)3 * maxm * WS * sizeof(uint32_t) + (size_t)slot * 4;
+    s.N = lds + (size_t)3 * maxm * WS * sizeof(uint32_t) + (size_t)3 * hw * WS * 4 + (size_t)slot * 4;
+    s.lane_extra(lds + walk_lds_base_bytes(maxm), slot);
     s.err = A.err;
     s.maxm = maxm;
     s.hw = hw;
@@ -477,7 +497,7 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w) {
         }
     }
 
-    if (active) {
+    if (active && s.side == 0) {
         A.m[w] = s.m;
         A.best_m[w] = best_m;
         A.best_rank[w] = best_rank;

@@ -10,16 +10,16 @@
 
 namespace fgk {
 
-__global__ void __launch_bounds__(kWalkersPerGroup) walk_z2_kernel(WalkArgs A) {
+__global__ void __launch_bounds__(kThreadsPerGroup) walk_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     const int lane = threadIdx.x;
-    lane::walk_lane<lane::Z2Lane>(A, lds, lane, blockIdx.x * kWalkersPerGroup + lane);
+    lane::walk_lane<lane::Z2Lane>(A, lds, lane, blockIdx.x * kWalkersPerGroup + lane / kLanesPerWalker);
 }
 
-__global__ void __launch_bounds__(kWalkersPerGroup) walk_zt_kernel(WalkArgs A) {
+__global__ void __launch_bounds__(kThreadsPerGroup) walk_zt_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     const int lane = threadIdx.x;
-    lane::walk_lane<lane::ZTLane>(A, lds, lane, blockIdx.x * kWalkersPerGroup + lane);
+    lane::walk_lane<lane::ZTLane>(A, lds, lane, blockIdx.x * kWalkersPerGroup + lane / kLanesPerWalker);
 }
 
 // Brent equations (src/schemes/scheme_z2.cu:3-29), bit-sliced: one thread
@@ -91,8 +91,8 @@ hipError_t launch_walk(int ring, const WalkArgs &a, hipStream_t stream) {
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     const int grid = (a.walkers + kWalkersPerGroup - 1) / kWalkersPerGroup;
-    if (ring) hipLaunchKernelGGL(walk_zt_kernel, dim3(grid), dim3(kWalkersPerGroup), lds, stream, a);
-    else hipLaunchKernelGGL(walk_z2_kernel, dim3(grid), dim3(kWalkersPerGroup), lds, stream, a);
+    if (ring) hipLaunchKernelGGL(walk_zt_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
+    else hipLaunchKernelGGL(walk_z2_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
     return hipGetLastError();
 }
 

@@ -7,8 +7,12 @@
 
 namespace fgk {
 
-// Walkers per workgroup: one wave64, one walker per lane.
-constexpr int kWalkersPerGroup = 64;
+// One wave64 per workgroup; each walker is served by an adjacent lane pair
+// (lane 2s handles the index1 side of a flip, lane 2s+1 the index2 side), so
+// a workgroup holds 32 walkers.
+constexpr int kThreadsPerGroup = 64;
+constexpr int kLanesPerWalker = 2;
+constexpr int kWalkersPerGroup = kThreadsPerGroup / kLanesPerWalker;
 
 // Device-resident walker pool.  Factor planes are structure-of-arrays,
 // word(c, t, w) = words[(c * max_m + t) * walkers + w], so a wave touching
@@ -64,7 +68,7 @@ __host__ __device__ inline size_t walk_lds_base_bytes(int max_m) {
 __host__ __device__ inline size_t z2_lds_bytes(int max_m) { return walk_lds_base_bytes(max_m); }
 
 // Integer ring: the candidate pairs one tryFlip visits are remembered in
-// LDS (16-bit indices, kZtMaxVisits per lane).
+// LDS (16-bit indices, kZtMaxVisits per walker).
 constexpr int kZtMaxVisits = 16;
 __host__ __device__ inline size_t zt_lds_bytes(int max_m) {
     return walk_lds_base_bytes(max_m) + (size_t)kWalkersPerGroup * kZtMaxVisits * 2;

@@ -52,9 +52,36 @@ struct Z2Lane : LaneBase {
         rebuild();
     }
 
-    // flip with every involved factor non-zero: the self-matches the
-    // reference excludes with `index != index1/2` cannot occur, so one scan
-    // finds a member of each new class and the classes give the rest.
+    // One half of a flip with every involved factor non-zero: term x of
+    // component mc gets the new factor nv.  The self-matches the reference
+    // excludes with `index != index1/2` cannot occur, so one scan of component
+    // mc finds a member of the new class; one walk of that class collects the
+    // join data and the first reduction candidate (rule 1: component i also
+    // equals fi, rule 2: component oc also equals ov).
+    struct Half {
+        int any, t, rule;
+        JoinInfo ji;
+    };
+    FGK_HD Half flip_half(int mc, int x, uint32_t nv, int i, int oc, uint32_t fi, uint32_t ov, bool check_reduce) {
+        Half hf;
+        hf.any = -1;
+        hf.t = 1 << 30;
+        hf.rule = 0;
+        const uint32_t *Fc = F + mc * maxm * WS;
+#pragma unroll 8
+        for (int t = 0; t < m; ++t) hf.any = Fc[t * WS] == nv ? t : hf.any;
+        if (hf.any >= 0)
+            class_walk(mc, x, hf.any, hf.ji,
+                       [&](int y) { return !check_reduce ? 0 : (f(i, y) == fi ? 1 : (f(oc, y) == ov ? 2 : 0)); },
+                       hf.t, hf.rule);
+        return hf;
+    }
+
+    // flip (scheme_z2.cu:531-577) when every involved factor is non-zero.
+    // Side 0 of the lane pair does the index1 half (component j, new factor
+    // A), side 1 the index2 half (component k, new factor B); the host
+    // emulation does both.  Reduction order as in the reference: the lowest
+    // term first, the index1 half before the index2 half at equal terms.
     FGK_HD void flip(int i, int j, int k, int i1, int i2, bool check_reduce) {
         const uint32_t fj1 = f(j, i1), fj2 = f(j, i2);
         const uint32_t fk1 = f(k, i1), fk2 = f(k, i2);
@@ -64,42 +91,40 @@ struct Z2Lane : LaneBase {
             flip_literal(i, j, k, i1, i2, check_reduce);
             return;
         }
-        uint32_t *Fj = F + j * maxm * WG;
-        uint32_t *Fk = F + k * maxm * WG;
-        int anyA = -1, anyB = -1;
-#pragma unroll 8
-        for (int t = 0; t < m; ++t) {
-            const uint32_t a = Fj[t * WG];
-            const uint32_t b = Fk[t * WG];
-            anyA = a == A ? t : anyA;
-            anyB = b == B ? t : anyB;
-        }
-        // one walk per new class: reduction candidates (reference order:
-        // lowest member first, index1's side before index2's) and join data
         const uint32_t fi = f(i, i1);
-        int tA = 1 << 30, cA = 0, tB = 1 << 30, cB = 0;
-        JoinInfo jA, jB;
-        if (anyA >= 0)
-            class_walk(j, i1, anyA, jA,
-                       [&](int y) { return !check_reduce ? 0 : (f(i, y) == fi ? 1 : (f(k, y) == fk1 ? 2 : 0)); },
-                       tA, cA);
-        if (anyB >= 0)
-            class_walk(k, i2, anyB, jB,
-                       [&](int y) { return !check_reduce ? 0 : (f(i, y) == fi ? 1 : (f(j, y) == fj2 ? 2 : 0)); },
-                       tB, cB);
-        if (cA || cB) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const int mc = side ? k : j, x = side ? i2 : i1, oc = side ? j : k;
+        const uint32_t nv = side ? B : A, ov = side ? fj2 : fk1;
+        const Half mine = flip_half(mc, x, nv, i, oc, fi, ov, check_reduce);
+        const int ot = pair_exchange(mine.t), orule = pair_exchange(mine.rule);
+        const int tA = side ? ot : mine.t, rA = side ? orule : mine.rule;
+        const int tB = side ? mine.t : ot, rB = side ? mine.rule : orule;
+#else
+        const Half ha = flip_half(j, i1, A, i, k, fi, fk1, check_reduce);
+        const Half hb = flip_half(k, i2, B, i, j, fi, fj2, check_reduce);
+        const int tA = ha.t, rA = ha.rule, tB = hb.t, rB = hb.rule;
+#endif
+        if (rA || rB) {
             f(j, i1) = A;
             f(k, i2) = B;
-            if (cA && tA <= tB) reduce(cA == 1 ? k : i, tA, i1);
-            else reduce(cB == 1 ? j : i, tB, i2);
+            if (rA && tA <= tB) reduce(rA == 1 ? k : i, tA, i1);
+            else reduce(rB == 1 ? j : i, tB, i2);
             return;
         }
+#if defined(__HIP_DEVICE_COMPILE__)
+        const uint32_t before = Sc(mc);
+        leave(mc, x);
+        if (mine.any >= 0) link_join(mc, x, mine.ji);
+        f(mc, x) = nv;
+        addS(oc, pair_exchange((int)(Sc(mc) - before)));
+#else
         leave(j, i1);
-        if (anyA >= 0) link_join(j, i1, jA);
-        leave(k, i2);
-        if (anyB >= 0) link_join(k, i2, jB);
+        if (ha.any >= 0) link_join(j, i1, ha.ji);
         f(j, i1) = A;
+        leave(k, i2);
+        if (hb.any >= 0) link_join(k, i2, hb.ji);
         f(k, i2) = B;
+#endif
     }
 
     // plus (scheme_z2.cu:579-612)

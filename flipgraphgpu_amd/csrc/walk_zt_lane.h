@@ -62,7 +62,7 @@ struct ZTLane : LaneBase {
     uint16_t *V;  // lane-offset base of the visited-candidate list
 
     FGK_HD void lane_extra(uint8_t *base, int lane) { V = reinterpret_cast<uint16_t *>(base) + lane; }
-    FGK_HD uint16_t &visited(int q) { return V[q * WG]; }
+    FGK_HD uint16_t &visited(int q) { return V[q * WS]; }
 
     // fixSigns (scheme_integer.cu:398-425)
     FGK_HD void fix_signs() {
@@ -146,9 +146,31 @@ struct ZTLane : LaneBase {
         rebuild();
     }
 
-    // flip with every involved factor non-zero (see the Z2 lane): one scan
-    // finds a member of each new class; the class members are the only
-    // reduction candidates.
+    // One half of a flip with every involved factor non-zero (see the Z2
+    // lane): term x of component mc gets the new factor nv; reduction rules
+    // as in scheme_integer.cu:447-495 with the other changed component oc
+    // whose value at x is ov.
+    struct Half {
+        int any, t, rule;
+        JoinInfo ji;
+    };
+    FGK_HD Half flip_half(int mc, int x, uint32_t nv, int i, int oc, uint32_t fi, uint32_t ov, bool check_reduce) {
+        Half hf;
+        hf.any = -1;
+        hf.t = 1 << 30;
+        hf.rule = 0;
+        const uint32_t *Fc = F + mc * maxm * WS;
+#pragma unroll 8
+        for (int t = 0; t < m; ++t) hf.any = Fc[t * WS] == nv ? t : hf.any;
+        if (hf.any >= 0)
+            class_walk(mc, x, hf.any, hf.ji,
+                       [&](int y) { return check_reduce ? reduce_rule(i, mc, oc, y, fi, ov) : 0; }, hf.t, hf.rule);
+        return hf;
+    }
+
+    // flip (scheme_integer.cu:428-501) when every involved factor is
+    // non-zero; the two halves run on the two lanes of the pair (both on the
+    // host), as in the Z2 lane.
     FGK_HD void flip(int i, int j, int k, int i1, int i2, bool check_reduce) {
         const uint32_t fj1 = f(j, i1), fj2 = f(j, i2);
         const uint32_t fk1 = f(k, i1), fk2 = f(k, i2);
@@ -159,27 +181,19 @@ struct ZTLane : LaneBase {
             flip_literal(i, j, k, i1, i2, check_reduce);
             return;
         }
-        uint32_t *Fj = F + j * maxm * WG;
-        uint32_t *Fk = F + k * maxm * WG;
-        int anyA = -1, anyB = -1;
-#pragma unroll 8
-        for (int t = 0; t < m; ++t) {
-            const uint32_t a = Fj[t * WG];
-            const uint32_t b = Fk[t * WG];
-            anyA = a == A ? t : anyA;
-            anyB = b == B ? t : anyB;
-        }
-        // one walk per new class: reduction candidates (reference order:
-        // lowest member first, index1's side before index2's) and join data
         const uint32_t fi = f(i, i1);  // == f(i, i2): the pair shares component i
-        int tA = 1 << 30, rA = 0, tB = 1 << 30, rB = 0;
-        JoinInfo jA, jB;
-        if (anyA >= 0)
-            class_walk(j, i1, anyA, jA, [&](int y) { return check_reduce ? reduce_rule(i, j, k, y, fi, fk1) : 0; },
-                       tA, rA);
-        if (anyB >= 0)
-            class_walk(k, i2, anyB, jB, [&](int y) { return check_reduce ? reduce_rule(i, k, j, y, fi, fj2) : 0; },
-                       tB, rB);
+#if defined(__HIP_DEVICE_COMPILE__)
+        const int mc = side ? k : j, x = side ? i2 : i1, oc = side ? j : k;
+        const uint32_t nv = side ? B : A, ov = side ? fj2 : fk1;
+        const Half mine = flip_half(mc, x, nv, i, oc, fi, ov, check_reduce);
+        const int ot = pair_exchange(mine.t), orule = pair_exchange(mine.rule);
+        const int tA = side ? ot : mine.t, rA = side ? orule : mine.rule;
+        const int tB = side ? mine.t : ot, rB = side ? mine.rule : orule;
+#else
+        const Half ha = flip_half(j, i1, A, i, k, fi, fk1, check_reduce);
+        const Half hb = flip_half(k, i2, B, i, j, fi, fj2, check_reduce);
+        const int tA = ha.t, rA = ha.rule, tB = hb.t, rB = hb.rule;
+#endif
         if (rA || rB) {
             f(j, i1) = A;
             f(k, i2) = B;
@@ -187,12 +201,20 @@ struct ZTLane : LaneBase {
             else apply_rule(rB, i, j, tB, i2);
             return;
         }
+#if defined(__HIP_DEVICE_COMPILE__)
+        const uint32_t before = Sc(mc);
+        leave(mc, x);
+        if (mine.any >= 0) link_join(mc, x, mine.ji);
+        f(mc, x) = nv;
+        addS(oc, pair_exchange((int)(Sc(mc) - before)));
+#else
         leave(j, i1);
-        if (anyA >= 0) link_join(j, i1, jA);
-        leave(k, i2);
-        if (anyB >= 0) link_join(k, i2, jB);
+        if (ha.any >= 0) link_join(j, i1, ha.ji);
         f(j, i1) = A;
+        leave(k, i2);
+        if (hb.any >= 0) link_join(k, i2, hb.ji);
         f(k, i2) = B;
+#endif
     }
 
     // plus (scheme_integer.cu:503-539)

@@ -53,9 +53,11 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
     const int G = (W + fgk::kWalkersPerGroup - 1) / fgk::kWalkersPerGroup;
     for (int l = 0; l < launches; ++l)
         for (int g = 0; g < G; ++g)
-            for (int lane = 0; lane < fgk::kWalkersPerGroup; ++lane) {
-                if (ring) fgk::lane::walk_lane<fgk::lane::ZTLane>(a, lds.data(), lane, g * fgk::kWalkersPerGroup + lane);
-                else fgk::lane::walk_lane<fgk::lane::Z2Lane>(a, lds.data(), lane, g * fgk::kWalkersPerGroup + lane);
+            for (int slot = 0; slot < fgk::kWalkersPerGroup; ++slot) {
+                // one call per walker: the host side of the lane code does both flip halves
+                const int lane = slot * fgk::kLanesPerWalker;
+                if (ring) fgk::lane::walk_lane<fgk::lane::ZTLane>(a, lds.data(), lane, g * fgk::kWalkersPerGroup + slot);
+                else fgk::lane::walk_lane<fgk::lane::Z2Lane>(a, lds.data(), lane, g * fgk::kWalkersPerGroup + slot);
             }
     for (int w = 0; w < W; ++w) {
         fg_walker_record &r = recs[w];
