@@ -275,8 +275,9 @@ struct LaneBase {
     }
 
     // The unordered pair at position r (< S0 + S1 + S2) of the fresh-list
-    // order: component, then t1, then t2.  H is scanned in chunks of 4 words
-    // (16 terms) with byte sums, then t2 is r' + 1 steps along N from t1.
+    // order: component, then t1, then t2.  H is scanned in blocks of 8 words
+    // (32 terms; the 8 loads are issued together) with byte sums, then t2 is
+    // r' + 1 steps along N from t1.
     FGK_HD bool select_pair(uint32_t r, int &c, int &t1, int &t2) {
         c = 0;
         if (r >= S0) {
@@ -288,22 +289,30 @@ struct LaneBase {
             }
         }
         t1 = -1;
-        for (int q = 0; q < hw; q += 4) {
-            const uint32_t w0 = hword(c, q), w1 = hword(c, q + 1), w2 = hword(c, q + 2), w3 = hword(c, q + 3);
-            const uint32_t s0 = byte_sum(w0), s1 = byte_sum(w1), s2 = byte_sum(w2), s3 = byte_sum(w3);
-            const uint32_t tot = s0 + s1 + s2 + s3;
+        for (int q = 0; q < hw; q += 8) {
+            uint32_t wd[8], sm[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) wd[e] = hword(c, q + e);
+            uint32_t tot = 0;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                sm[e] = byte_sum(wd[e]);
+                tot += sm[e];
+            }
             if (r >= tot) {
                 r -= tot;
                 continue;
             }
-            uint32_t word = w0;
-            int base = q * 4;
-            if (r >= s0) {
-                r -= s0; word = w1; base += 4;
-                if (r >= s1) {
-                    r -= s1; word = w2; base += 4;
-                    if (r >= s2) { r -= s2; word = w3; base += 4; }
-                }
+            // first word whose running sum passes r (values selected, no indexing)
+            uint32_t word = 0, acc = 0;
+            int base = -1;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const bool here = base < 0 && r < acc + sm[e];
+                word = here ? wd[e] : word;
+                base = here ? (q + e) * 4 : base;
+                r = here ? r - acc : r;
+                acc += sm[e];
             }
             for (int b = 0; b < 4; ++b) {
                 const uint32_t hv = (word >> (8 * b)) & 0xFFu;
@@ -354,10 +363,11 @@ FGK_HD void orientation(int c, uint32_t coin, int &j, int &k) {
     }
 }
 
-// The value held by the other lane of this lane's pair (GPU only).
+// The value held by the other lane of this lane's pair (GPU only): a DPP
+// quad permutation [1, 0, 3, 2] swaps adjacent lanes in one VALU op.
 FGK_HD int pair_exchange(int v) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __shfl_xor(v, 1);
+    return __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false);
 #else
     return v;
 #endif
@@ -386,7 +396,7 @@ FGK_HD void copy_best(Lane &s, const WalkArgs &A, bool want, int lane, int w) {
     while (mask) {
         const int src = __builtin_ctzll(mask);
         mask &= mask - 1;
-        const int ms = __shfl(s.m, src);
+        const int ms = __builtin_amdgcn_readlane(s.m, src);
         const int slot = lane / kLanesPerWalker, src_slot = src / kLanesPerWalker;
         const uint32_t *Fs = s.F - slot + src_slot;
         uint32_t *dst = A.best + (size_t)(w - slot + src_slot) * 3 * A.max_m;

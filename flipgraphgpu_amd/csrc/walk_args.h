@@ -55,9 +55,9 @@ constexpr int32_t kErrSelect = 1;   // pair draw found no term (partner counts i
 constexpr int32_t kErrWalk = 2;     // class list walk exceeded the term count
 constexpr int32_t kErrCapacity = 4; // a term would exceed the LDS capacity
 
-// Terms per packed H word and the padding of the H plane (a multiple of 16
-// terms so the selection scan reads whole 4-word chunks).
-constexpr int kHTermsPad = 16;
+// Padding of the packed H / N planes: a multiple of 32 terms (8 words of 4
+// terms) so the selection scan reads whole 8-word blocks.
+constexpr int kHTermsPad = 32;
 __host__ __device__ inline int h_words(int max_m) { return ((max_m + kHTermsPad - 1) / kHTermsPad) * (kHTermsPad / 4); }
 
 // LDS bytes per workgroup of the common planes: 32-bit factor words F and

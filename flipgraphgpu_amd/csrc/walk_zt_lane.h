@@ -160,7 +160,7 @@ struct ZTLane : LaneBase {
         hf.t = 1 << 30;
         hf.rule = 0;
         const uint32_t *Fc = F + mc * maxm * WS;
-#pragma unroll 8
+#pragma unroll 16
         for (int t = 0; t < m; ++t) hf.any = Fc[t * WS] == nv ? t : hf.any;
         if (hf.any >= 0)
             class_walk(mc, x, hf.any, hf.ji,
