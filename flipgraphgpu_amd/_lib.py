@@ -47,6 +47,8 @@ class FgConfig(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("max_rank", ctypes.c_int32),
         ("zt_visit_cap", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+        ("grid", ctypes.c_int32),
     ]
 
 
@@ -70,6 +72,8 @@ def load_library():
     L.fg_destroy.argtypes = [vp]
     L.fg_init_naive.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.fg_init_from_schemes.argtypes = [vp, i32p, ctypes.c_int64]
+    L.fg_init_batched.argtypes = [vp, ctypes.c_int, i32p, i32p]
+    L.fg_autotune.argtypes = [vp, ctypes.c_int, i32p]
     L.fg_walk.argtypes = [vp, ctypes.c_int]
     L.fg_best.argtypes = [vp, i32p, ctypes.c_int64, i32p, i32p]
     L.fg_walker_scheme.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int64]
@@ -78,6 +82,7 @@ def load_library():
                               ctypes.POINTER(ctypes.c_uint64)]
     L.fg_last_walk_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     L.fg_read_walkers.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.fg_summary.argtypes = [vp, i32p, i32p, i32p, i32p, i32p]
     L.fg_write_walkers.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     L.fg_device_info.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                  ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_int]
@@ -109,12 +114,12 @@ class Engine:
 
     def __init__(self, walkers, seed, ring=RING_Z2, max_iterations=10000, plus_iterations=8000,
                  random_iterations=False, check_reduce=True, expand_probability=0.0,
-                 reduce_probability=0.0, max_rank=0, zt_visit_cap=64):
+                 reduce_probability=0.0, max_rank=0, zt_visit_cap=16, use_graph=False, grid=0):
         self.L = load_library()
         self.ring = ring
         self.cfg = FgConfig(ring, walkers, max_iterations, plus_iterations, int(random_iterations),
                             int(check_reduce), expand_probability, reduce_probability, seed, max_rank,
-                            zt_visit_cap)
+                            zt_visit_cap, int(use_graph), grid)
         self.ctx = ctypes.c_void_p()
         _check(self.L.fg_create(ctypes.byref(self.cfg), ctypes.byref(self.ctx)), "fg_create")
         self.walkers = walkers
@@ -142,6 +147,26 @@ class Engine:
     def init_from_schemes(self, blob):
         b, p = _i32(blob)
         _check(self.L.fg_init_from_schemes(self.ctx, p, len(b)), "fg_init_from_schemes")
+
+    def init_batched(self, formats, weights):
+        f = np.asarray(formats, dtype=np.int32).reshape(-1, 3)
+        w = np.asarray(weights, dtype=np.int32)
+        f, pf = _i32(f.ravel())
+        w, pw = _i32(w)
+        _check(self.L.fg_init_batched(self.ctx, len(w), pf, pw), "fg_init_batched")
+
+    def autotune(self, iterations):
+        g = ctypes.c_int32()
+        _check(self.L.fg_autotune(self.ctx, iterations, ctypes.byref(g)), "fg_autotune")
+        return g.value
+
+    def summary(self):
+        W = self.walkers
+        arrs = [np.zeros(W, dtype=np.int32) for _ in range(4)] + [np.zeros(3 * W, dtype=np.int32)]
+        ptrs = [a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) for a in arrs]
+        _check(self.L.fg_summary(self.ctx, *ptrs), "fg_summary")
+        return {"m": arrs[0], "best_m": arrs[1], "best_rank": arrs[2], "flips_count": arrs[3],
+                "n": arrs[4].reshape(W, 3)}
 
     def walk(self, iterations=0):
         """One launch; returns the number of current schemes failing the GPU Brent check."""

@@ -8,7 +8,12 @@
 namespace fgk {
 
 // ring 0 = Z2, 1 = integer ring with packed ternary words
-hipError_t launch_walk(int ring, const WalkArgs &a, hipStream_t stream);
+size_t walk_lds_bytes(int ring, int max_m);
+// Sets the kernel's LDS limit and reports how many workgroups fit per CU.
+hipError_t prepare_walk(int ring, int max_m, int *groups_per_cu);
+// Resets a.queue and launches `grid` persistent workgroups.
+hipError_t launch_walk(int ring, const WalkArgs &a, int grid, hipStream_t stream);
 hipError_t launch_verify(int ring, const VerifyArgs &v, hipStream_t stream);
+hipError_t launch_fill(int32_t *p, int32_t v, int n, hipStream_t stream);
 
 }  // namespace fgk

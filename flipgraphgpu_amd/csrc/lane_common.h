@@ -417,8 +417,8 @@ FGK_HD void copy_best(Lane &s, const WalkArgs &A, bool want, int lane, int w) {
 // (walker slot lane / 2, flip side lane % 2), `w` the walker index.  The host
 // emulation calls it once per walker with an even lane and does both sides.
 template <class Lane>
-FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w) {
-    const bool active = w < A.walkers;
+FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_end) {
+    const bool active = w < w_end;
     const int W = A.walkers;
     const int maxm = A.max_m;
 

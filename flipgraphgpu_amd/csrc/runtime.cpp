@@ -71,15 +71,33 @@ struct fg_ctx {
     int32_t *m = nullptr, *best_m = nullptr, *best_rank = nullptr, *flips_count = nullptr, *aux_idx = nullptr;
     int32_t *fmt = nullptr, *err = nullptr, *ok = nullptr;
     uint64_t *it = nullptr, *aux = nullptr, *total_flips = nullptr, *sum_m = nullptr;
+    int32_t *item_begin = nullptr, *item_end = nullptr, *queue = nullptr;
+    int nitems = 0;
+    int cus = 0;
+    int groups_per_cu = 0;
+    int grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_iterations = -1;
 
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_iterations = -1;
+    }
     void release() {
+        drop_graph();
         void *ptrs[] = {cur, cur_s, best, best_s, m, best_m, best_rank, flips_count, aux_idx,
-                        fmt, err, ok, it, aux, total_flips, sum_m};
+                        fmt, err, ok, it, aux, total_flips, sum_m, item_begin, item_end, queue};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         cur = cur_s = best = best_s = nullptr;
         m = best_m = best_rank = flips_count = aux_idx = fmt = err = ok = nullptr;
         it = aux = total_flips = sum_m = nullptr;
+        item_begin = item_end = queue = nullptr;
+        nitems = 0;
         ready = false;
     }
 };
@@ -118,6 +136,7 @@ extern "C" int fg_create(const fg_config *cfg, fg_ctx **out) {
     c->cfg = *cfg;
     if (c->cfg.zt_visit_cap <= 0) c->cfg.zt_visit_cap = 64;
     c->lds_limit = prop.sharedMemPerBlock;
+    c->cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
@@ -143,7 +162,8 @@ namespace {
 // set bestRanks as FlipGraph::updateRanks does (flip_graph.cu:249-277): the
 // smallest rank per sorted format key, seeded with the naive rank of the
 // configured format.
-int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, int n1, int n2, int n3) {
+int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vector<int> &assign, int n1, int n2,
+              int n3) {
     if (starts.empty()) return fail("init: no schemes");
     const int W = ctx->cfg.walkers;
     int maxm = ctx->cfg.max_rank;
@@ -172,7 +192,7 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, int n1, int n2
     std::map<std::string, int> best_by_key;
     best_by_key[fg::format_key(n1, n2, n3)] = n1 * n2 * n3;
     for (int w = 0; w < W; ++w) {
-        const fg::Scheme &s = starts[w % starts.size()];
+        const fg::Scheme &s = starts[assign[w]];
         m[w] = s.m;
         best_m[w] = s.m;
         fmt[4 * w] = s.n[0];
@@ -189,8 +209,32 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, int n1, int n2
         if (f == best_by_key.end() || s.m < f->second) best_by_key[key] = s.m;
     }
     for (int w = 0; w < W; ++w) {
-        const fg::Scheme &s = starts[w % starts.size()];
+        const fg::Scheme &s = starts[assign[w]];
         best_rank[w] = best_by_key[fg::format_key(s.n[0], s.n[1], s.n[2])];
+    }
+
+    // work items: blocks of up to kWalkersPerGroup walkers of one format,
+    // queued by decreasing naive rank (largest formats first)
+    struct Item {
+        int begin, end, cost;
+    };
+    std::vector<Item> items;
+    for (int w = 0; w < W;) {
+        const fg::Scheme &s = starts[assign[w]];
+        int e = w + 1;
+        while (e < W && e - w < fgk::kWalkersPerGroup) {
+            const fg::Scheme &t = starts[assign[e]];
+            if (t.n[0] != s.n[0] || t.n[1] != s.n[1] || t.n[2] != s.n[2]) break;
+            ++e;
+        }
+        items.push_back({w, e, s.naive_rank()});
+        w = e;
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
+    std::vector<int32_t> ib(items.size()), ie(items.size());
+    for (size_t q = 0; q < items.size(); ++q) {
+        ib[q] = items[q].begin;
+        ie[q] = items[q].end;
     }
 
     HIPCHK(dalloc(&ctx->cur, plane));
@@ -207,6 +251,16 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, int n1, int n2
     HIPCHK(dalloc(&ctx->aux, W));
     HIPCHK(dalloc(&ctx->total_flips, W));
     HIPCHK(dalloc(&ctx->sum_m, W));
+    HIPCHK(dalloc(&ctx->item_begin, ib.size()));
+    HIPCHK(dalloc(&ctx->item_end, ie.size()));
+    HIPCHK(dalloc(&ctx->queue, 1));
+    ctx->nitems = (int)ib.size();
+    HIPCHK(hipMemcpy(ctx->item_begin, ib.data(), ib.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->item_end, ie.data(), ie.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(fgk::prepare_walk(ring, maxm, &ctx->groups_per_cu));
+    if (ctx->groups_per_cu < 1) return fail("init: the walk kernel does not fit on a CU with this scheme size");
+    ctx->grid = ctx->cfg.grid > 0 ? ctx->cfg.grid : ctx->groups_per_cu * ctx->cus;
+    ctx->grid = std::min(ctx->grid, ctx->nitems);
     HIPCHK(hipMemcpy(ctx->cur, words.data(), plane * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->best, best_words.data(), plane * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->m, m.data(), W * 4, hipMemcpyHostToDevice));
@@ -233,7 +287,7 @@ int verify_planes(fg_ctx *ctx, int ring, const uint32_t *words, const int32_t *m
     v.count = count;
     v.max_m = maxm;
     v.ok = ok_dev;
-    HIPCHK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(ok_dev), 1, count, ctx->stream));
+    HIPCHK(fgk::launch_fill(ok_dev, 1, count, ctx->stream));
     HIPCHK(fgk::launch_verify(ring, v, ctx->stream));
     return 0;
 }
@@ -245,7 +299,7 @@ extern "C" int fg_init_naive(fg_ctx *ctx, int n1, int n2, int n3) {
     if (n1 < 1 || n2 < 1 || n3 < 1) return fail("fg_init_naive: sizes must be positive");
     if (n1 * n2 > 64 || n2 * n3 > 64 || n3 * n1 > 64) return fail("fg_init_naive: a factor exceeds 64 coefficients");
     std::vector<fg::Scheme> starts{fg::naive_scheme(ctx->cfg.ring, n1, n2, n3)};
-    return init_pool(ctx, starts, n1, n2, n3);
+    return init_pool(ctx, starts, std::vector<int>(ctx->cfg.walkers, 0), n1, n2, n3);
 }
 
 extern "C" int fg_init_from_schemes(fg_ctx *ctx, const int32_t *blob, int64_t len) {
@@ -265,7 +319,43 @@ extern "C" int fg_init_from_schemes(fg_ctx *ctx, const int32_t *blob, int64_t le
         starts.push_back(std::move(s));
     }
     const fg::Scheme &s0 = starts[0];
-    return init_pool(ctx, starts, s0.n[0], s0.n[1], s0.n[2]);
+    std::vector<int> assign(ctx->cfg.walkers);
+    for (int w = 0; w < ctx->cfg.walkers; ++w) assign[w] = w % (int)starts.size();  // initializeCopyKernel
+    return init_pool(ctx, starts, assign, s0.n[0], s0.n[1], s0.n[2]);
+}
+
+extern "C" int fg_init_batched(fg_ctx *ctx, int nformats, const int32_t *formats, const int32_t *weights) {
+    if (!ctx || nformats < 1 || !formats || !weights) return fail("fg_init_batched: bad arguments");
+    const int W = ctx->cfg.walkers;
+    long long total = 0;
+    for (int f = 0; f < nformats; ++f) {
+        if (weights[f] < 0) return fail("fg_init_batched: negative weight");
+        total += weights[f];
+        const int a = formats[3 * f], b = formats[3 * f + 1], c = formats[3 * f + 2];
+        if (a < 1 || b < 1 || c < 1 || a * b > 64 || b * c > 64 || c * a > 64)
+            return fail("fg_init_batched: bad format " + std::to_string(f));
+    }
+    if (total <= 0) return fail("fg_init_batched: weights sum to zero");
+    std::vector<long long> count(nformats);
+    std::vector<std::pair<double, int>> frac;
+    long long given = 0;
+    for (int f = 0; f < nformats; ++f) {
+        const double exact = (double)W * weights[f] / (double)total;
+        count[f] = (long long)exact;
+        given += count[f];
+        frac.push_back({exact - (double)count[f], f});
+    }
+    std::stable_sort(frac.begin(), frac.end(), [](const std::pair<double, int> &a, const std::pair<double, int> &b) {
+        return a.first > b.first;
+    });
+    for (size_t q = 0; given < W; ++q, ++given) ++count[frac[q % frac.size()].second];
+    std::vector<fg::Scheme> starts;
+    std::vector<int> assign;
+    for (int f = 0; f < nformats; ++f) {
+        starts.push_back(fg::naive_scheme(ctx->cfg.ring, formats[3 * f], formats[3 * f + 1], formats[3 * f + 2]));
+        for (long long q = 0; q < count[f]; ++q) assign.push_back(f);
+    }
+    return init_pool(ctx, starts, assign, formats[0], formats[1], formats[2]);
 }
 
 extern "C" int fg_walk(fg_ctx *ctx, int iterations) {
@@ -299,12 +389,40 @@ extern "C" int fg_walk(fg_ctx *ctx, int iterations) {
     a.check_reduce = ctx->cfg.check_reduce;
     a.zt_visit_cap = ctx->cfg.zt_visit_cap;
     a.err = ctx->err;
-    HIPCHK(hipMemsetAsync(ctx->err, 0, sizeof(int32_t), ctx->stream));
-    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    HIPCHK(fgk::launch_walk(ctx->cfg.ring, a, ctx->stream));
-    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-    int rc = verify_planes(ctx, ctx->cfg.ring, ctx->cur, ctx->m, ctx->fmt, ctx->W, ctx->maxm, ctx->ok);
-    if (rc) return rc;
+    a.item_begin = ctx->item_begin;
+    a.item_end = ctx->item_end;
+    a.nitems = ctx->nitems;
+    a.queue = ctx->queue;
+    // one launch: error flag reset, queue reset + persistent walk (timed),
+    // then the Brent check of every current scheme
+    // (in graph mode the events bracket the whole graph instead)
+    auto enqueue = [&](bool timed) -> int {
+        HIPCHK(fgk::launch_fill(ctx->err, 0, 1, ctx->stream));
+        if (timed) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+        HIPCHK(fgk::launch_walk(ctx->cfg.ring, a, ctx->grid, ctx->stream));
+        if (timed) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        return verify_planes(ctx, ctx->cfg.ring, ctx->cur, ctx->m, ctx->fmt, ctx->W, ctx->maxm, ctx->ok);
+    };
+    if (ctx->cfg.use_graph) {
+        if (ctx->graph_iterations != a.iterations) {
+            ctx->drop_graph();
+            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue(false);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+            if (rc) return rc;
+            if (ec != hipSuccess) return fail(std::string("fg_walk: graph capture failed: ") + hipGetErrorString(ec));
+            ctx->graph = g;
+            HIPCHK(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
+            ctx->graph_iterations = a.iterations;
+        }
+        HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+        HIPCHK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+        HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    } else {
+        const int rc = enqueue(true);
+        if (rc) return rc;
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
     int32_t err = 0;
@@ -315,6 +433,33 @@ extern "C" int fg_walk(fg_ctx *ctx, int iterations) {
     int bad = 0;
     for (int x : ok) bad += x == 0;
     return bad;
+}
+
+extern "C" int fg_autotune(fg_ctx *ctx, int iterations, int32_t *grid) {
+    if (!ctx || !ctx->ready) return fail("fg_autotune: context not initialised");
+    if (iterations <= 0) return fail("fg_autotune: iterations must be positive");
+    const int maxg = std::max(1, std::min(ctx->groups_per_cu * ctx->cus, ctx->nitems));
+    int best = maxg;
+    double best_rate = -1;
+    for (int mult = 1;; ++mult) {
+        const int g = std::min(mult * ctx->cus, maxg);
+        ctx->grid = g;
+        uint64_t f0 = 0, f1 = 0;
+        if (fg_counters(ctx, &f0, nullptr, nullptr) < 0) return -1;
+        const int bad = fg_walk(ctx, iterations);
+        if (bad < 0) return bad;
+        if (fg_counters(ctx, &f1, nullptr, nullptr) < 0) return -1;
+        const double rate = (double)(f1 - f0) / std::max(1e-3, (double)ctx->last_ms);
+        if (rate > best_rate) {
+            best_rate = rate;
+            best = g;
+        }
+        if (g >= maxg) break;
+    }
+    ctx->grid = best;
+    ctx->drop_graph();
+    if (grid) *grid = best;
+    return 0;
 }
 
 extern "C" int fg_last_walk_ms(fg_ctx *ctx, float *ms) {

@@ -10,16 +10,37 @@
 
 namespace fgk {
 
+// Persistent walk: each workgroup (one wave, 32 walkers) claims blocks of
+// walkers from the queue until none is left.  Blocks never straddle two
+// formats and are queued largest format first, so uneven formats share one
+// launch with little tail.  Every wave exits once the queue is exhausted.
+template <class Lane>
+__device__ void walk_persistent(const WalkArgs &A, uint8_t *lds) {
+    const int lane = threadIdx.x;
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = atomicAdd(A.queue, 1);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= A.nitems) break;
+        lane::walk_lane<Lane>(A, lds, lane, A.item_begin[item] + lane / kLanesPerWalker, A.item_end[item]);
+    }
+}
+
 __global__ void __launch_bounds__(kThreadsPerGroup) walk_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
-    const int lane = threadIdx.x;
-    lane::walk_lane<lane::Z2Lane>(A, lds, lane, blockIdx.x * kWalkersPerGroup + lane / kLanesPerWalker);
+    walk_persistent<lane::Z2Lane>(A, lds);
 }
 
 __global__ void __launch_bounds__(kThreadsPerGroup) walk_zt_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
-    const int lane = threadIdx.x;
-    lane::walk_lane<lane::ZTLane>(A, lds, lane, blockIdx.x * kWalkersPerGroup + lane / kLanesPerWalker);
+    walk_persistent<lane::ZTLane>(A, lds);
+}
+
+// Fills n int32 with v (used instead of a memset so launches can be captured
+// in a HIP graph).
+__global__ void fill_i32_kernel(int32_t *p, int32_t v, int n) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) p[g] = v;
 }
 
 // Brent equations (src/schemes/scheme_z2.cu:3-29), bit-sliced: one thread
@@ -85,14 +106,27 @@ __global__ void verify_zt_kernel(VerifyArgs V) {
     if (!ok) V.ok[s] = 0;
 }
 
-hipError_t launch_walk(int ring, const WalkArgs &a, hipStream_t stream) {
-    const size_t lds = ring ? zt_lds_bytes(a.max_m) : z2_lds_bytes(a.max_m);
+size_t walk_lds_bytes(int ring, int max_m) { return ring ? zt_lds_bytes(max_m) : z2_lds_bytes(max_m); }
+
+hipError_t prepare_walk(int ring, int max_m, int *groups_per_cu) {
+    const size_t lds = walk_lds_bytes(ring, max_m);
     const void *fn = ring ? reinterpret_cast<const void *>(walk_zt_kernel) : reinterpret_cast<const void *>(walk_z2_kernel);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    const int grid = (a.walkers + kWalkersPerGroup - 1) / kWalkersPerGroup;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu, fn, kThreadsPerGroup, lds);
+}
+
+hipError_t launch_walk(int ring, const WalkArgs &a, int grid, hipStream_t stream) {
+    const size_t lds = walk_lds_bytes(ring, a.max_m);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, stream, a.queue, 0, 1);
     if (ring) hipLaunchKernelGGL(walk_zt_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
     else hipLaunchKernelGGL(walk_z2_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(int32_t *p, int32_t v, int n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, v, n);
     return hipGetLastError();
 }
 

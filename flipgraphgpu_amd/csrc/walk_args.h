@@ -48,6 +48,12 @@ struct WalkArgs {
     int32_t check_reduce;
     int32_t zt_visit_cap;
     int32_t *err;         // invariant violations (bit flags), host checks after each launch
+    // persistent launch: workgroups take blocks [item_begin[q], item_end[q])
+    // of at most kWalkersPerGroup walkers from a queue until it runs dry
+    const int32_t *item_begin;
+    const int32_t *item_end;
+    int32_t nitems;
+    int32_t *queue;       // next unclaimed item; reset to 0 before each launch
 };
 
 // Error bits written by the walk kernels.

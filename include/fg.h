@@ -41,7 +41,9 @@ typedef struct {
     double reduce_probability;  /* --reduce-probability (divided by max iterations) */
     uint64_t seed;              /* --seed (0 is replaced by time(0) in the CLI, not here) */
     int32_t max_rank;           /* term capacity per walker; 0 = max(naive rank, input ranks) */
-    int32_t zt_visit_cap;       /* integer ring: candidates one tryFlip may visit (0 = 64) */
+    int32_t zt_visit_cap;       /* integer ring: candidates one tryFlip may visit (0 = 16) */
+    int32_t use_graph;          /* 1: replay each launch (queue reset, walk, Brent check) as a captured HIP graph */
+    int32_t grid;               /* persistent workgroups per launch; 0 = all that fit on the GPU at once */
 } fg_config;
 
 /* Per-walker state, for tests and checkpointing. */
@@ -80,6 +82,21 @@ FG_API int fg_init_naive(fg_ctx *ctx, int n1, int n2, int n3);
  * reduced with tryReduce until it no longer shrinks, then walkers are filled
  * round-robin.  Returns -2 if a scheme fails the Brent check. */
 FG_API int fg_init_from_schemes(fg_ctx *ctx, const int32_t *blob, int64_t len);
+
+/* Several formats in one pool (no reference counterpart: the reference runs
+ * one format per process; its resize edges mix formats only through
+ * src/entities/flip_graph.cu:575-613).  Walkers are split between the
+ * nformats formats (n1 n2 n3 triples) in proportion to weights: format f gets
+ * floor(walkers * w_f / sum w) and the remaining walkers go one each to the
+ * formats with the largest fractional parts (ties: lower f).  Every walker
+ * starts from its format's naive scheme; each format's walkers are
+ * contiguous, in the given order. */
+FG_API int fg_init_batched(fg_ctx *ctx, int nformats, const int32_t *formats, const int32_t *weights);
+
+/* Picks the persistent grid size: times one short launch of `iterations`
+ * per candidate (1, 2, ... times the CU count, up to the occupancy limit),
+ * keeps the fastest in flips per second and returns it in *grid. */
+FG_API int fg_autotune(fg_ctx *ctx, int iterations, int32_t *grid);
 
 /* FlipGraph::randomWalk / randomWalkKernel (src/entities/flip_graph.cu:235-240,
  * 520-573): one launch.  iterations <= 0 uses max_iterations (random in

@@ -56,8 +56,8 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
             for (int slot = 0; slot < fgk::kWalkersPerGroup; ++slot) {
                 // one call per walker: the host side of the lane code does both flip halves
                 const int lane = slot * fgk::kLanesPerWalker;
-                if (ring) fgk::lane::walk_lane<fgk::lane::ZTLane>(a, lds.data(), lane, g * fgk::kWalkersPerGroup + slot);
-                else fgk::lane::walk_lane<fgk::lane::Z2Lane>(a, lds.data(), lane, g * fgk::kWalkersPerGroup + slot);
+                if (ring) fgk::lane::walk_lane<fgk::lane::ZTLane>(a, lds.data(), lane, g * fgk::kWalkersPerGroup + slot, W);
+                else fgk::lane::walk_lane<fgk::lane::Z2Lane>(a, lds.data(), lane, g * fgk::kWalkersPerGroup + slot, W);
             }
     for (int w = 0; w < W; ++w) {
         fg_walker_record &r = recs[w];

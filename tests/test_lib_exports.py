@@ -47,7 +47,7 @@ def test_last_error_is_callable_without_gpu():
 
 def test_create_rejects_bad_config_before_touching_the_device():
     L = fg.load_library()
-    cfg = fg.FgConfig(fg.RING_Z2, 0, 10, 10, 0, 1, 0.0, 0.0, 1, 0, 64)
+    cfg = fg.FgConfig(fg.RING_Z2, 0, 10, 10, 0, 1, 0.0, 0.0, 1, 0, 16, 0, 0)
     ctx = ctypes.c_void_p()
     assert L.fg_create(ctypes.byref(cfg), ctypes.byref(ctx)) < 0
     assert b"walkers" in L.fg_last_error()
