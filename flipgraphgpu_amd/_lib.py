@@ -73,6 +73,10 @@ def load_library():
     L.fg_init_naive.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.fg_init_from_schemes.argtypes = [vp, i32p, ctypes.c_int64]
     L.fg_init_batched.argtypes = [vp, ctypes.c_int, i32p, i32p]
+    L.fg_init_minimizer.argtypes = [vp, i32p, ctypes.c_int64]
+    L.fg_minimize.argtypes = [vp]
+    L.fg_complexities.argtypes = [vp, i32p, i32p]
+    L.fg_broadcast_best.argtypes = [vp, ctypes.c_int, i32p, ctypes.c_int]
     L.fg_autotune.argtypes = [vp, ctypes.c_int, i32p]
     L.fg_walk.argtypes = [vp, ctypes.c_int]
     L.fg_best.argtypes = [vp, i32p, ctypes.c_int64, i32p, i32p]
@@ -154,6 +158,29 @@ class Engine:
         f, pf = _i32(f.ravel())
         w, pw = _i32(w)
         _check(self.L.fg_init_batched(self.ctx, len(w), pf, pw), "fg_init_batched")
+
+    def init_minimizer(self, blob):
+        """Minimizer pool from a count-prefixed blob of schemes of one format."""
+        b, p = _i32(blob)
+        _check(self.L.fg_init_minimizer(self.ctx, p, len(b)), "fg_init_minimizer")
+
+    def minimize(self):
+        """One minimizer launch; returns the number of current schemes failing the GPU Brent check."""
+        return _check(self.L.fg_minimize(self.ctx), "fg_minimize")
+
+    def complexities(self, best=True):
+        """(best complexities or None, current complexities) per walker."""
+        W = self.walkers
+        cur = np.zeros(W, dtype=np.int32)
+        bst = np.zeros(W, dtype=np.int32) if best else None
+        pb = bst.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if best else None
+        _check(self.L.fg_complexities(self.ctx, pb, cur.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+               "fg_complexities")
+        return bst, cur
+
+    def broadcast_best(self, src, dst):
+        d, p = _i32(dst)
+        _check(self.L.fg_broadcast_best(self.ctx, int(src), p, len(d)), "fg_broadcast_best")
 
     def autotune(self, iterations):
         g = ctypes.c_int32()

@@ -219,4 +219,29 @@ bool make_dirs(const std::string &path) {
     return true;
 }
 
+std::string save_path(const std::string &dir, int ring, const fg::Scheme &s, int iteration, int run_id) {
+    std::ostringstream ss;
+    ss << dir << "/" << fg::format_key(s.n[0], s.n[1], s.n[2]) << "_m" << s.m << "_c" << fg::complexity(s)
+       << "_iteration" << iteration << "_run" << run_id << "_" << fg::format_key(s.n[0], s.n[1], s.n[2], false) << "_"
+       << ring_name(ring) << ".json";
+    return ss.str();
+}
+
+bool verified(int ring, const fg::Scheme &s) {
+    std::vector<int32_t> blob{1};
+    fg::scheme_to_blob(s, &blob);
+    int32_t ok = 0;
+    const int good = fg_verify(ring, blob.data(), (int64_t)blob.size(), &ok);
+    return good == 1 && ok && fg::validate(s);
+}
+
+bool fetch_scheme(fg_ctx *ctx, int ring, int walker, int which, fg::Scheme *out) {
+    std::vector<int32_t> buf(1 << 20);
+    const int n = fg_walker_scheme(ctx, walker, which, buf.data(), (int64_t)buf.size());
+    if (n < 0) return false;
+    int64_t pos = 0;
+    std::string err;
+    return fg::scheme_from_blob(ring, buf.data(), n, &pos, out, &err);
+}
+
 }  // namespace fgcli

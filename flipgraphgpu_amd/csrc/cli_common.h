@@ -8,6 +8,9 @@
 #include <unordered_map>
 #include <vector>
 
+#include "fg.h"
+#include "scheme_host.h"
+
 namespace fgcli {
 
 enum class ArgType { String, Natural, Real };
@@ -43,5 +46,13 @@ std::string pretty_int(long long value);
 const char *ring_name(int ring);
 // mkdir -p
 bool make_dirs(const std::string &path);
+
+// getSavePath (flip_graph.cu:386-399, complexity_minimizer.cu:136-149):
+// {path}/{sorted key}_m{m}_c{complexity}_iteration{i}_run{r}_{real key}_{ring}.json
+std::string save_path(const std::string &dir, int ring, const fg::Scheme &s, int iteration, int run_id);
+// GPU Brent check plus host re-check of one scheme before it is written.
+bool verified(int ring, const fg::Scheme &s);
+// Best (which = 1) or current (which = 0) scheme of one walker.
+bool fetch_scheme(fg_ctx *ctx, int ring, int walker, int which, fg::Scheme *out);
 
 }  // namespace fgcli

@@ -34,32 +34,6 @@ struct Run {
     std::map<std::string, int> best_by_key;  // n2bestRank
 };
 
-std::string save_path(const Run &r, const fg::Scheme &s, int iteration, int run_id) {  // flip_graph.cu:386-399
-    std::ostringstream ss;
-    ss << r.path << "/" << fg::format_key(s.n[0], s.n[1], s.n[2]) << "_m" << s.m << "_c" << fg::complexity(s)
-       << "_iteration" << iteration << "_run" << run_id << "_" << fg::format_key(s.n[0], s.n[1], s.n[2], false) << "_"
-       << fgcli::ring_name(r.ring) << ".json";
-    return ss.str();
-}
-
-// GPU Brent check plus host re-check of one scheme before it is written.
-bool verified(int ring, const fg::Scheme &s) {
-    std::vector<int32_t> blob{1};
-    fg::scheme_to_blob(s, &blob);
-    int32_t ok = 0;
-    const int good = fg_verify(ring, blob.data(), (int64_t)blob.size(), &ok);
-    return good == 1 && ok && fg::validate(s);
-}
-
-bool fetch_best(const Run &r, int walker, fg::Scheme *out) {
-    std::vector<int32_t> buf(1 << 20);
-    const int n = fg_walker_scheme(r.ctx, walker, 1, buf.data(), (int64_t)buf.size());
-    if (n < 0) return false;
-    int64_t pos = 0;
-    std::string err;
-    return fg::scheme_from_blob(r.ring, buf.data(), n, &pos, out, &err);
-}
-
 // FlipGraph::report (src/entities/flip_graph.cu:303-380).  Returns the best
 // rank seen for the first format key.
 int report(Run &r, std::chrono::steady_clock::time_point start, int iteration, const std::vector<double> &times,
@@ -82,8 +56,8 @@ int report(Run &r, std::chrono::steady_clock::time_point start, int iteration, c
         const int prev = it == r.best_by_key.end() ? 1 << 30 : it->second;
         if (br[w] < prev) {
             fg::Scheme s;
-            if (fetch_best(r, w, &s) && verified(r.ring, s)) {
-                const std::string p = save_path(r, s, iteration, w);
+            if (fgcli::fetch_scheme(r.ctx, r.ring, w, 1, &s) && fgcli::verified(r.ring, s)) {
+                const std::string p = fgcli::save_path(r.path, r.ring, s, iteration, w);
                 fgcli::make_dirs(r.path);
                 fg::save_json(s, p);
                 std::cout << "Best rank of " << kv.first << " was improved from " << prev << " to " << br[w]

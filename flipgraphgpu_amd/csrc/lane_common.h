@@ -110,6 +110,8 @@ struct LaneBase {
     uint32_t a0, a1, a2, a3;
     int aidx;
     int visit_cap;  // integer ring: candidates one tryFlip may visit
+    uint32_t vmask; // coefficient bits of a factor word (all for Z2, bits 0-15 for the integer ring)
+    int pop;        // coefficient count of the scheme, kept for getComplexity
 
     // Extra per-lane LDS beyond the common planes (none by default).
     FGK_HD void lane_extra(uint8_t *, int) {}
@@ -133,6 +135,13 @@ struct LaneBase {
     }
     FGK_HD uint32_t Sc(int c) const { return (c == 0 ? S0 : 0u) + (c == 1 ? S1 : 0u) + (c == 2 ? S2 : 0u); }
     FGK_HD int naive_rank() const { return n0 * n1 * n2; }
+    // getComplexity (scheme_z2.cu:127-139, scheme_integer.cu:139-147)
+    FGK_HD int complexity() const { return pop - 2 * m - nn2; }
+    FGK_HD static int popc(uint32_t x) { return __builtin_popcount(x); }
+    // coefficient count change of a flip that replaces factors a0 -> a1 and b0 -> b1
+    FGK_HD void pop_update(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1) {
+        pop += popc(a1 & vmask) + popc(b1 & vmask) - popc(a0 & vmask) - popc(b0 & vmask);
+    }
 
     // ---- aux RNG stream ----
     FGK_HD void aux_fill(uint64_t block) {
@@ -179,6 +188,9 @@ struct LaneBase {
     // initFlips (src/schemes/scheme_z2.cu:141-150) for the N/H representation.
     FGK_HD void rebuild() {
         S0 = S1 = S2 = 0;
+        pop = 0;
+        for (int c = 0; c < 3; ++c)
+            for (int t = 0; t < m; ++t) pop += popc(f(c, t) & vmask);
         for (int c = 0; c < 3; ++c) {
             for (int q = 0; q < hw; ++q) hword(c, q) = 0;
             for (int t = 0; t < m; ++t) nx(c, t) = 0xFF;
@@ -412,11 +424,24 @@ FGK_HD void copy_best(Lane &s, const WalkArgs &A, bool want, int lane, int w) {
 #endif
 }
 
-// The walk of one lane (randomWalkKernel body, src/entities/flip_graph.cu:520-573).
-// `lds` is the workgroup's LDS block, `lane` the lane in the workgroup
-// (walker slot lane / 2, flip side lane % 2), `w` the walker index.  The host
-// emulation calls it once per walker with an even lane and does both sides.
-template <class Lane>
+// True when `v` holds on any lane of the wave (on the host: for this walker).
+FGK_HD bool wave_any(bool v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __ballot(v) != 0;
+#else
+    return v;
+#endif
+}
+
+// The walk of one lane (randomWalkKernel body, src/entities/flip_graph.cu:520-573),
+// or with kMinimize the complexity minimizer's (minimizeKernel,
+// src/entities/complexity_minimizer.cu:190-214): flips without reductions
+// until one fails, keeping the scheme of lowest complexity; best_rank then
+// holds that complexity.  `lds` is the workgroup's LDS block, `lane` the lane
+// in the workgroup (walker slot lane / 2, flip side lane % 2), `w` the walker
+// index.  The host emulation calls it once per walker with an even lane and
+// does both sides.
+template <class Lane, bool kMinimize>
 FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_end) {
     const bool active = w < w_end;
     const int W = A.walkers;
@@ -439,6 +464,8 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     s.m = 0;
     s.n0 = s.n1 = s.n2 = 1;
     s.visit_cap = A.zt_visit_cap;
+    s.vmask = Lane::kValueMask;
+    s.pop = 0;
     int best_rank = 0, best_m = 0, flips_count = 0;
     uint64_t it = 0, total_flips = 0, sum_m = 0;
     s.aux = 0;
@@ -465,6 +492,33 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     s.nn2 = s.n2 * s.n0;
     if (active) s.rebuild();
 
+    if constexpr (kMinimize) {
+        const int iters = active ? A.iterations : 0;
+        const int trips = wave_max(iters);
+        bool live = active;
+        for (int iter = 0; iter < trips && wave_any(live); ++iter) {
+            bool copy = false;
+            if (live && iter < iters) {
+                uint32_t b0, b1, b2, b3;
+                philox10((uint32_t)it, (uint32_t)(it >> 32), (uint32_t)w, 0u, s.k0, s.k1, b0, b1, b2, b3);
+                ++it;
+                const int rank = s.m;
+                if (!s.try_flip(b0, b1, false)) {  // complexity_minimizer.cu:200-201
+                    live = false;
+                } else {
+                    ++total_flips;
+                    sum_m += (uint64_t)rank;
+                    const int cx = s.complexity();
+                    if (cx < best_rank) {  // :203-207
+                        best_rank = cx;
+                        best_m = s.m;
+                        copy = true;
+                    }
+                }
+            }
+            copy_best(s, A, copy, lane, w);
+        }
+    } else {
     int iters = 0;
     if (active) {
         iters = A.iterations;
@@ -506,6 +560,7 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
             if (flips_count >= A.plus_iterations && s.try_plus()) flips_count = 0;  // :564-565
         }
     }
+    }
 
     if (active && s.side == 0) {
         A.m[w] = s.m;
@@ -517,6 +572,7 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
         A.aux_idx[w] = s.aidx;
         A.total_flips[w] = total_flips;
         A.sum_m[w] = sum_m;
+        if (A.cx) A.cx[w] = s.complexity();
         for (int c = 0; c < 3; ++c)
             for (int t = 0; t < s.m; ++t) A.cur[((size_t)c * maxm + t) * W + w] = s.f(c, t);
     }

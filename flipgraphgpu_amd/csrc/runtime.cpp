@@ -72,7 +72,9 @@ struct fg_ctx {
     int32_t *fmt = nullptr, *err = nullptr, *ok = nullptr;
     uint64_t *it = nullptr, *aux = nullptr, *total_flips = nullptr, *sum_m = nullptr;
     int32_t *item_begin = nullptr, *item_end = nullptr, *queue = nullptr;
+    int32_t *cx = nullptr, *dst = nullptr;
     int nitems = 0;
+    bool minimize = false;  // pool set up by fg_init_minimizer
     int cus = 0;
     int groups_per_cu = 0;
     int grid = 0;
@@ -90,14 +92,15 @@ struct fg_ctx {
     void release() {
         drop_graph();
         void *ptrs[] = {cur, cur_s, best, best_s, m, best_m, best_rank, flips_count, aux_idx,
-                        fmt, err, ok, it, aux, total_flips, sum_m, item_begin, item_end, queue};
+                        fmt, err, ok, it, aux, total_flips, sum_m, item_begin, item_end, queue, cx, dst};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         cur = cur_s = best = best_s = nullptr;
         m = best_m = best_rank = flips_count = aux_idx = fmt = err = ok = nullptr;
         it = aux = total_flips = sum_m = nullptr;
-        item_begin = item_end = queue = nullptr;
+        item_begin = item_end = queue = cx = dst = nullptr;
         nitems = 0;
+        minimize = false;
         ready = false;
     }
 };
@@ -163,7 +166,7 @@ namespace {
 // smallest rank per sorted format key, seeded with the naive rank of the
 // configured format.
 int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vector<int> &assign, int n1, int n2,
-              int n3) {
+              int n3, bool minimize = false) {
     if (starts.empty()) return fail("init: no schemes");
     const int W = ctx->cfg.walkers;
     int maxm = ctx->cfg.max_rank;
@@ -254,10 +257,13 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
     HIPCHK(dalloc(&ctx->item_begin, ib.size()));
     HIPCHK(dalloc(&ctx->item_end, ie.size()));
     HIPCHK(dalloc(&ctx->queue, 1));
+    HIPCHK(dalloc(&ctx->cx, W));
+    HIPCHK(dalloc(&ctx->dst, W));
+    ctx->minimize = minimize;
     ctx->nitems = (int)ib.size();
     HIPCHK(hipMemcpy(ctx->item_begin, ib.data(), ib.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->item_end, ie.data(), ie.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(fgk::prepare_walk(ring, maxm, &ctx->groups_per_cu));
+    HIPCHK(fgk::prepare_walk(ring, maxm, minimize, &ctx->groups_per_cu));
     if (ctx->groups_per_cu < 1) return fail("init: the walk kernel does not fit on a CU with this scheme size");
     ctx->grid = ctx->cfg.grid > 0 ? ctx->cfg.grid : ctx->groups_per_cu * ctx->cus;
     ctx->grid = std::min(ctx->grid, ctx->nitems);
@@ -322,6 +328,65 @@ extern "C" int fg_init_from_schemes(fg_ctx *ctx, const int32_t *blob, int64_t le
     std::vector<int> assign(ctx->cfg.walkers);
     for (int w = 0; w < ctx->cfg.walkers; ++w) assign[w] = w % (int)starts.size();  // initializeCopyKernel
     return init_pool(ctx, starts, assign, s0.n[0], s0.n[1], s0.n[2]);
+}
+
+extern "C" int fg_init_minimizer(fg_ctx *ctx, const int32_t *blob, int64_t len) {
+    if (!ctx || !blob || len < 1) return fail("fg_init_minimizer: bad arguments");
+    const int count = blob[0];
+    if (count < 1) return fail("fg_init_minimizer: scheme count must be positive");
+    std::vector<fg::Scheme> starts;
+    int64_t pos = 1;
+    for (int q = 0; q < count; ++q) {
+        fg::Scheme s;
+        std::string err;
+        if (!fg::scheme_from_blob(ctx->cfg.ring, blob, len, &pos, &s, &err))
+            return fail("fg_init_minimizer: scheme " + std::to_string(q) + ": " + err);
+        if (!fg::validate(s)) return fail("fg_init_minimizer: scheme " + std::to_string(q) + " fails the Brent check", -2);
+        if (!starts.empty() && (s.n[0] != starts[0].n[0] || s.n[1] != starts[0].n[1] || s.n[2] != starts[0].n[2]))
+            return fail("fg_init_minimizer: schemes must share one format");
+        starts.push_back(std::move(s));
+    }
+    // initialize() + initializeKernel (complexity_minimizer.cu:67-77, 172-188):
+    // walker w starts from scheme w % count, every best complexity from the
+    // lowest complexity among the inputs
+    int best_c = INT_MAX;
+    for (const auto &s : starts) best_c = std::min(best_c, fg::complexity(s));
+    std::vector<int> assign(ctx->cfg.walkers);
+    for (int w = 0; w < ctx->cfg.walkers; ++w) assign[w] = w % (int)starts.size();
+    const fg::Scheme &s0 = starts[0];
+    const int rc = init_pool(ctx, starts, assign, s0.n[0], s0.n[1], s0.n[2], true);
+    if (rc) return rc;
+    std::vector<int32_t> bc(ctx->W, best_c);
+    HIPCHK(hipMemcpy(ctx->best_rank, bc.data(), bc.size() * 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int fg_minimize(fg_ctx *ctx) {
+    if (!ctx || !ctx->ready) return fail("fg_minimize: context not initialised");
+    if (!ctx->minimize) return fail("fg_minimize: the pool was not set up by fg_init_minimizer");
+    return fg_walk(ctx, 0);
+}
+
+extern "C" int fg_complexities(fg_ctx *ctx, int32_t *best, int32_t *cur) {
+    if (!ctx || !ctx->ready) return fail("fg_complexities: context not initialised");
+    if (best && !ctx->minimize) return fail("fg_complexities: best complexities are kept by minimizer pools only");
+    if (best) HIPCHK(hipMemcpy(best, ctx->best_rank, (size_t)ctx->W * 4, hipMemcpyDeviceToHost));
+    if (cur) HIPCHK(hipMemcpy(cur, ctx->cx, (size_t)ctx->W * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int fg_broadcast_best(fg_ctx *ctx, int src, const int32_t *dst, int count) {
+    if (!ctx || !ctx->ready) return fail("fg_broadcast_best: context not initialised");
+    if (src < 0 || src >= ctx->W) return fail("fg_broadcast_best: source walker out of range");
+    if (count < 0 || count > ctx->W || (count > 0 && !dst)) return fail("fg_broadcast_best: bad destination list");
+    for (int q = 0; q < count; ++q)
+        if (dst[q] < 0 || dst[q] >= ctx->W) return fail("fg_broadcast_best: destination walker out of range");
+    if (count == 0) return 0;
+    HIPCHK(hipMemcpyAsync(ctx->dst, dst, (size_t)count * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(fgk::launch_broadcast_best(ctx->cur, ctx->m, ctx->best, ctx->best_m, ctx->W, ctx->maxm, src, ctx->dst, count,
+                                      ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
 }
 
 extern "C" int fg_init_batched(fg_ctx *ctx, int nformats, const int32_t *formats, const int32_t *weights) {
@@ -393,13 +458,21 @@ extern "C" int fg_walk(fg_ctx *ctx, int iterations) {
     a.item_end = ctx->item_end;
     a.nitems = ctx->nitems;
     a.queue = ctx->queue;
+    a.cx = ctx->cx;
+    if (ctx->minimize) {
+        // minimizeKernel: maxIterations flips without reductions or expansions
+        a.iterations = a.max_iterations = ctx->cfg.max_iterations;
+        a.random_iterations = 0;
+        a.use_reduce = a.use_expand = 0;
+        a.check_reduce = 0;
+    }
     // one launch: error flag reset, queue reset + persistent walk (timed),
     // then the Brent check of every current scheme
     // (in graph mode the events bracket the whole graph instead)
     auto enqueue = [&](bool timed) -> int {
         HIPCHK(fgk::launch_fill(ctx->err, 0, 1, ctx->stream));
         if (timed) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-        HIPCHK(fgk::launch_walk(ctx->cfg.ring, a, ctx->grid, ctx->stream));
+        HIPCHK(fgk::launch_walk(ctx->cfg.ring, ctx->minimize, a, ctx->grid, ctx->stream));
         if (timed) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         return verify_planes(ctx, ctx->cfg.ring, ctx->cur, ctx->m, ctx->fmt, ctx->W, ctx->maxm, ctx->ok);
     };
@@ -679,8 +752,9 @@ extern "C" int fg_walker_scheme(fg_ctx *ctx, int walker, int which, int32_t *out
 
 extern "C" int fg_best(fg_ctx *ctx, int32_t *out_blob, int64_t cap, int32_t *out_rank, int32_t *out_walker) {
     if (!ctx || !ctx->ready) return fail("fg_best: context not initialised");
+    // a minimizer pool ranks walkers by best complexity, a walk pool by best rank
     std::vector<int32_t> bm(ctx->W);
-    HIPCHK(hipMemcpy(bm.data(), ctx->best_m, ctx->W * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(bm.data(), ctx->minimize ? ctx->best_rank : ctx->best_m, ctx->W * 4, hipMemcpyDeviceToHost));
     int bw = 0;
     for (int w = 1; w < ctx->W; ++w)
         if (bm[w] < bm[bw]) bw = w;

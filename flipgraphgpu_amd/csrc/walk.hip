@@ -14,26 +14,57 @@ namespace fgk {
 // walkers from the queue until none is left.  Blocks never straddle two
 // formats and are queued largest format first, so uneven formats share one
 // launch with little tail.  Every wave exits once the queue is exhausted.
-template <class Lane>
-__device__ void walk_persistent(const WalkArgs &A, uint8_t *lds) {
+// Inlined into each kernel so the argument block is read straight from the
+// kernel-argument segment (an out-of-line call would copy it to scratch).
+template <class Lane, bool kMinimize>
+__device__ __attribute__((always_inline)) void walk_persistent(const WalkArgs &A, uint8_t *lds) {
     const int lane = threadIdx.x;
     for (;;) {
         int item = 0;
         if (lane == 0) item = atomicAdd(A.queue, 1);
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= A.nitems) break;
-        lane::walk_lane<Lane>(A, lds, lane, A.item_begin[item] + lane / kLanesPerWalker, A.item_end[item]);
+        lane::walk_lane<Lane, kMinimize>(A, lds, lane, A.item_begin[item] + lane / kLanesPerWalker, A.item_end[item]);
     }
 }
 
 __global__ void __launch_bounds__(kThreadsPerGroup) walk_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
-    walk_persistent<lane::Z2Lane>(A, lds);
+    walk_persistent<lane::Z2Lane, false>(A, lds);
 }
 
 __global__ void __launch_bounds__(kThreadsPerGroup) walk_zt_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
-    walk_persistent<lane::ZTLane>(A, lds);
+    walk_persistent<lane::ZTLane, false>(A, lds);
+}
+
+// Complexity minimizer (minimizeKernel, src/entities/complexity_minimizer.cu:190-214).
+__global__ void __launch_bounds__(kThreadsPerGroup) minimize_z2_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2Lane, true>(A, lds);
+}
+
+__global__ void __launch_bounds__(kThreadsPerGroup) minimize_zt_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::ZTLane, true>(A, lds);
+}
+
+// copyTo of the minimizer's re-broadcast (complexity_minimizer.cu:130-133):
+// the best scheme of walker `src` becomes the current scheme of walkers
+// dst[0..count).  One thread per (destination, component, term).
+__global__ void broadcast_best_kernel(uint32_t *cur, int32_t *m, const uint32_t *best, const int32_t *best_m,
+                                      int walkers, int max_m, int src, const int32_t *dst, int count) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long per = 3L * max_m;
+    if (g >= per * count) return;
+    const int q = (int)(g / per);
+    const int c = (int)((g % per) / max_m);
+    const int t = (int)(g % max_m);
+    const int d = dst[q];
+    if (d < 0 || d >= walkers) return;
+    const int ms = best_m[src];
+    if (t < ms) cur[((size_t)c * max_m + t) * walkers + d] = best[((size_t)src * 3 + c) * max_m + t];
+    if (c == 0 && t == 0) m[d] = ms;
 }
 
 // Fills n int32 with v (used instead of a memset so launches can be captured
@@ -108,19 +139,40 @@ __global__ void verify_zt_kernel(VerifyArgs V) {
 
 size_t walk_lds_bytes(int ring, int max_m) { return ring ? zt_lds_bytes(max_m) : z2_lds_bytes(max_m); }
 
-hipError_t prepare_walk(int ring, int max_m, int *groups_per_cu) {
+static const void *walk_kernel(int ring, bool minimize) {
+    if (minimize)
+        return ring ? reinterpret_cast<const void *>(minimize_zt_kernel) : reinterpret_cast<const void *>(minimize_z2_kernel);
+    return ring ? reinterpret_cast<const void *>(walk_zt_kernel) : reinterpret_cast<const void *>(walk_z2_kernel);
+}
+
+hipError_t prepare_walk(int ring, int max_m, bool minimize, int *groups_per_cu) {
     const size_t lds = walk_lds_bytes(ring, max_m);
-    const void *fn = ring ? reinterpret_cast<const void *>(walk_zt_kernel) : reinterpret_cast<const void *>(walk_z2_kernel);
+    const void *fn = walk_kernel(ring, minimize);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu, fn, kThreadsPerGroup, lds);
 }
 
-hipError_t launch_walk(int ring, const WalkArgs &a, int grid, hipStream_t stream) {
+hipError_t launch_walk(int ring, bool minimize, const WalkArgs &a, int grid, hipStream_t stream) {
     const size_t lds = walk_lds_bytes(ring, a.max_m);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, stream, a.queue, 0, 1);
-    if (ring) hipLaunchKernelGGL(walk_zt_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
-    else hipLaunchKernelGGL(walk_z2_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
+    if (minimize) {
+        if (ring) hipLaunchKernelGGL(minimize_zt_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
+        else hipLaunchKernelGGL(minimize_z2_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
+    } else {
+        if (ring) hipLaunchKernelGGL(walk_zt_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
+        else hipLaunchKernelGGL(walk_z2_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_broadcast_best(uint32_t *cur, int32_t *m, const uint32_t *best, const int32_t *best_m, int walkers,
+                                 int max_m, int src, const int32_t *dst, int count, hipStream_t stream) {
+    const long threads = 3L * max_m * count;
+    if (threads == 0) return hipSuccess;
+    const int block = 256;
+    hipLaunchKernelGGL(broadcast_best_kernel, dim3((unsigned)((threads + block - 1) / block)), dim3(block), 0, stream,
+                       cur, m, best, best_m, walkers, max_m, src, dst, count);
     return hipGetLastError();
 }
 

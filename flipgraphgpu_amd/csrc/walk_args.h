@@ -54,6 +54,7 @@ struct WalkArgs {
     const int32_t *item_end;
     int32_t nitems;
     int32_t *queue;       // next unclaimed item; reset to 0 before each launch
+    int32_t *cx;          // [walkers] complexity of the current scheme after the launch (may be null)
 };
 
 // Error bits written by the walk kernels.

@@ -9,6 +9,8 @@ namespace fgk {
 namespace lane {
 
 struct Z2Lane : LaneBase {
+    static constexpr uint32_t kValueMask = 0xFFFFFFFFu;
+
     // removeZeroes (scheme_z2.cu:152-164), including its trailing trim and
     // the missing re-check of a swapped-in term.
     FGK_HD void remove_zeroes() {
@@ -111,6 +113,7 @@ struct Z2Lane : LaneBase {
             else reduce(rB == 1 ? j : i, tB, i2);
             return;
         }
+        pop_update(fj1, A, fk2, B);
 #if defined(__HIP_DEVICE_COMPILE__)
         const uint32_t before = Sc(mc);
         leave(mc, x);

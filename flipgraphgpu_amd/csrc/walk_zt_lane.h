@@ -59,6 +59,8 @@ FGK_HD uint32_t inverse(uint32_t a) { return pack(val(a), ~sgn(a) & val(a)); }
 }  // namespace tern
 
 struct ZTLane : LaneBase {
+    static constexpr uint32_t kValueMask = 0xFFFFu;  // magnitudes; signs sit in bits 16-31
+
     uint16_t *V;  // lane-offset base of the visited-candidate list
 
     FGK_HD void lane_extra(uint8_t *base, int lane) { V = reinterpret_cast<uint16_t *>(base) + lane; }
@@ -201,6 +203,7 @@ struct ZTLane : LaneBase {
             else apply_rule(rB, i, j, tB, i2);
             return;
         }
+        pop_update(fj1, A, fk2, B);
 #if defined(__HIP_DEVICE_COMPILE__)
         const uint32_t before = Sc(mc);
         leave(mc, x);

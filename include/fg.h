@@ -83,6 +83,36 @@ FG_API int fg_init_naive(fg_ctx *ctx, int n1, int n2, int n3);
  * round-robin.  Returns -2 if a scheme fails the Brent check. */
 FG_API int fg_init_from_schemes(fg_ctx *ctx, const int32_t *blob, int64_t len);
 
+/* ComplexityMinimizer::read + initialize + initializeKernel
+ * (src/entities/complexity_minimizer.cu:22-37, 67-77, 172-188): blob as for
+ * fg_init_from_schemes (one format for all schemes, no reduction).  Walker w
+ * starts from scheme w % count, its best scheme is its start and every
+ * walker's best complexity starts at the lowest complexity among the inputs.
+ * The pool then runs fg_minimize instead of fg_walk; fg_best picks the
+ * walker of lowest best complexity.  Returns -2 if a scheme fails the Brent
+ * check. */
+FG_API int fg_init_minimizer(fg_ctx *ctx, const int32_t *blob, int64_t len);
+
+/* ComplexityMinimizer::minimizeIteration / minimizeKernel
+ * (src/entities/complexity_minimizer.cu:79-84, 190-214): every walker flips
+ * without reductions (tryFlip(state, false)) up to max_iterations times,
+ * stopping at the first flip that finds no pair, and keeps the scheme of
+ * lowest complexity below its best complexity.  Runs the GPU Brent check of
+ * every current scheme; returns the number of invalid ones or < 0. */
+FG_API int fg_minimize(fg_ctx *ctx);
+
+/* bestComplexities[] (minimizer pools only; best may be null otherwise) and
+ * getComplexity() of every walker's current scheme after the last launch
+ * (src/entities/complexity_minimizer.cu:108-128).  Each non-null output
+ * holds `walkers` values. */
+FG_API int fg_complexities(fg_ctx *ctx, int32_t *best, int32_t *cur);
+
+/* schemesBest[src].copyTo(schemes[dst[q]]) for q < count
+ * (src/entities/complexity_minimizer.cu:130-133): the best scheme of walker
+ * src becomes the current scheme of every listed walker (src may be listed).
+ * Random streams and best schemes are untouched. */
+FG_API int fg_broadcast_best(fg_ctx *ctx, int src, const int32_t *dst, int count);
+
 /* Several formats in one pool (no reference counterpart: the reference runs
  * one format per process; its resize edges mix formats only through
  * src/entities/flip_graph.cu:575-613).  Walkers are split between the

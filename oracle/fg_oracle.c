@@ -782,7 +782,32 @@ void fgo_walker_init(fgo_walker *w, const fgo_scheme *start, int best_rank, uint
     fgo_rng_init(&w->rng, seed, walker);
 }
 
+/* minimizeKernel (src/entities/complexity_minimizer.cu:190-214): flips without
+ * reductions until a flip fails; the walker's best_rank holds its best
+ * complexity and best the scheme that reached it. */
+static void minimize(fgo_walker *w, const fgo_walk_params *p) {
+    fgo_scheme *s = &w->cur;
+    for (int it = 0; it < p->iterations; it++) {
+        uint32_t b[4];
+        fgo_rng_main_block(&w->rng, b);
+        int rank = s->m;
+        int ok = s->ring == FGO_RING_Z2 ? try_flip_z2(s, b, p) : try_flip_zt(s, b, &w->rng, p);
+        if (!ok) break;
+        w->total_flips++;
+        w->sum_m += (uint64_t)rank;
+        int cx = fgo_complexity(s);
+        if (cx < w->best_rank) {
+            w->best_rank = cx;
+            fgo_copy(&w->best, s, 0);
+        }
+    }
+}
+
 void fgo_walk(fgo_walker *w, const fgo_walk_params *p) {
+    if (p->mode == 1) {
+        minimize(w, p);
+        return;
+    }
     fgo_scheme *s = &w->cur;
     int iters = p->iterations;
     if (p->random_iterations) iters = aux_randint(&w->rng, 1, p->max_iterations); /* :530 */
@@ -1102,7 +1127,8 @@ int fgo_check_flipset_walk(int ring, int n1, int n2, int n3, uint64_t seed, int 
     p.thr_best_eq = 42949673u;
     p.select_mode = FGO_SELECT_REFERENCE;
     p.check_reduce = 1;
-    p.zt_visit_cap = 64;
+    p.zt_visit_cap = 16;
+    p.mode = 0;
     int bad = 0;
     for (int q = 0; q < walkers; q++) {
         fgo_walker_init(w, s0, s0->m, seed, (uint32_t)q);

@@ -18,7 +18,7 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                                    int max_iterations, int random_iterations, int plus_iterations,
                                    int use_reduce, uint32_t thr_reduce, int use_expand, uint32_t thr_expand,
                                    uint32_t thr_best_eq, int check_reduce, int launches, int visit_cap,
-                                   int32_t *err_out) {
+                                   int minimize, int32_t *err_out) {
     int maxm = 1;
     for (int q = 0; q < count; ++q) {
         maxm = std::max(maxm, std::max(recs[q].m, recs[q].best_m));
@@ -29,6 +29,7 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
     std::vector<uint32_t> cur(plane), best(plane);
     std::vector<int32_t> m(W), bm(W), br(W), fc(W), ai(W), fmt((size_t)4 * W);
     std::vector<uint64_t> it(W), aux(W), tf(W), sm(W);
+    std::vector<int32_t> cx(W);
     for (int w = 0; w < W; ++w) {
         const fg_walker_record &r = recs[w];
         for (int c = 0; c < 3; ++c) fmt[4 * w + c] = r.n[c];
@@ -49,6 +50,7 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
     a.plus_iterations = plus_iterations; a.use_reduce = use_reduce; a.thr_reduce = thr_reduce;
     a.use_expand = use_expand; a.thr_expand = thr_expand; a.thr_best_eq = thr_best_eq;
     a.check_reduce = check_reduce; a.zt_visit_cap = visit_cap; a.err = &err;
+    a.cx = cx.data();
     std::vector<uint8_t> lds(ring ? fgk::zt_lds_bytes(maxm) : fgk::z2_lds_bytes(maxm), 0xCD);
     const int G = (W + fgk::kWalkersPerGroup - 1) / fgk::kWalkersPerGroup;
     for (int l = 0; l < launches; ++l)
@@ -56,8 +58,14 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
             for (int slot = 0; slot < fgk::kWalkersPerGroup; ++slot) {
                 // one call per walker: the host side of the lane code does both flip halves
                 const int lane = slot * fgk::kLanesPerWalker;
-                if (ring) fgk::lane::walk_lane<fgk::lane::ZTLane>(a, lds.data(), lane, g * fgk::kWalkersPerGroup + slot, W);
-                else fgk::lane::walk_lane<fgk::lane::Z2Lane>(a, lds.data(), lane, g * fgk::kWalkersPerGroup + slot, W);
+                const int w = g * fgk::kWalkersPerGroup + slot;
+                if (minimize) {
+                    if (ring) fgk::lane::walk_lane<fgk::lane::ZTLane, true>(a, lds.data(), lane, w, W);
+                    else fgk::lane::walk_lane<fgk::lane::Z2Lane, true>(a, lds.data(), lane, w, W);
+                } else {
+                    if (ring) fgk::lane::walk_lane<fgk::lane::ZTLane, false>(a, lds.data(), lane, w, W);
+                    else fgk::lane::walk_lane<fgk::lane::Z2Lane, false>(a, lds.data(), lane, w, W);
+                }
             }
     for (int w = 0; w < W; ++w) {
         fg_walker_record &r = recs[w];

@@ -19,7 +19,8 @@ def lib():
         L = ctypes.CDLL(LIB)
         L.emu_walk_records.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64] + \
             [ctypes.c_int] * 4 + [ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
-                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
 
@@ -31,6 +32,6 @@ def walk(recs, params, seed, launches=1, ring=O.RING_Z2):
     rc = lib().emu_walk_records(ring, recs.ctypes.data, len(recs), seed, p.iterations, p.max_iterations,
                                 p.random_iterations, p.plus_iterations, p.use_reduce, p.thr_reduce,
                                 p.use_expand, p.thr_expand, p.thr_best_eq, p.check_reduce, launches,
-                                p.zt_visit_cap, ctypes.byref(err))
+                                p.zt_visit_cap, p.mode, ctypes.byref(err))
     assert rc == 0
     return recs, err.value

@@ -49,6 +49,7 @@ class WalkParams(ctypes.Structure):
         ("select_mode", ctypes.c_int),
         ("check_reduce", ctypes.c_int),
         ("zt_visit_cap", ctypes.c_int),
+        ("mode", ctypes.c_int),
     ]
 
 
@@ -63,12 +64,12 @@ THR_BEST_EQ = 42949673  # 0.01 * 2^32, rounded up
 
 def make_params(iterations=1000, max_iterations=None, plus_iterations=8000, p_expand=0.0,
                 p_reduce=0.0, random_iterations=False, select_mode=SELECT_CANONICAL,
-                check_reduce=True, zt_visit_cap=16):
+                check_reduce=True, zt_visit_cap=16, mode=0):
     mi = max_iterations if max_iterations is not None else iterations
     return WalkParams(iterations, mi, int(random_iterations), plus_iterations,
                       int(p_reduce > 0), threshold(p_reduce, mi),
                       int(p_expand > 0), threshold(p_expand, mi),
-                      THR_BEST_EQ, select_mode, int(check_reduce), zt_visit_cap)
+                      THR_BEST_EQ, select_mode, int(check_reduce), zt_visit_cap, mode)
 
 
 def _build():

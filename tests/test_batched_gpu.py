@@ -71,6 +71,11 @@ def test_autotune_keeps_results_exact():
         c = e.counters()
     assert (gpu["it"] == gpu["it"][0]).all()
     iters_done = int(gpu["it"][0])
-    # autotune launches are real walk launches: replay the same launch sequence on the oracle
     assert c["iterations"] == W * iters_done
-    assert iters_done > 400
+    # 64 blocks of 32 walkers fit in one grid size, so the autotuner ran
+    # exactly one 100-iteration launch; replay 100 + 400 on the oracle
+    assert iters_done == 500
+    mid = O.walk_records(start, O.make_params(iterations=100, max_iterations=400, plus_iterations=200), seed,
+                         nthreads=8)
+    ora = O.walk_records(mid, O.make_params(iterations=400, plus_iterations=200), seed, nthreads=8)
+    assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
