@@ -88,6 +88,20 @@ def load_library():
     L.fg_read_walkers.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     L.fg_summary.argtypes = [vp, i32p, i32p, i32p, i32p, i32p]
     L.fg_write_walkers.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.fg_reducer_create.argtypes = [i32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                    ctypes.POINTER(vp)]
+    L.fg_reducer_destroy.argtypes = [vp]
+    L.fg_reducer_info.argtypes = [vp, i32p]
+    L.fg_reducer_scheme_stats.argtypes = [vp, i32p]
+    L.fg_reducer_run.argtypes = [vp, ctypes.c_int]
+    L.fg_reducer_update.argtypes = [vp, i32p]
+    L.fg_reducer_best.argtypes = [vp, i32p]
+    L.fg_reducer_rows.argtypes = [vp, ctypes.c_int, i32p]
+    L.fg_reducer_best_part.argtypes = [vp, ctypes.c_int, i32p, ctypes.c_int64]
+    L.fg_reducer_best_part.restype = ctypes.c_int64
+    L.fg_reducer_part.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int64]
+    L.fg_reducer_part.restype = ctypes.c_int64
+    L.fg_reducer_words.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
     L.fg_device_info.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                  ctypes.POINTER(ctypes.c_int), ctypes.c_char_p, ctypes.c_int]
     _lib = L
@@ -242,3 +256,86 @@ def verify(blobs, ring=RING_Z2):
     ok = np.zeros(len(blobs), dtype=np.int32)
     _check(L.fg_verify(ring, p, len(f), ok.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "fg_verify")
     return ok.astype(bool)
+
+
+def _parse_part(buf):
+    """Decode one serialised reducer component (fg_reducer_part layout)."""
+    count, real, fresh, naive, mode, sbits, additions = (int(x) for x in buf[:7])
+    p = 7
+    vars_ = [(int(buf[p + 2 * v]), int(buf[p + 2 * v + 1])) for v in range(fresh)]
+    p += 2 * fresh
+    expr = []
+    for _ in range(count):
+        n = int(buf[p])
+        expr.append([int(x) for x in buf[p + 1:p + 1 + n]])
+        p += 1 + n
+    scale = float(np.array([sbits], dtype=np.int32).view(np.float32)[0])
+    return {"count": count, "real": real, "fresh": fresh, "naive": naive, "mode": mode, "scale": scale,
+            "additions": additions, "vars": vars_, "expr": expr}
+
+
+class Reducer:
+    """Batched additions reducer on the current GPU (fg_reducer)."""
+
+    def __init__(self, blob, count=32, schemes_count=1, max_flips=0, seed=1):
+        self.L = load_library()
+        b, p = _i32(blob)
+        self.r = ctypes.c_void_p()
+        _check(self.L.fg_reducer_create(p, len(b), count, schemes_count, max_flips, seed, ctypes.byref(self.r)),
+               "fg_reducer_create")
+        self.count = count
+
+    def close(self):
+        if self.r:
+            self.L.fg_reducer_destroy(self.r)
+            self.r = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        out = np.zeros(16, dtype=np.int32)
+        _check(self.L.fg_reducer_info(self.r, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "fg_reducer_info")
+        keys = ["n1", "n2", "n3", "m", "count", "schemes_count", "max_flips", "top", "e_cap", "l_cap", "f_cap",
+                "p_cap", "h_bits", "lds"]
+        return {k: int(v) for k, v in zip(keys, out)}
+
+    def run(self, iteration=1):
+        _check(self.L.fg_reducer_run(self.r, iteration), "fg_reducer_run")
+
+    def update(self):
+        x = ctypes.c_int32()
+        _check(self.L.fg_reducer_update(self.r, ctypes.byref(x)), "fg_reducer_update")
+        return x.value
+
+    def best(self):
+        out = np.zeros(11, dtype=np.int32)
+        _check(self.L.fg_reducer_best(self.r, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "fg_reducer_best")
+        return {"additions": out[0:3].tolist(), "fresh": out[3:6].tolist(), "naive": out[6:9].tolist(),
+                "total": int(out[9]), "total_fresh": int(out[10])}
+
+    def part(self, reducer, component, cap=1 << 20):
+        out = np.zeros(cap, dtype=np.int32)
+        n = self.L.fg_reducer_part(self.r, reducer, component, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cap)
+        _check(int(n), "fg_reducer_part")
+        return _parse_part(out[:n])
+
+    def best_part(self, component, cap=1 << 20):
+        out = np.zeros(cap, dtype=np.int32)
+        n = self.L.fg_reducer_best_part(self.r, component, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cap)
+        _check(int(n), "fg_reducer_best_part")
+        return _parse_part(out[:n])
+
+    def words(self):
+        out = np.zeros(self.count * 4, dtype=np.uint64)
+        _check(self.L.fg_reducer_words(self.r, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))), "fg_reducer_words")
+        return out

@@ -107,6 +107,11 @@ struct fg_ctx {
 
 extern "C" const char *fg_last_error(void) { return g_err.c_str(); }
 
+namespace fgrt {
+// Error text for the other translation units of the library.
+void set_error(const std::string &msg) { g_err = msg; }
+}  // namespace fgrt
+
 extern "C" int fg_device_info(int *cu_count, int *clock_khz, int *lds_per_block, char *name, int name_cap) {
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));

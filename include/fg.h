@@ -168,6 +168,50 @@ FG_API int fg_shape(fg_ctx *ctx, int32_t *walkers, int32_t *n1, int32_t *n2, int
 FG_API int fg_read_walkers(fg_ctx *ctx, int begin, int count, fg_walker_record *out);
 FG_API int fg_write_walkers(fg_ctx *ctx, int begin, int count, const fg_walker_record *in);
 
+/* ---- Additions reducer (src/entities/scheme_additions_reducer.cu,
+ * src/entities/additions_reducer.cuh) ---- */
+typedef struct fg_reducer fg_reducer;
+
+/* SchemeAdditionsReducer constructor + read + initialize
+ * (src/entities/scheme_additions_reducer.cu:3-27, 29-101, 144-175): blob is
+ * one scheme [n1 n2 n3 m U V W] over the integers (fixSigns is applied, the
+ * Brent check must pass; -2 otherwise).  count reducers run per pass;
+ * schemes_count as --schemes-count; max_flips > 0 is not supported. */
+FG_API int fg_reducer_create(const int32_t *blob, int64_t len, int count, int schemes_count, int max_flips,
+                             uint64_t seed, fg_reducer **out);
+FG_API int fg_reducer_destroy(fg_reducer *r);
+/* 16 values: n1 n2 n3 m count schemes_count max_flips top e_cap l_cap f_cap
+ * p_cap h_bits lds_bytes 0 0. */
+FG_API int fg_reducer_info(fg_reducer *r, int32_t *out);
+/* naive additions, max real variables and max subexpressions of U, V, W
+ * (getMaxRealVariables / getMaxSubexpressions, src/schemes/scheme_integer.cu:149-211). */
+FG_API int fg_reducer_scheme_stats(fg_reducer *r, int32_t *out9);
+/* reduceIteration (src/entities/scheme_additions_reducer.cu:177-188):
+ * one runReducersKernel pass of every reducer on the GPU, results read back. */
+FG_API int fg_reducer_run(fg_reducer *r, int iteration);
+/* updateBest (src/entities/scheme_additions_reducer.cu:287-307) without the
+ * printing and saving: *improved = 0 (no), 1 (fewer additions) or 2 (same
+ * additions, fewer fresh variables). */
+FG_API int fg_reducer_update(fg_reducer *r, int32_t *improved);
+/* 11 values: best additions U V W, best fresh variables U V W, naive
+ * additions U V W, total additions, total fresh variables. */
+FG_API int fg_reducer_best(fg_reducer *r, int32_t *out11);
+/* Report rows (src/entities/scheme_additions_reducer.cu:309-363): for each
+ * row and component 6 values: reducer, mode, scale (float bits), naive,
+ * reduced, fresh.  Returns the number of rows. */
+FG_API int fg_reducer_rows(fg_reducer *r, int rows, int32_t *out);
+/* getMode (src/entities/additions_reducer.cuh:185-204). */
+FG_API int fg_reducer_mode_name(int mode, float scale, char *buf, int cap);
+/* One component of the best reducer (reducers[count] of the reference) or
+ * of reducer `reducer` after the last pass, serialised as int32: count real
+ * fresh naive mode scale_bits additions, 2 * fresh fresh-variable operands,
+ * then per expression its size and entries (+-(index + 1)).  Returns the
+ * number of values written. */
+FG_API int64_t fg_reducer_best_part(fg_reducer *r, int component, int32_t *out, int64_t cap);
+FG_API int64_t fg_reducer_part(fg_reducer *r, int reducer, int component, int32_t *out, int64_t cap);
+/* Words consumed per random stream: [count][4] (components U V W, shared decisions). */
+FG_API int fg_reducer_words(fg_reducer *r, uint64_t *out);
+
 /* Device properties used for roofline reporting. */
 FG_API int fg_device_info(int *cu_count, int *clock_khz, int *lds_per_block, char *name, int name_cap);
 

@@ -173,6 +173,21 @@ int fgo_blob_pairs(int ring, const int32_t *blob, int mode, uint32_t *out, int c
 int fgo_check_flipset_walk(int ring, int n1, int n2, int n3, uint64_t seed, int iterations, int walkers,
                            int plus_iterations);
 
+/* Additions reducer (fg_reducer.c). */
+typedef struct fgo_reducer fgo_reducer;
+fgo_reducer *fgo_reducer_new(int max_expr, int max_len, int max_fresh, int max_pairs);
+void fgo_reducer_free(fgo_reducer *r);
+void fgo_reducer_clear(fgo_reducer *r);
+int fgo_reducer_add(fgo_reducer *r, const int *values, int n);
+void fgo_reducer_set_mode(fgo_reducer *r, int mode);
+int fgo_reducer_additions(const fgo_reducer *r);
+int fgo_reducer_iteration(const int32_t *blob, int64_t len, int count, uint64_t seed, int independent,
+                          const int32_t *best_fresh, const int32_t *best_vars, uint64_t *words, int e_cap,
+                          int l_cap, int f_cap, int32_t *info, int32_t *size, int32_t *expr, int32_t *vars);
+int fgo_reducer_single(const int32_t *matrix, int n_expr, int n_vars, int mode, uint64_t seed, int r, int c,
+                       int max_fresh, int e_cap, int l_cap, int f_cap, int32_t *info, int32_t *size, int32_t *expr,
+                       int32_t *vars);
+
 #ifdef __cplusplus
 }
 #endif
