@@ -81,6 +81,19 @@ def cpu_baseline(n, plus_iterations, target_seconds):
                       f"from the naive scheme, {dt:.1f} s wall"}
 
 
+def profiled_traffic(path):
+    """HBM bytes per walk launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (counters cannot be read from inside
+    the timed run); None when the summary is absent."""
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     a = parse()
     if a.calibrate_copy:
@@ -119,6 +132,9 @@ def main():
     e.close()
 
     peak = info["cu"] * info["clock_khz"] * 1e3 * 64
+    traffic_src = "profiles/walk_z2_3x3x3_hbm_traffic.md"
+    default_shape = n == (3, 3, 3) and a.walkers == 65536 and a.iterations == 10000
+    traffic = profiled_traffic(os.path.join(ROOT, traffic_src)) if default_shape else None
     achieved = ops / (kernel_ms / 1e3) if kernel_ms > 0 else 0.0
     out = {
         "metric": "flips/s (Z2 flip-graph random walk, 3x3x3, 65536 walkers)",
@@ -148,7 +164,9 @@ def main():
             "peak": peak / 1e12,
             "unit": "TOP/s",
             "frac": achieved / peak if peak else None,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "traffic_source": traffic_src if traffic is not None else None,
             "kernel_ms_per_step": kernel_ms / a.steps,
             "ops_per_flip": ops / flips if flips else None,
             "device": info,
