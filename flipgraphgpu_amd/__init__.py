@@ -3,8 +3,8 @@
 The search itself runs in HIP kernels behind the C ABI in include/fg.h;
 this package only binds that ABI for tests and benchmarks.
 """
-from ._lib import (RECORD_DTYPE, RING_Z2, RING_ZT, Engine, FgConfig, FgError, Reducer, device_info,
+from ._lib import (RECORD_DTYPE, RING_Z2, RING_ZT, Engine, FgConfig, FgError, Reducer, Scorer, device_info,
                    load_library, verify)
 
-__all__ = ["Engine", "FgConfig", "FgError", "RECORD_DTYPE", "RING_Z2", "RING_ZT", "Reducer", "device_info",
+__all__ = ["Engine", "FgConfig", "FgError", "RECORD_DTYPE", "RING_Z2", "RING_ZT", "Reducer", "Scorer", "device_info",
            "load_library", "verify"]

@@ -88,6 +88,25 @@ def load_library():
     L.fg_read_walkers.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     L.fg_summary.argtypes = [vp, i32p, i32p, i32p, i32p, i32p]
     L.fg_write_walkers.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.fg_log_enable.argtypes = [vp, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
+    L.fg_log_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                              ctypes.POINTER(ctypes.c_int64)]
+    L.fg_log_read.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, vp, vp, vp, vp]
+    L.fg_set_forced.argtypes = [vp, vp]
+    L.fg_get_forced.argtypes = [vp, vp]
+    L.fg_walk_forced.argtypes = [vp, ctypes.c_int]
+    L.fg_scorer_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                   ctypes.POINTER(vp)]
+    L.fg_scorer_destroy.argtypes = [vp]
+    L.fg_scorer_num_params.argtypes = [vp]
+    L.fg_scorer_num_params.restype = ctypes.c_int64
+    L.fg_scorer_layout.argtypes = [vp, i32p]
+    L.fg_scorer_get_params.argtypes = [vp, vp]
+    L.fg_scorer_set_params.argtypes = [vp, vp]
+    L.fg_scorer_eval.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp]
+    L.fg_scorer_train.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, vp]
+    L.fg_scorer_propose.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32]
+    L.fg_walk_learned.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32]
     L.fg_reducer_create.argtypes = [i32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                     ctypes.POINTER(vp)]
     L.fg_reducer_destroy.argtypes = [vp]
@@ -195,6 +214,40 @@ class Engine:
     def broadcast_best(self, src, dst):
         d, p = _i32(dst)
         _check(self.L.fg_broadcast_best(self.ctx, int(src), p, len(d)), "fg_broadcast_best")
+
+    # ---- learned proposals ----
+    def log_enable(self, max_batches, every=1, horizon=10):
+        _check(self.L.fg_log_enable(self.ctx, max_batches, every, horizon), "fg_log_enable")
+
+    def log_info(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(self.L.fg_log_info(self.ctx, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "fg_log_info")
+        return {"rows": a.value, "positives": b.value, "batches": c.value}
+
+    def log_read(self, begin, count, max_terms):
+        words = np.zeros((count, 3, max_terms), dtype=np.uint32)
+        m = np.zeros(count, dtype=np.int32)
+        pairs = np.zeros((count, 3), dtype=np.int32)
+        y = np.zeros(count, dtype=np.float32)
+        _check(self.L.fg_log_read(self.ctx, begin, count, words.ctypes.data, m.ctypes.data, pairs.ctypes.data,
+                                  y.ctypes.data), "fg_log_read")
+        return words, m, pairs, y
+
+    def set_forced(self, forced):
+        f = np.ascontiguousarray(forced, dtype=np.int32)
+        _check(self.L.fg_set_forced(self.ctx, f.ctypes.data), "fg_set_forced")
+
+    def get_forced(self):
+        f = np.zeros((self.walkers, 4), dtype=np.int32)
+        _check(self.L.fg_get_forced(self.ctx, f.ctypes.data), "fg_get_forced")
+        return f
+
+    def walk_forced(self, iterations=0):
+        return _check(self.L.fg_walk_forced(self.ctx, iterations), "fg_walk_forced")
+
+    def walk_learned(self, scorer, iterations, candidates=8, every=2, round_=0):
+        return _check(self.L.fg_walk_learned(self.ctx, scorer.s, iterations, candidates, every, round_),
+                      "fg_walk_learned")
 
     def autotune(self, iterations):
         g = ctypes.c_int32()
@@ -339,3 +392,70 @@ class Reducer:
         out = np.zeros(self.count * 4, dtype=np.uint64)
         _check(self.L.fg_reducer_words(self.r, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))), "fg_reducer_words")
         return out
+
+
+class Scorer:
+    """Learned flip scorer (fg_scorer): BF16 term-set network on the MFMA units."""
+
+    def __init__(self, n1, n2, n3, max_terms, seed=1):
+        self.L = load_library()
+        self.s = ctypes.c_void_p()
+        _check(self.L.fg_scorer_create(n1, n2, n3, max_terms, seed, ctypes.byref(self.s)), "fg_scorer_create")
+        self.max_terms = max_terms
+        self.nparams = int(self.L.fg_scorer_num_params(self.s))
+
+    def close(self):
+        if self.s:
+            self.L.fg_scorer_destroy(self.s)
+            self.s = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def layout(self):
+        out = np.zeros(16, dtype=np.int32)
+        _check(self.L.fg_scorer_layout(self.s, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "fg_scorer_layout")
+        return out
+
+    def get_params(self):
+        out = np.zeros(self.nparams, dtype=np.float32)
+        _check(self.L.fg_scorer_get_params(self.s, out.ctypes.data), "fg_scorer_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(self.L.fg_scorer_set_params(self.s, p.ctypes.data), "fg_scorer_set_params")
+
+    def eval(self, words, m, pairs, labels=None):
+        """(scores, loss, grads); loss and grads only with labels."""
+        words = np.ascontiguousarray(words, dtype=np.uint32)
+        m = np.ascontiguousarray(m, dtype=np.int32)
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32)
+        n = len(m)
+        scores = np.zeros(n, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        grads = np.zeros(self.nparams, dtype=np.float32) if labels is not None else None
+        lab = None if labels is None else np.ascontiguousarray(labels, dtype=np.float32)
+        _check(self.L.fg_scorer_eval(self.s, n, words.ctypes.data, m.ctypes.data, pairs.ctypes.data,
+                                     None if lab is None else lab.ctypes.data, scores.ctypes.data,
+                                     None if lab is None else loss.ctypes.data,
+                                     None if lab is None else grads.ctypes.data), "fg_scorer_eval")
+        return scores, (float(loss[0]) if lab is not None else None), grads
+
+    def train(self, engine, steps, batch=256, lr=1e-3, seed=1):
+        losses = np.zeros(steps, dtype=np.float32)
+        _check(self.L.fg_scorer_train(self.s, engine.ctx, steps, batch, lr, seed, losses.ctypes.data),
+               "fg_scorer_train")
+        return losses
+
+    def propose(self, engine, candidates=8, every=2, round_=0):
+        _check(self.L.fg_scorer_propose(self.s, engine.ctx, candidates, every, round_), "fg_scorer_propose")

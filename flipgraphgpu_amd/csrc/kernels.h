@@ -9,15 +9,19 @@ namespace fgk {
 
 // ring 0 = Z2, 1 = integer ring with packed ternary words
 size_t walk_lds_bytes(int ring, int max_m);
-// Sets the kernel's LDS limit and reports how many workgroups fit per CU
-// (minimize selects the complexity-minimizer kernel).
-hipError_t prepare_walk(int ring, int max_m, bool minimize, int *groups_per_cu);
+// Sets the kernel's LDS limit and reports how many workgroups fit per CU.
+// mode: 0 walk, 1 complexity minimizer, 2 walk with proposals and logging (Z2 only).
+hipError_t prepare_walk(int ring, int max_m, int mode, int *groups_per_cu);
 // Resets a.queue and launches `grid` persistent workgroups.
-hipError_t launch_walk(int ring, bool minimize, const WalkArgs &a, int grid, hipStream_t stream);
+hipError_t launch_walk(int ring, int mode, const WalkArgs &a, int grid, hipStream_t stream);
 // Current scheme of walkers dst[0..count) := best scheme of walker src.
 hipError_t launch_broadcast_best(uint32_t *cur, int32_t *m, const uint32_t *best, const int32_t *best_m, int walkers,
                                  int max_m, int src, const int32_t *dst, int count, hipStream_t stream);
 hipError_t launch_verify(int ring, const VerifyArgs &v, hipStream_t stream);
 hipError_t launch_fill(int32_t *p, int32_t v, int n, hipStream_t stream);
+// Labels one logged batch of the trajectory ring into the log rows.
+hipError_t launch_log_label(const uint32_t *rwords, const int32_t *rmeta, const int32_t *rminm, int horizon, int W,
+                            int every, int nslot, int maxm, uint32_t *words, int32_t *m, int32_t *pairs, float *y,
+                            hipStream_t stream);
 
 }  // namespace fgk

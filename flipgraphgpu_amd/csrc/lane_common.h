@@ -112,6 +112,7 @@ struct LaneBase {
     int visit_cap;  // integer ring: candidates one tryFlip may visit
     uint32_t vmask; // coefficient bits of a factor word (all for Z2, bits 0-15 for the integer ring)
     int pop;        // coefficient count of the scheme, kept for getComplexity
+    int sel_c, sel_t1, sel_t2;  // pair of the last tryFlip (component, index1 < index2)
 
     // Extra per-lane LDS beyond the common planes (none by default).
     FGK_HD void lane_extra(uint8_t *, int) {}
@@ -433,15 +434,24 @@ FGK_HD bool wave_any(bool v) {
 #endif
 }
 
+// Modes of walk_lane.
+constexpr int kModeWalk = 0;      // randomWalkKernel
+constexpr int kModeMinimize = 1;  // minimizeKernel
+constexpr int kModeLearn = 2;     // walk + proposed first flip + trajectory logging
+
 // The walk of one lane (randomWalkKernel body, src/entities/flip_graph.cu:520-573),
-// or with kMinimize the complexity minimizer's (minimizeKernel,
+// or in kModeMinimize the complexity minimizer's (minimizeKernel,
 // src/entities/complexity_minimizer.cu:190-214): flips without reductions
 // until one fails, keeping the scheme of lowest complexity; best_rank then
-// holds that complexity.  `lds` is the workgroup's LDS block, `lane` the lane
-// in the workgroup (walker slot lane / 2, flip side lane % 2), `w` the walker
-// index.  The host emulation calls it once per walker with an even lane and
-// does both sides.
-template <class Lane, bool kMinimize>
+// holds that complexity.  kModeLearn is the walk with two additions at the
+// first iteration of a launch: a proposed pair (A.forced) replaces the
+// uniform pair draw when it is still a flip, and logged walkers record their
+// scheme and the pair they flip (A.log_words / A.log_meta); the lowest rank
+// of the launch goes to A.minm.  `lds` is the workgroup's LDS block, `lane`
+// the lane in the workgroup (walker slot lane / 2, flip side lane % 2), `w`
+// the walker index.  The host emulation calls it once per walker with an
+// even lane and does both sides.
+template <class Lane, int kMode>
 FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_end) {
     const bool active = w < w_end;
     const int W = A.walkers;
@@ -492,7 +502,7 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     s.nn2 = s.n2 * s.n0;
     if (active) s.rebuild();
 
-    if constexpr (kMinimize) {
+    if constexpr (kMode == kModeMinimize) {
         const int iters = active ? A.iterations : 0;
         const int trips = wave_max(iters);
         bool live = active;
@@ -528,6 +538,8 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     // every lane runs the same number of loop trips so the wave stays
     // converged at the cooperative best copy
     const int trips = wave_max(iters);
+    int min_m = s.m;
+    const int log_slot = (kMode == kModeLearn && A.log_every > 0 && w % A.log_every == 0) ? w / A.log_every : -1;
 
     for (int iter = 0; iter < trips; ++iter) {
         bool flipped = false, copy = false;
@@ -536,7 +548,28 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
             philox10((uint32_t)it, (uint32_t)(it >> 32), (uint32_t)w, 0u, s.k0, s.k1, b0, b1, b2, b3);
             ++it;
             const int rank = s.m;
-            if (!s.try_flip(b0, b1, check_reduce)) {  // flip_graph.cu:535-537
+            bool ok = false;
+            if constexpr (kMode == kModeLearn) {
+                if (iter == 0) {
+                    if (log_slot >= 0 && A.log_words && s.side == 0)
+                        for (int c = 0; c < 3; ++c)
+                            for (int t = 0; t < s.m; ++t)
+                                A.log_words[((size_t)log_slot * 3 + c) * maxm + t] = s.f(c, t);
+                    if (A.forced && A.forced[4 * w] >= 0)
+                        ok = s.try_flip_pair(A.forced[4 * w], A.forced[4 * w + 1], A.forced[4 * w + 2], b1, check_reduce);
+                }
+                if (!ok) ok = s.try_flip(b0, b1, check_reduce);
+                if (iter == 0 && log_slot >= 0 && A.log_meta && s.side == 0) {
+                    int32_t *lm = A.log_meta + (size_t)log_slot * 4;
+                    lm[0] = ok ? s.sel_c : -1;
+                    lm[1] = s.sel_t1;
+                    lm[2] = s.sel_t2;
+                    lm[3] = rank;
+                }
+            } else {
+                ok = s.try_flip(b0, b1, check_reduce);
+            }
+            if (!ok) {  // flip_graph.cu:535-537
                 s.try_expand(1 + (int)(b1 % 2u));
             } else {
                 flipped = true;
@@ -559,7 +592,9 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
             }
             if (flips_count >= A.plus_iterations && s.try_plus()) flips_count = 0;  // :564-565
         }
+        min_m = s.m < min_m ? s.m : min_m;
     }
+    if (kMode == kModeLearn && active && s.side == 0 && A.minm) A.minm[w] = min_m;
     }
 
     if (active && s.side == 0) {

@@ -14,6 +14,7 @@
 
 #include "fg.h"
 #include "kernels.h"
+#include "runtime_internal.h"
 #include "scheme_host.h"
 
 namespace {
@@ -73,8 +74,35 @@ struct fg_ctx {
     uint64_t *it = nullptr, *aux = nullptr, *total_flips = nullptr, *sum_m = nullptr;
     int32_t *item_begin = nullptr, *item_end = nullptr, *queue = nullptr;
     int32_t *cx = nullptr, *dst = nullptr;
+    int32_t *forced = nullptr;  // [W][4] proposed first flips (learned walks)
     int nitems = 0;
     bool minimize = false;  // pool set up by fg_init_minimizer
+
+    // trajectory log: ring of the last `horizon` launches, labelled rows
+    struct Log {
+        bool on = false;
+        int every = 1, horizon = 10, nslot = 0;
+        long long max_batches = 0, launches = 0, labelled = 0, positives = 0;
+        uint32_t *ring_words = nullptr, *words = nullptr;
+        int32_t *ring_meta = nullptr, *ring_minm = nullptr, *m = nullptr, *pairs = nullptr;
+        float *y = nullptr;
+        std::vector<int64_t> valid;   // labelled rows
+        std::vector<int32_t> vpairs;  // their pairs (3 per row) and labels, on the host
+        std::vector<float> vy;
+        void release() {
+            void *ptrs[] = {ring_words, words, ring_meta, ring_minm, m, pairs, y};
+            for (void *p : ptrs)
+                if (p) (void)hipFree(p);
+            ring_words = words = nullptr;
+            ring_meta = ring_minm = m = pairs = nullptr;
+            y = nullptr;
+            valid.clear();
+            vpairs.clear();
+            vy.clear();
+            on = false;
+            launches = labelled = positives = 0;
+        }
+    } log;
     int cus = 0;
     int groups_per_cu = 0;
     int grid = 0;
@@ -92,13 +120,14 @@ struct fg_ctx {
     void release() {
         drop_graph();
         void *ptrs[] = {cur, cur_s, best, best_s, m, best_m, best_rank, flips_count, aux_idx,
-                        fmt, err, ok, it, aux, total_flips, sum_m, item_begin, item_end, queue, cx, dst};
+                        fmt, err, ok, it, aux, total_flips, sum_m, item_begin, item_end, queue, cx, dst, forced};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         cur = cur_s = best = best_s = nullptr;
         m = best_m = best_rank = flips_count = aux_idx = fmt = err = ok = nullptr;
         it = aux = total_flips = sum_m = nullptr;
-        item_begin = item_end = queue = cx = dst = nullptr;
+        item_begin = item_end = queue = cx = dst = forced = nullptr;
+        log.release();
         nitems = 0;
         minimize = false;
         ready = false;
@@ -264,11 +293,18 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
     HIPCHK(dalloc(&ctx->queue, 1));
     HIPCHK(dalloc(&ctx->cx, W));
     HIPCHK(dalloc(&ctx->dst, W));
+    HIPCHK(dalloc(&ctx->forced, (size_t)4 * W));
+    HIPCHK(fgk::launch_fill(ctx->forced, -1, 4 * W, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->minimize = minimize;
     ctx->nitems = (int)ib.size();
     HIPCHK(hipMemcpy(ctx->item_begin, ib.data(), ib.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->item_end, ie.data(), ie.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(fgk::prepare_walk(ring, maxm, minimize, &ctx->groups_per_cu));
+    HIPCHK(fgk::prepare_walk(ring, maxm, minimize ? 1 : 0, &ctx->groups_per_cu));
+    if (!minimize && ring == FG_RING_Z2) {
+        int g2 = 0;
+        HIPCHK(fgk::prepare_walk(ring, maxm, 2, &g2));
+    }
     if (ctx->groups_per_cu < 1) return fail("init: the walk kernel does not fit on a CU with this scheme size");
     ctx->grid = ctx->cfg.grid > 0 ? ctx->cfg.grid : ctx->groups_per_cu * ctx->cus;
     ctx->grid = std::min(ctx->grid, ctx->nitems);
@@ -428,8 +464,49 @@ extern "C" int fg_init_batched(fg_ctx *ctx, int nformats, const int32_t *formats
     return init_pool(ctx, starts, assign, formats[0], formats[1], formats[2]);
 }
 
-extern "C" int fg_walk(fg_ctx *ctx, int iterations) {
+namespace {
+int walk_launch(fg_ctx *ctx, int iterations, bool use_forced);
+}
+
+extern "C" int fg_walk(fg_ctx *ctx, int iterations) { return walk_launch(ctx, iterations, false); }
+
+namespace fgrt {
+int walk_forced(fg_ctx *ctx, int iterations) { return walk_launch(ctx, iterations, true); }
+
+bool pool_view(fg_ctx *ctx, PoolView *v) {
+    if (!ctx || !ctx->ready) return false;
+    v->ring = ctx->cfg.ring;
+    v->W = ctx->W;
+    v->maxm = ctx->maxm;
+    v->cur = ctx->cur;
+    v->m = ctx->m;
+    v->fmt = ctx->fmt;
+    v->forced = ctx->forced;
+    v->stream = ctx->stream;
+    v->key0 = (uint32_t)ctx->cfg.seed;
+    v->key1 = (uint32_t)(ctx->cfg.seed >> 32);
+    return true;
+}
+
+bool log_view(fg_ctx *ctx, LogView *v) {
+    if (!ctx || !ctx->log.on) return false;
+    v->words = ctx->log.words;
+    v->m = ctx->log.m;
+    v->pairs = ctx->log.pairs;
+    v->y = ctx->log.y;
+    v->valid = &ctx->log.valid;
+    v->vpairs = &ctx->log.vpairs;
+    v->vy = &ctx->log.vy;
+    v->maxm = ctx->maxm;
+    return true;
+}
+}  // namespace fgrt
+
+namespace {
+int walk_launch(fg_ctx *ctx, int iterations, bool use_forced) {
     if (!ctx || !ctx->ready) return fail("fg_walk: context not initialised");
+    const bool learn = !ctx->minimize && (use_forced || ctx->log.on);
+    if (learn && ctx->cfg.ring != FG_RING_Z2) return fail("fg_walk: proposals and trajectory logs are built for Z2 walks");
     fgk::WalkArgs a{};
     a.cur = ctx->cur;
     a.best = ctx->best;
@@ -464,6 +541,17 @@ extern "C" int fg_walk(fg_ctx *ctx, int iterations) {
     a.nitems = ctx->nitems;
     a.queue = ctx->queue;
     a.cx = ctx->cx;
+    const int mode = ctx->minimize ? 1 : (learn ? 2 : 0);
+    const int slot = ctx->log.on ? (int)(ctx->log.launches % ctx->log.horizon) : 0;
+    if (learn) {
+        a.forced = use_forced ? ctx->forced : nullptr;
+        if (ctx->log.on) {
+            a.log_every = ctx->log.every;
+            a.log_words = ctx->log.ring_words + (size_t)slot * ctx->log.nslot * 3 * ctx->maxm;
+            a.log_meta = ctx->log.ring_meta + (size_t)slot * ctx->log.nslot * 4;
+            a.minm = ctx->log.ring_minm + (size_t)slot * ctx->W;
+        }
+    }
     if (ctx->minimize) {
         // minimizeKernel: maxIterations flips without reductions or expansions
         a.iterations = a.max_iterations = ctx->cfg.max_iterations;
@@ -477,11 +565,11 @@ extern "C" int fg_walk(fg_ctx *ctx, int iterations) {
     auto enqueue = [&](bool timed) -> int {
         HIPCHK(fgk::launch_fill(ctx->err, 0, 1, ctx->stream));
         if (timed) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-        HIPCHK(fgk::launch_walk(ctx->cfg.ring, ctx->minimize, a, ctx->grid, ctx->stream));
+        HIPCHK(fgk::launch_walk(ctx->cfg.ring, mode, a, ctx->grid, ctx->stream));
         if (timed) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         return verify_planes(ctx, ctx->cfg.ring, ctx->cur, ctx->m, ctx->fmt, ctx->W, ctx->maxm, ctx->ok);
     };
-    if (ctx->cfg.use_graph) {
+    if (ctx->cfg.use_graph && mode != 2) {
         if (ctx->graph_iterations != a.iterations) {
             ctx->drop_graph();
             HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -510,8 +598,98 @@ extern "C" int fg_walk(fg_ctx *ctx, int iterations) {
     HIPCHK(hipMemcpy(ok.data(), ctx->ok, ok.size() * 4, hipMemcpyDeviceToHost));
     int bad = 0;
     for (int x : ok) bad += x == 0;
+    if (ctx->log.on) {
+        // rows logged `horizon` launches ago now have their whole window
+        auto &L = ctx->log;
+        const long long s = L.launches - L.horizon + 1;
+        if (s >= 0 && s < L.max_batches) {
+            const size_t base = (size_t)s * L.nslot;
+            HIPCHK(fgk::launch_log_label(L.ring_words + (size_t)(s % L.horizon) * L.nslot * 3 * ctx->maxm,
+                                         L.ring_meta + (size_t)(s % L.horizon) * L.nslot * 4, L.ring_minm, L.horizon,
+                                         ctx->W, L.every, L.nslot, ctx->maxm, L.words + base * 3 * ctx->maxm,
+                                         L.m + base, L.pairs + base * 3, L.y + base, ctx->stream));
+            std::vector<float> y(L.nslot);
+            std::vector<int32_t> pr((size_t)L.nslot * 3);
+            HIPCHK(hipMemcpyAsync(y.data(), L.y + base, (size_t)L.nslot * 4, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipMemcpyAsync(pr.data(), L.pairs + base * 3, pr.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            for (int i = 0; i < L.nslot; ++i)
+                if (y[i] >= 0.f) {
+                    L.valid.push_back((int64_t)(base + i));
+                    L.vpairs.insert(L.vpairs.end(), pr.begin() + 3 * i, pr.begin() + 3 * i + 3);
+                    L.vy.push_back(y[i]);
+                    L.positives += y[i] > 0.5f;
+                }
+            ++L.labelled;
+        }
+        ++L.launches;
+    }
     return bad;
 }
+}  // namespace
+
+extern "C" int fg_log_enable(fg_ctx *ctx, int64_t max_batches, int every, int horizon) {
+    if (!ctx || !ctx->ready) return fail("fg_log_enable: context not initialised");
+    if (ctx->cfg.ring != FG_RING_Z2 || ctx->minimize) return fail("fg_log_enable: trajectory logs are built for Z2 walks");
+    if (max_batches < 1 || every < 1 || horizon < 1) return fail("fg_log_enable: bad arguments");
+    auto &L = ctx->log;
+    L.release();
+    L.every = every;
+    L.horizon = horizon;
+    L.nslot = (ctx->W + every - 1) / every;
+    L.max_batches = max_batches;
+    const size_t rows = (size_t)max_batches * L.nslot, wpr = (size_t)3 * ctx->maxm;
+    HIPCHK(dalloc(&L.ring_words, (size_t)horizon * L.nslot * wpr));
+    HIPCHK(dalloc(&L.ring_meta, (size_t)horizon * L.nslot * 4));
+    HIPCHK(dalloc(&L.ring_minm, (size_t)horizon * ctx->W));
+    HIPCHK(dalloc(&L.words, rows * wpr));
+    HIPCHK(dalloc(&L.m, rows));
+    HIPCHK(dalloc(&L.pairs, rows * 3));
+    HIPCHK(dalloc(&L.y, rows));
+    HIPCHK(fgk::launch_fill(L.ring_meta, -1, (int)std::min<size_t>((size_t)horizon * L.nslot * 4, INT_MAX), ctx->stream));
+    HIPCHK(fgk::launch_fill(L.ring_minm, INT_MAX, (int)std::min<size_t>((size_t)horizon * ctx->W, INT_MAX), ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    L.on = true;
+    return 0;
+}
+
+extern "C" int fg_log_info(fg_ctx *ctx, int64_t *rows, int64_t *positives, int64_t *batches) {
+    if (!ctx || !ctx->log.on) return fail("fg_log_info: no trajectory log");
+    if (rows) *rows = (int64_t)ctx->log.valid.size();
+    if (positives) *positives = ctx->log.positives;
+    if (batches) *batches = ctx->log.labelled;
+    return 0;
+}
+
+extern "C" int fg_log_read(fg_ctx *ctx, int64_t begin, int64_t count, uint32_t *words, int32_t *m, int32_t *pairs,
+                           float *labels) {
+    if (!ctx || !ctx->log.on) return fail("fg_log_read: no trajectory log");
+    const std::vector<int64_t> &v = ctx->log.valid;
+    if (begin < 0 || count < 0 || begin + count > (int64_t)v.size()) return fail("fg_log_read: range out of bounds");
+    const size_t wpr = (size_t)3 * ctx->maxm;
+    for (int64_t q = 0; q < count; ++q) {
+        const size_t r = (size_t)v[begin + q];
+        if (words) HIPCHK(hipMemcpy(words + q * wpr, ctx->log.words + r * wpr, wpr * 4, hipMemcpyDeviceToHost));
+        if (m) HIPCHK(hipMemcpy(m + q, ctx->log.m + r, 4, hipMemcpyDeviceToHost));
+        if (pairs) HIPCHK(hipMemcpy(pairs + q * 3, ctx->log.pairs + r * 3, 12, hipMemcpyDeviceToHost));
+        if (labels) HIPCHK(hipMemcpy(labels + q, ctx->log.y + r, 4, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+extern "C" int fg_set_forced(fg_ctx *ctx, const int32_t *forced) {
+    if (!ctx || !ctx->ready || !forced) return fail("fg_set_forced: bad arguments");
+    HIPCHK(hipMemcpy(ctx->forced, forced, (size_t)ctx->W * 16, hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int fg_get_forced(fg_ctx *ctx, int32_t *forced) {
+    if (!ctx || !ctx->ready || !forced) return fail("fg_get_forced: bad arguments");
+    HIPCHK(hipMemcpy(forced, ctx->forced, (size_t)ctx->W * 16, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int fg_walk_forced(fg_ctx *ctx, int iterations) { return fgrt::walk_forced(ctx, iterations); }
 
 extern "C" int fg_autotune(fg_ctx *ctx, int iterations, int32_t *grid) {
     if (!ctx || !ctx->ready) return fail("fg_autotune: context not initialised");

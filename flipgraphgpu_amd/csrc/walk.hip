@@ -16,7 +16,7 @@ namespace fgk {
 // launch with little tail.  Every wave exits once the queue is exhausted.
 // Inlined into each kernel so the argument block is read straight from the
 // kernel-argument segment (an out-of-line call would copy it to scratch).
-template <class Lane, bool kMinimize>
+template <class Lane, int kMode>
 __device__ __attribute__((always_inline)) void walk_persistent(const WalkArgs &A, uint8_t *lds) {
     const int lane = threadIdx.x;
     for (;;) {
@@ -24,29 +24,35 @@ __device__ __attribute__((always_inline)) void walk_persistent(const WalkArgs &A
         if (lane == 0) item = atomicAdd(A.queue, 1);
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= A.nitems) break;
-        lane::walk_lane<Lane, kMinimize>(A, lds, lane, A.item_begin[item] + lane / kLanesPerWalker, A.item_end[item]);
+        lane::walk_lane<Lane, kMode>(A, lds, lane, A.item_begin[item] + lane / kLanesPerWalker, A.item_end[item]);
     }
 }
 
 __global__ void __launch_bounds__(kThreadsPerGroup) walk_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
-    walk_persistent<lane::Z2Lane, false>(A, lds);
+    walk_persistent<lane::Z2Lane, lane::kModeWalk>(A, lds);
 }
 
 __global__ void __launch_bounds__(kThreadsPerGroup) walk_zt_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
-    walk_persistent<lane::ZTLane, false>(A, lds);
+    walk_persistent<lane::ZTLane, lane::kModeWalk>(A, lds);
 }
 
 // Complexity minimizer (minimizeKernel, src/entities/complexity_minimizer.cu:190-214).
 __global__ void __launch_bounds__(kThreadsPerGroup) minimize_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
-    walk_persistent<lane::Z2Lane, true>(A, lds);
+    walk_persistent<lane::Z2Lane, lane::kModeMinimize>(A, lds);
 }
 
 __global__ void __launch_bounds__(kThreadsPerGroup) minimize_zt_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
-    walk_persistent<lane::ZTLane, true>(A, lds);
+    walk_persistent<lane::ZTLane, lane::kModeMinimize>(A, lds);
+}
+
+// Walk with a proposed first flip and trajectory logging (Z2).
+__global__ void __launch_bounds__(kThreadsPerGroup) learn_z2_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2Lane, lane::kModeLearn>(A, lds);
 }
 
 // copyTo of the minimizer's re-broadcast (complexity_minimizer.cu:130-133):
@@ -65,6 +71,38 @@ __global__ void broadcast_best_kernel(uint32_t *cur, int32_t *m, const uint32_t 
     const int ms = best_m[src];
     if (t < ms) cur[((size_t)c * max_m + t) * walkers + d] = best[((size_t)src * 3 + c) * max_m + t];
     if (c == 0 && t == 0) m[d] = ms;
+}
+
+// Labels the flips logged `horizon` launches ago: one thread per log slot;
+// label 1 when the walker's rank fell below the rank at the flip during the
+// launches since (minm ring), 0 when it did not, -1 when no flip was logged.
+__global__ void log_label_kernel(const uint32_t *rwords, const int32_t *rmeta, const int32_t *rminm, int horizon, int W,
+                                 int every, int nslot, int maxm, uint32_t *words, int32_t *m, int32_t *pairs, float *y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslot) return;
+    const int32_t *meta = rmeta + (size_t)i * 4;
+    const int w = i * every;
+    if (meta[0] < 0 || w >= W) {
+        y[i] = -1.f;
+        pairs[3 * i] = -1;
+        m[i] = 0;
+        return;
+    }
+    int lo = 0x7FFFFFFF;
+    for (int q = 0; q < horizon; ++q) {
+        const int v = rminm[(size_t)q * W + w];
+        lo = v < lo ? v : lo;
+    }
+    const int rank = meta[3];
+    y[i] = lo < rank ? 1.f : 0.f;
+    m[i] = rank;
+    pairs[3 * i] = meta[0];
+    pairs[3 * i + 1] = meta[1];
+    pairs[3 * i + 2] = meta[2];
+    for (int q = 0; q < 3 * maxm; ++q) {
+        const int t = q % maxm;
+        words[(size_t)i * 3 * maxm + q] = t < rank ? rwords[(size_t)i * 3 * maxm + q] : 0u;
+    }
 }
 
 // Fills n int32 with v (used instead of a memset so launches can be captured
@@ -139,26 +177,31 @@ __global__ void verify_zt_kernel(VerifyArgs V) {
 
 size_t walk_lds_bytes(int ring, int max_m) { return ring ? zt_lds_bytes(max_m) : z2_lds_bytes(max_m); }
 
-static const void *walk_kernel(int ring, bool minimize) {
-    if (minimize)
+static const void *walk_kernel(int ring, int mode) {
+    if (mode == lane::kModeMinimize)
         return ring ? reinterpret_cast<const void *>(minimize_zt_kernel) : reinterpret_cast<const void *>(minimize_z2_kernel);
+    if (mode == lane::kModeLearn) return ring ? nullptr : reinterpret_cast<const void *>(learn_z2_kernel);
     return ring ? reinterpret_cast<const void *>(walk_zt_kernel) : reinterpret_cast<const void *>(walk_z2_kernel);
 }
 
-hipError_t prepare_walk(int ring, int max_m, bool minimize, int *groups_per_cu) {
+hipError_t prepare_walk(int ring, int max_m, int mode, int *groups_per_cu) {
     const size_t lds = walk_lds_bytes(ring, max_m);
-    const void *fn = walk_kernel(ring, minimize);
+    const void *fn = walk_kernel(ring, mode);
+    if (!fn) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu, fn, kThreadsPerGroup, lds);
 }
 
-hipError_t launch_walk(int ring, bool minimize, const WalkArgs &a, int grid, hipStream_t stream) {
+hipError_t launch_walk(int ring, int mode, const WalkArgs &a, int grid, hipStream_t stream) {
     const size_t lds = walk_lds_bytes(ring, a.max_m);
+    if (!walk_kernel(ring, mode)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, stream, a.queue, 0, 1);
-    if (minimize) {
+    if (mode == lane::kModeMinimize) {
         if (ring) hipLaunchKernelGGL(minimize_zt_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
         else hipLaunchKernelGGL(minimize_z2_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
+    } else if (mode == lane::kModeLearn) {
+        hipLaunchKernelGGL(learn_z2_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
     } else {
         if (ring) hipLaunchKernelGGL(walk_zt_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
         else hipLaunchKernelGGL(walk_z2_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
@@ -173,6 +216,14 @@ hipError_t launch_broadcast_best(uint32_t *cur, int32_t *m, const uint32_t *best
     const int block = 256;
     hipLaunchKernelGGL(broadcast_best_kernel, dim3((unsigned)((threads + block - 1) / block)), dim3(block), 0, stream,
                        cur, m, best, best_m, walkers, max_m, src, dst, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_log_label(const uint32_t *rwords, const int32_t *rmeta, const int32_t *rminm, int horizon, int W,
+                            int every, int nslot, int maxm, uint32_t *words, int32_t *m, int32_t *pairs, float *y,
+                            hipStream_t stream) {
+    hipLaunchKernelGGL(log_label_kernel, dim3((nslot + 255) / 256), dim3(256), 0, stream, rwords, rmeta, rminm, horizon,
+                       W, every, nslot, maxm, words, m, pairs, y);
     return hipGetLastError();
 }
 

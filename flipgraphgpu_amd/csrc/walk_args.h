@@ -55,6 +55,12 @@ struct WalkArgs {
     int32_t nitems;
     int32_t *queue;       // next unclaimed item; reset to 0 before each launch
     int32_t *cx;          // [walkers] complexity of the current scheme after the launch (may be null)
+    // learned-proposal launches (Z2 only)
+    const int32_t *forced;  // [walkers][4]: component, index1, index2, 1 of a proposed first flip (component -1: none)
+    int32_t log_every;      // walkers w with w % log_every == 0 log their first flip (0: none)
+    uint32_t *log_words;    // [walkers / log_every][3][max_m] scheme before that flip
+    int32_t *log_meta;      // [walkers / log_every][4]: component (-1: no flip), index1, index2, rank
+    int32_t *minm;          // [walkers] lowest rank during the launch
 };
 
 // Error bits written by the walk kernels.

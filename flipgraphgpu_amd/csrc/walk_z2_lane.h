@@ -259,6 +259,27 @@ struct Z2Lane : LaneBase {
         if (!total) return false;
         int c, t1, t2, j, k;
         if (!select_pair(draw_below(r, total), c, t1, t2)) return false;
+        sel_c = c;
+        sel_t1 = t1;
+        sel_t2 = t2;
+        orientation(c, coin, j, k);
+        if (coin & 2u) {
+            const int t = t1;
+            t1 = t2;
+            t2 = t;
+        }
+        flip(c, j, k, t1, t2, check_reduce);
+        return true;
+    }
+
+    // A given pair (component c, index1 t1 < index2 t2) flipped with the
+    // orientation coins of tryFlip; false when the factors differ.
+    FGK_HD bool try_flip_pair(int c, int t1, int t2, uint32_t coin, bool check_reduce) {
+        if (c < 0 || c > 2 || t1 < 0 || t2 <= t1 || t2 >= m || f(c, t1) != f(c, t2)) return false;
+        sel_c = c;
+        sel_t1 = t1;
+        sel_t2 = t2;
+        int j, k;
         orientation(c, coin, j, k);
         if (coin & 2u) {
             const int t = t1;

@@ -168,6 +168,52 @@ FG_API int fg_shape(fg_ctx *ctx, int32_t *walkers, int32_t *n1, int32_t *n2, int
 FG_API int fg_read_walkers(fg_ctx *ctx, int begin, int count, fg_walker_record *out);
 FG_API int fg_write_walkers(fg_ctx *ctx, int begin, int count, const fg_walker_record *in);
 
+/* ---- Learned proposals (no reference counterpart: the walk's uniform pair
+ * draw, src/schemes/scheme_z2.cu:982-1027, stays the fallback) ---- */
+
+/* Trajectory log of a Z2 pool: from now on every fg_walk launch records, for
+ * walkers w with w % every == 0, the scheme before the launch's first flip
+ * and the pair flipped.  `horizon` launches later the row is labelled 1 if
+ * the walker's rank fell below the rank at that flip in the meantime, else
+ * 0.  At most max_batches launches are kept. */
+FG_API int fg_log_enable(fg_ctx *ctx, int64_t max_batches, int every, int horizon);
+/* Labelled rows, rows labelled 1, labelled launches. */
+FG_API int fg_log_info(fg_ctx *ctx, int64_t *rows, int64_t *positives, int64_t *batches);
+/* Labelled rows begin..begin+count-1: words [count][3][max_terms] (factor
+ * masks, zero beyond the rank), m [count], pairs [count][3] (component,
+ * index1 < index2), labels [count].  Any output may be null. */
+FG_API int fg_log_read(fg_ctx *ctx, int64_t begin, int64_t count, uint32_t *words, int32_t *m, int32_t *pairs,
+                       float *labels);
+/* Proposed first flips per walker [walkers][4] = component (-1: none),
+ * index1, index2, 1; fg_walk_forced runs one launch whose first iteration
+ * flips the proposed pair when it still is a flip (uniform draw otherwise). */
+FG_API int fg_set_forced(fg_ctx *ctx, const int32_t *forced);
+FG_API int fg_get_forced(fg_ctx *ctx, int32_t *forced);
+FG_API int fg_walk_forced(fg_ctx *ctx, int iterations);
+
+typedef struct fg_scorer fg_scorer;
+/* Term-set flip scorer for n1 x n2 x n3 Z2 schemes of up to max_terms terms
+ * (BF16 weights on the MFMA units, FP32 accumulation). */
+FG_API int fg_scorer_create(int n1, int n2, int n3, int max_terms, uint64_t seed, fg_scorer **out);
+FG_API int fg_scorer_destroy(fg_scorer *s);
+FG_API int64_t fg_scorer_num_params(fg_scorer *s);
+/* 16 values: F D Z max_terms and the offsets of W0 b0 A1 B1 c1 A2 B2 c2 Wz bz wo bo. */
+FG_API int fg_scorer_layout(fg_scorer *s, int32_t *out);
+FG_API int fg_scorer_get_params(fg_scorer *s, float *out);
+FG_API int fg_scorer_set_params(fg_scorer *s, const float *in);
+/* Forward on a host batch: words [count][3][max_terms], m [count], pairs
+ * [count][3]; scores [count].  With labels: mean BCE loss and the gradient
+ * of every parameter (fg_scorer_layout order). */
+FG_API int fg_scorer_eval(fg_scorer *s, int count, const uint32_t *words, const int32_t *m, const int32_t *pairs,
+                          const float *labels, float *scores, float *loss, float *grads);
+/* Adam on batches sampled from ctx's trajectory log; losses[steps] (may be null). */
+FG_API int fg_scorer_train(fg_scorer *s, fg_ctx *ctx, int steps, int batch, float lr, uint64_t seed, float *losses);
+/* For walkers w with w % every == 0: draw `candidates` equal-factor pairs
+ * uniformly, score them and propose the best (fg_set_forced's layout). */
+FG_API int fg_scorer_propose(fg_scorer *s, fg_ctx *ctx, int candidates, int every, uint32_t round);
+/* fg_scorer_propose + fg_walk_forced. */
+FG_API int fg_walk_learned(fg_ctx *ctx, fg_scorer *s, int iterations, int candidates, int every, uint32_t round);
+
 /* ---- Additions reducer (src/entities/scheme_additions_reducer.cu,
  * src/entities/additions_reducer.cuh) ---- */
 typedef struct fg_reducer fg_reducer;

@@ -566,14 +566,30 @@ static void select_pair(const fgo_scheme *s, int mode, uint32_t r, int *c, int *
 
 /* Z2 tryFlip (scheme_z2.cu:982-1027): b[0] picks the pair (multiply-high), bit 0 of b[1]
  * swaps j/k, bit 1 swaps index1/index2. */
+/* pair (component, index1 < index2) of the last Z2 tryFlip of this thread */
+static __thread int tl_sel[3];
+
 static int try_flip_z2(fgo_scheme *s, const uint32_t b[4], const fgo_walk_params *p) {
     int size = total_pairs(s, p->select_mode);
     if (!size) return 0;
     int c, t1, t2, i, j, k;
     select_pair(s, p->select_mode, draw_below(b[0], (uint32_t)size), &c, &t1, &t2);
+    tl_sel[0] = c; tl_sel[1] = t1; tl_sel[2] = t2;
     pick_orientation(c, (int)b[1], &i, &j, &k);
     if (b[1] & 2) { int t = t1; t1 = t2; t2 = t; }
     flip_z2(s, i, j, k, t1, t2, p->check_reduce);
+    return 1;
+}
+
+/* A given pair flipped with tryFlip's orientation coins; 0 when the two
+ * factors differ (the pair is no longer a flip). */
+static int try_flip_pair_z2(fgo_scheme *s, int c, int t1, int t2, uint32_t coin, int check_reduce) {
+    if (c < 0 || c > 2 || t1 < 0 || t2 <= t1 || t2 >= s->m || !fac_eq(s, c, t1, t2)) return 0;
+    tl_sel[0] = c; tl_sel[1] = t1; tl_sel[2] = t2;
+    int i, j, k;
+    pick_orientation(c, (int)coin, &i, &j, &k);
+    if (coin & 2) { int t = t1; t1 = t2; t2 = t; }
+    flip_z2(s, i, j, k, t1, t2, check_reduce);
     return 1;
 }
 
@@ -780,6 +796,8 @@ void fgo_walker_init(fgo_walker *w, const fgo_scheme *start, int best_rank, uint
     w->best_rank = best_rank;
     w->flips_count = 0;
     fgo_rng_init(&w->rng, seed, walker);
+    w->forced[0] = -1;
+    w->log[0] = -1;
 }
 
 /* minimizeKernel (src/entities/complexity_minimizer.cu:190-214): flips without
@@ -811,13 +829,26 @@ void fgo_walk(fgo_walker *w, const fgo_walk_params *p) {
     fgo_scheme *s = &w->cur;
     int iters = p->iterations;
     if (p->random_iterations) iters = aux_randint(&w->rng, 1, p->max_iterations); /* :530 */
+    w->min_m = s->m;
+    w->log[0] = -1;
     for (int it = 0; it < iters; it++) {
         uint32_t b[4];
         fgo_rng_main_block(&w->rng, b);
         int rank = s->m;
-        int ok = s->ring == FGO_RING_Z2 ? try_flip_z2(s, b, p) : try_flip_zt(s, b, &w->rng, p);
+        int ok = 0;
+        if (it == 0 && w->forced[0] >= 0 && s->ring == FGO_RING_Z2)
+            ok = try_flip_pair_z2(s, w->forced[0], w->forced[1], w->forced[2], b[1], p->check_reduce);
+        if (!ok) ok = s->ring == FGO_RING_Z2 ? try_flip_z2(s, b, p) : try_flip_zt(s, b, &w->rng, p);
+        if (it == 0) {
+            w->forced[0] = -1;
+            w->log[0] = ok && s->ring == FGO_RING_Z2 ? tl_sel[0] : -1;
+            w->log[1] = tl_sel[1];
+            w->log[2] = tl_sel[2];
+            w->log[3] = rank;
+        }
         if (!ok) { /* :535-537 */
             try_expand(s, 1 + (int)(b[1] % 2u), &w->rng);
+            if (s->m < w->min_m) w->min_m = s->m;
             continue;
         }
         w->total_flips++;
@@ -833,6 +864,7 @@ void fgo_walk(fgo_walker *w, const fgo_walk_params *p) {
             if (try_expand(s, aux_randint(&w->rng, 1, 2), &w->rng)) w->flips_count = 0;
         }
         if (w->flips_count >= p->plus_iterations && try_plus(s, &w->rng)) w->flips_count = 0; /* :564-565 */
+        if (s->m < w->min_m) w->min_m = s->m;
     }
 }
 
@@ -880,6 +912,8 @@ static void scheme_from_words(fgo_scheme *s, int ring, const int32_t n[3], int m
 int fgo_walker_from_record(fgo_walker *w, int ring, const fgo_walker_record *rec, uint64_t seed, uint32_t walker) {
     if (rec->m < 0 || rec->m > FGO_REC_MAX_M || rec->best_m < 0 || rec->best_m > FGO_REC_MAX_M) return -1;
     memset(w, 0, sizeof(*w));
+    w->forced[0] = -1;
+    w->log[0] = -1;
     scheme_from_words(&w->cur, ring, rec->n, rec->m, rec->cur_v, rec->cur_s);
     fgo_init_flips(&w->cur);
     scheme_from_words(&w->best, ring, rec->n, rec->best_m, rec->best_v, rec->best_s);
@@ -946,6 +980,22 @@ int fgo_walk_records(int ring, fgo_walker_record *recs, int count, uint32_t walk
         if (jobs[t].rc) rc = jobs[t].rc;
     }
     return rc;
+}
+
+int fgo_walk_records_forced(fgo_walker_record *recs, int count, uint32_t walker_base, uint64_t seed,
+                            const fgo_walk_params *p, const int32_t *forced, int32_t *log, int32_t *minm) {
+    fgo_walker *w = (fgo_walker *)malloc(sizeof(fgo_walker));
+    if (!w) return -2;
+    for (int q = 0; q < count; q++) {
+        if (fgo_walker_from_record(w, FGO_RING_Z2, &recs[q], seed, walker_base + (uint32_t)q)) { free(w); return -1; }
+        if (forced) { w->forced[0] = forced[4 * q]; w->forced[1] = forced[4 * q + 1]; w->forced[2] = forced[4 * q + 2]; }
+        fgo_walk(w, p);
+        fgo_walker_to_record(w, &recs[q]);
+        if (log) for (int k = 0; k < 4; k++) log[4 * q + k] = w->log[k];
+        if (minm) minm[q] = w->min_m;
+    }
+    free(w);
+    return 0;
 }
 
 int fgo_naive_records(int ring, int n1, int n2, int n3, fgo_walker_record *recs, int count) {

@@ -95,6 +95,9 @@ typedef struct {
     fgo_rng rng;
     uint64_t total_flips;
     uint64_t sum_m;
+    int forced[3];  /* proposed first flip of the next launch (component -1: none) */
+    int log[4];     /* first flip of the last launch: component (-1: none), index1, index2, rank */
+    int min_m;      /* lowest rank during the last launch */
 } fgo_walker;
 
 /* Walker record exchanged with tests; same layout as fg_walker_record in
@@ -158,6 +161,12 @@ int fgo_walker_from_record(fgo_walker *w, int ring, const fgo_walker_record *rec
 int fgo_walk_records(int ring, fgo_walker_record *recs, int count, uint32_t walker_base,
                      uint64_t seed, const fgo_walk_params *p, int launches, int nthreads);
 int fgo_naive_records(int ring, int n1, int n2, int n3, fgo_walker_record *recs, int count);
+/* One launch of the Z2 walk with a proposed first flip per walker
+ * (forced[count][4]: component (-1: none), index1, index2, unused); writes
+ * the first flip taken (log[count][4]) and the launch's lowest rank
+ * (minm[count]); either output may be null. */
+int fgo_walk_records_forced(fgo_walker_record *recs, int count, uint32_t walker_base, uint64_t seed,
+                            const fgo_walk_params *p, const int32_t *forced, int32_t *log, int32_t *minm);
 int fgo_record_validate(int ring, const fgo_walker_record *rec, int best);
 
 /* Blob-level helpers for ctypes. */

@@ -18,7 +18,8 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                                    int max_iterations, int random_iterations, int plus_iterations,
                                    int use_reduce, uint32_t thr_reduce, int use_expand, uint32_t thr_expand,
                                    uint32_t thr_best_eq, int check_reduce, int launches, int visit_cap,
-                                   int minimize, int32_t *err_out) {
+                                   int mode, const int32_t *forced, int32_t *log_meta, int32_t *minm,
+                                   int32_t *err_out) {
     int maxm = 1;
     for (int q = 0; q < count; ++q) {
         maxm = std::max(maxm, std::max(recs[q].m, recs[q].best_m));
@@ -51,22 +52,37 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
     a.use_expand = use_expand; a.thr_expand = thr_expand; a.thr_best_eq = thr_best_eq;
     a.check_reduce = check_reduce; a.zt_visit_cap = visit_cap; a.err = &err;
     a.cx = cx.data();
+    std::vector<uint32_t> log_words((size_t)W * 3 * maxm);
+    std::vector<int32_t> lm((size_t)W * 4, -1), mm(W, 0);
+    if (mode == fgk::lane::kModeLearn) {
+        a.forced = forced;
+        a.log_every = 1;
+        a.log_words = log_words.data();
+        a.log_meta = lm.data();
+        a.minm = mm.data();
+    }
     std::vector<uint8_t> lds(ring ? fgk::zt_lds_bytes(maxm) : fgk::z2_lds_bytes(maxm), 0xCD);
     const int G = (W + fgk::kWalkersPerGroup - 1) / fgk::kWalkersPerGroup;
-    for (int l = 0; l < launches; ++l)
+    for (int l = 0; l < launches; ++l) {
+        if (l > 0) a.forced = nullptr;  // proposals apply to the first launch
         for (int g = 0; g < G; ++g)
             for (int slot = 0; slot < fgk::kWalkersPerGroup; ++slot) {
                 // one call per walker: the host side of the lane code does both flip halves
                 const int lane = slot * fgk::kLanesPerWalker;
                 const int w = g * fgk::kWalkersPerGroup + slot;
-                if (minimize) {
-                    if (ring) fgk::lane::walk_lane<fgk::lane::ZTLane, true>(a, lds.data(), lane, w, W);
-                    else fgk::lane::walk_lane<fgk::lane::Z2Lane, true>(a, lds.data(), lane, w, W);
+                if (mode == fgk::lane::kModeMinimize) {
+                    if (ring) fgk::lane::walk_lane<fgk::lane::ZTLane, fgk::lane::kModeMinimize>(a, lds.data(), lane, w, W);
+                    else fgk::lane::walk_lane<fgk::lane::Z2Lane, fgk::lane::kModeMinimize>(a, lds.data(), lane, w, W);
+                } else if (mode == fgk::lane::kModeLearn && !ring) {
+                    fgk::lane::walk_lane<fgk::lane::Z2Lane, fgk::lane::kModeLearn>(a, lds.data(), lane, w, W);
                 } else {
-                    if (ring) fgk::lane::walk_lane<fgk::lane::ZTLane, false>(a, lds.data(), lane, w, W);
-                    else fgk::lane::walk_lane<fgk::lane::Z2Lane, false>(a, lds.data(), lane, w, W);
+                    if (ring) fgk::lane::walk_lane<fgk::lane::ZTLane, fgk::lane::kModeWalk>(a, lds.data(), lane, w, W);
+                    else fgk::lane::walk_lane<fgk::lane::Z2Lane, fgk::lane::kModeWalk>(a, lds.data(), lane, w, W);
                 }
             }
+    }
+    if (log_meta) std::copy(lm.begin(), lm.end(), log_meta);
+    if (minm) std::copy(mm.begin(), mm.end(), minm);
     for (int w = 0; w < W; ++w) {
         fg_walker_record &r = recs[w];
         std::memset(r.cur_v, 0, sizeof r.cur_v);

@@ -20,18 +20,27 @@ def lib():
         L.emu_walk_records.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64] + \
             [ctypes.c_int] * 4 + [ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                  ctypes.POINTER(ctypes.c_int32)]
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
 
 
-def walk(recs, params, seed, launches=1, ring=O.RING_Z2):
+def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=False):
+    """Emulated launches; with learn (Z2) the first launch takes the proposed
+    pairs `forced` [walkers][4] and the result adds (log_meta, minm)."""
     recs = np.ascontiguousarray(recs.copy())
     err = ctypes.c_int32()
     p = params
+    mode = 2 if learn else p.mode
+    fz = None if forced is None else np.ascontiguousarray(forced, dtype=np.int32)
+    lm = np.zeros((len(recs), 4), dtype=np.int32)
+    mm = np.zeros(len(recs), dtype=np.int32)
     rc = lib().emu_walk_records(ring, recs.ctypes.data, len(recs), seed, p.iterations, p.max_iterations,
                                 p.random_iterations, p.plus_iterations, p.use_reduce, p.thr_reduce,
                                 p.use_expand, p.thr_expand, p.thr_best_eq, p.check_reduce, launches,
-                                p.zt_visit_cap, p.mode, ctypes.byref(err))
+                                p.zt_visit_cap, mode, None if fz is None else fz.ctypes.data,
+                                lm.ctypes.data, mm.ctypes.data, ctypes.byref(err))
     assert rc == 0
+    if learn:
+        return recs, err.value, lm, mm
     return recs, err.value

@@ -170,6 +170,22 @@ def walk_records(recs, params, seed, ring=RING_Z2, walker_base=0, launches=1, nt
     return recs
 
 
+def walk_records_forced(recs, params, seed, forced=None, walker_base=0):
+    """One Z2 launch with proposed first flips; returns (records, log [n][4], minm [n])."""
+    recs = np.array(recs, dtype=RECORD_DTYPE, copy=True, order="C")
+    L = lib()
+    L.fgo_walk_records_forced.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64,
+                                          ctypes.POINTER(WalkParams), ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p]
+    fz = None if forced is None else np.ascontiguousarray(forced, dtype=np.int32)
+    log = np.zeros((len(recs), 4), dtype=np.int32)
+    mm = np.zeros(len(recs), dtype=np.int32)
+    rc = L.fgo_walk_records_forced(recs.ctypes.data, len(recs), walker_base, seed, ctypes.byref(params),
+                                   None if fz is None else fz.ctypes.data, log.ctypes.data, mm.ctypes.data)
+    assert rc == 0
+    return recs, log, mm
+
+
 def record_valid(rec_array, index, best=False, ring=RING_Z2):
     return lib().fgo_record_validate(ring, rec_array[index:index + 1].ctypes.data, int(best))
 

@@ -1,0 +1,121 @@
+"""Learned proposals on the GPU: the BF16/MFMA scorer's forward and backward
+against the float64 oracle (same bfloat16 rounding points), the forced-flip
+walk and the trajectory log against the oracle's walk, training on the log,
+and the proposals the trained scorer makes."""
+import numpy as np
+import pytest
+
+import flipgraphgpu_amd as fg
+import oracle_ffi as O
+from learn_util import SR, random_batch, random_forced, record_pairs
+from walk_compare import first_mismatch
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12))
+
+
+@pytest.mark.parametrize("count,M,n", [(70, 64, (4, 4, 4)), (37, 27, (3, 3, 3))])
+def test_scorer_forward_backward_match_oracle(count, M, n):
+    print(f"[scorer] eval count={count} M={M} n={n}", flush=True)
+    nn = (n[0] * n[1], n[1] * n[2], n[2] * n[0])
+    rng = np.random.default_rng(count)
+    with fg.Scorer(*n, M, seed=3) as sc:
+        layout = sc.layout()
+        ref_layout, npar = SR.make_layout(sum(nn), M)
+        assert list(layout) == list(ref_layout) and sc.nparams == npar
+        p = SR.init_params(sum(nn), 11).astype(np.float32)
+        sc.set_params(p)
+        assert np.array_equal(sc.get_params(), p)
+        words, m, pairs, y = random_batch(rng, count, M, nn)
+        s_gpu, loss_gpu, g_gpu = sc.eval(words, m, pairs, y)
+        loss_ref, g_ref, s_ref = SR.loss_and_grads(p.astype(np.float64), layout, words, m, pairs, y, nn)
+        assert _rel(s_gpu, s_ref) < 1e-2
+        assert abs(loss_gpu - loss_ref) <= 1e-3 * max(1.0, abs(loss_ref))
+        off = list(layout[4:]) + [npar]
+        for k in range(12):
+            a, b = off[k], off[k + 1]
+            assert _rel(g_gpu[a:b], g_ref[a:b]) < 3e-2, (k, _rel(g_gpu[a:b], g_ref[a:b]))
+        s2, _, _ = sc.eval(words, m, pairs)
+        assert np.array_equal(s2, s_gpu)
+
+
+def test_forced_walk_matches_oracle():
+    print("[learned] forced walk 4x4x4", flush=True)
+    rng = np.random.default_rng(7)
+    W = 256
+    with fg.Engine(W, 21, max_iterations=300, plus_iterations=300) as e:
+        e.init_naive(4, 4, 4)
+        assert e.walk(300) == 0
+        recs = e.read_walkers()
+        forced = random_forced(recs, rng)
+        e.set_forced(forced)
+        assert np.array_equal(e.get_forced(), forced)
+        assert e.walk_forced(300) == 0
+        gpu = e.read_walkers()
+    ora, log, mm = O.walk_records_forced(recs, O.make_params(iterations=300, plus_iterations=300), 21, forced)
+    assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
+
+
+def test_trajectory_log_matches_oracle():
+    print("[learned] trajectory log", flush=True)
+    W, L, H, n = 128, 150, 3, (3, 3, 3)
+    p = O.make_params(iterations=L, plus_iterations=400)
+    with fg.Engine(W, 5, max_iterations=L, plus_iterations=400) as e:
+        e.init_naive(*n)
+        e.log_enable(max_batches=8, every=1, horizon=H)
+        for _ in range(6):
+            assert e.walk(L) == 0
+        info = e.log_info()
+        assert info["batches"] == 6 - H + 1
+        words, m, pairs, y = e.log_read(0, info["rows"], 27)
+    # the same launches in the oracle
+    recs = O.naive_records(*n, W)
+    states, logs, mins = [], [], []
+    for _ in range(6):
+        states.append(recs.copy())
+        recs, log, mm = O.walk_records_forced(recs, p, 5, None)
+        logs.append(log)
+        mins.append(mm)
+    rows = []
+    for s in range(6 - H + 1):
+        lo = np.min(np.stack(mins[s:s + H]), axis=0)
+        for w in range(W):
+            if logs[s][w, 0] >= 0:
+                rows.append((s, w, tuple(logs[s][w, :3]), int(logs[s][w, 3]), float(lo[w] < logs[s][w, 3])))
+    assert len(rows) == info["rows"] and info["positives"] == sum(r[4] for r in rows)
+    for q, (s, w, pr, rank, lab) in enumerate(rows):
+        assert tuple(pairs[q]) == pr and m[q] == rank and y[q] == lab
+        st = states[s][w]
+        assert np.array_equal(words[q][:, :rank], st["cur_v"][:, :rank].astype(np.uint32))
+        assert pr in record_pairs(st)
+
+
+def test_training_and_proposals():
+    print("[learned] train + propose 4x4x4", flush=True)
+    W = 4096
+    with fg.Engine(W, 3, max_iterations=200, plus_iterations=2000) as e, fg.Scorer(4, 4, 4, 64, seed=2) as sc:
+        e.init_naive(4, 4, 4)
+        e.log_enable(max_batches=16, every=1, horizon=4)
+        for _ in range(12):
+            assert e.walk(200) == 0
+        info = e.log_info()
+        assert info["rows"] > 20000 and 0 < info["positives"] < info["rows"]
+        losses = sc.train(e, steps=150, batch=512, lr=2e-3, seed=5)
+        assert np.isfinite(losses).all()
+        assert losses[-20:].mean() < losses[:10].mean()
+        sc.propose(e, candidates=8, every=2, round_=1)
+        forced = e.get_forced()
+        recs = e.read_walkers()
+        proposed = 0
+        for w in range(0, W, 97):
+            if w % 2:
+                assert forced[w, 0] == -1
+            elif forced[w, 0] >= 0:
+                proposed += 1
+                assert tuple(forced[w, :3]) in record_pairs(recs[w])
+        assert proposed > 0
+        bad = e.walk_learned(sc, 200, candidates=8, every=2, round_=2)
+        assert bad == 0
