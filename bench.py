@@ -40,6 +40,21 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
+    tt = ap.add_argument_group("time to target (learned vs uniform proposals, 4x4x4 Z2)")
+    tt.add_argument("--ttt", action="store_true", help="run the time-to-target study instead of the throughput bench")
+    tt.add_argument("--seeds", type=int, default=10)
+    tt.add_argument("--ttt-walkers", type=int, default=131072)
+    tt.add_argument("--ttt-launch", type=int, default=1000, help="walk iterations per launch (one proposal round each)")
+    tt.add_argument("--ttt-target", type=int, default=49)
+    tt.add_argument("--ttt-timeout", type=float, default=60.0, help="seconds per run before giving up")
+    tt.add_argument("--ttt-plus", type=int, default=20000)
+    tt.add_argument("--learned-every", type=int, default=2, help="every k-th walker takes the learned proposal")
+    tt.add_argument("--candidates", type=int, default=8, help="uniform candidates scored per proposal")
+    tt.add_argument("--log-launches", type=int, default=86, help="uniform launches logged for training")
+    tt.add_argument("--log-horizon", type=int, default=10, help="launches until a logged flip is labelled")
+    tt.add_argument("--train-steps", type=int, default=1500)
+    tt.add_argument("--train-batch", type=int, default=1024)
+    tt.add_argument("--lr", type=float, default=1e-3)
     return ap.parse_args()
 
 
@@ -94,10 +109,106 @@ def profiled_traffic(path):
     return None
 
 
+def time_to_target(a):
+    """Median wall time over seeds to a verified n=4 Z2 scheme of rank <= target,
+    uniform walks vs learned + uniform proposals.  The scorer is trained once,
+    on trajectories logged by uniform walks of the same shape (seed 1000);
+    its training time is reported separately.  A run ends at the first launch
+    after which the best scheme has rank <= target and passes the GPU Brent
+    check and the host re-check."""
+    import numpy as np
+
+    import flipgraphgpu_amd as fg
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+
+    W, L = a.ttt_walkers, a.ttt_launch
+
+    def progress(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    def run(seed, scorer):
+        e = fg.Engine(W, seed, max_iterations=L, plus_iterations=a.ttt_plus, expand_probability=0.0)
+        e.init_naive(4, 4, 4)
+        t0 = time.perf_counter()
+        launches, best = 0, 64
+        while time.perf_counter() - t0 < a.ttt_timeout:
+            if scorer is None:
+                bad = e.walk(L)
+            else:
+                bad = e.walk_learned(scorer, L, candidates=a.candidates, every=a.learned_every, round_=launches)
+            if bad:
+                raise RuntimeError(f"{bad} invalid schemes")
+            launches += 1
+            blob, best, _ = e.best()
+            if best <= a.ttt_target:
+                ok = fg.verify([blob])[0] and O.validate(blob) == 1
+                if ok:
+                    break
+        dt = time.perf_counter() - t0
+        e.close()
+        return dt, best, launches
+
+    # training data: uniform walks with the trajectory log
+    t0 = time.perf_counter()
+    e = fg.Engine(W, 1000, max_iterations=L, plus_iterations=a.ttt_plus, expand_probability=0.0)
+    e.init_naive(4, 4, 4)
+    e.log_enable(max_batches=a.log_launches, every=1, horizon=a.log_horizon)
+    for q in range(a.log_launches + a.log_horizon - 1):
+        if e.walk(L):
+            raise RuntimeError("invalid schemes while logging")
+    info = e.log_info()
+    t_log = time.perf_counter() - t0
+    progress(f"[ttt] logged {info['rows']} flips ({info['positives']} positive) in {t_log:.1f} s")
+    sc = fg.Scorer(4, 4, 4, 64, seed=7)
+    t0 = time.perf_counter()
+    losses = sc.train(e, steps=a.train_steps, batch=a.train_batch, lr=a.lr, seed=11)
+    t_train = time.perf_counter() - t0
+    e.close()
+    progress(f"[ttt] trained {a.train_steps} steps in {t_train:.1f} s, loss {losses[:20].mean():.4f} -> {losses[-50:].mean():.4f}")
+    res = {"uniform": [], "learned": []}
+    for seed in range(1, a.seeds + 1):
+        for arm, scorer in (("uniform", None), ("learned", sc)):
+            dt, best, launches = run(seed, scorer)
+            res[arm].append({"seed": seed, "seconds": round(dt, 3), "best_rank": int(best), "launches": launches,
+                             "reached": bool(best <= a.ttt_target)})
+            progress(f"[ttt] seed {seed} {arm}: rank {best} after {launches} launches, {dt:.2f} s")
+    sc.close()
+
+    def med(arm):
+        v = [r["seconds"] if r["reached"] else float("inf") for r in res[arm]]
+        return float(np.median(v))
+
+    mu, ml = med("uniform"), med("learned")
+    out = {
+        "metric": f"time to verified 4x4x4 Z2 rank-{a.ttt_target} (median over {a.seeds} seeds)",
+        "value": ml, "unit": "s", "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
+        "uniform_median_s": mu, "learned_median_s": ml,
+        "speedup_learned_vs_uniform": (mu / ml) if ml > 0 else None,
+        "config": {"workload": f"4x4x4 Z2 from the naive rank-64 scheme, {W} walkers, {L} iterations per launch, "
+                               f"plus-iterations {a.ttt_plus}; learned arm: every {a.learned_every}th walker's first "
+                               f"flip of each launch is the scorer's best of {a.candidates} uniform candidates",
+                   "walkers": W, "launch_iterations": L},
+        "training": {"logged_flips": info["rows"], "positives": info["positives"],
+                     "log_seconds": round(t_log, 2), "train_seconds": round(t_train, 2),
+                     "steps": a.train_steps, "batch": a.train_batch,
+                     "loss_first": float(losses[:20].mean()), "loss_last": float(losses[-50:].mean())},
+        "runs": res,
+    }
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
+        return
+    if a.ttt:
+        if a.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+            sys.exit(2)
+        time_to_target(a)
         return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
