@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--n", type=int, nargs=3, default=[3, 3, 3])
     ap.add_argument("--plus-iterations", type=int, default=20000)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--word-bits", type=int, default=0, choices=[0, 16, 32],
+                    help="Z2 LDS factor words (0: the runtime's choice)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -218,7 +220,7 @@ def main():
     info = fg.device_info()
     n = tuple(a.n)
     e = fg.Engine(a.walkers, a.seed, max_iterations=a.iterations, plus_iterations=a.plus_iterations,
-                  expand_probability=0.0)
+                  expand_probability=0.0, word_bits=a.word_bits)
     e.init_naive(*n)
     for _ in range(a.warmup):
         bad = e.walk(a.iterations)

@@ -49,6 +49,7 @@ class FgConfig(ctypes.Structure):
         ("zt_visit_cap", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
         ("grid", ctypes.c_int32),
+        ("word_bits", ctypes.c_int32),
     ]
 
 
@@ -151,12 +152,12 @@ class Engine:
 
     def __init__(self, walkers, seed, ring=RING_Z2, max_iterations=10000, plus_iterations=8000,
                  random_iterations=False, check_reduce=True, expand_probability=0.0,
-                 reduce_probability=0.0, max_rank=0, zt_visit_cap=16, use_graph=False, grid=0):
+                 reduce_probability=0.0, max_rank=0, zt_visit_cap=16, use_graph=False, grid=0, word_bits=0):
         self.L = load_library()
         self.ring = ring
         self.cfg = FgConfig(ring, walkers, max_iterations, plus_iterations, int(random_iterations),
                             int(check_reduce), expand_probability, reduce_probability, seed, max_rank,
-                            zt_visit_cap, int(use_graph), grid)
+                            zt_visit_cap, int(use_graph), grid, word_bits)
         self.ctx = ctypes.c_void_p()
         _check(self.L.fg_create(ctypes.byref(self.cfg), ctypes.byref(self.ctx)), "fg_create")
         self.walkers = walkers

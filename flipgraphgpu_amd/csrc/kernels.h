@@ -7,13 +7,16 @@
 
 namespace fgk {
 
-// ring 0 = Z2, 1 = integer ring with packed ternary words
-size_t walk_lds_bytes(int ring, int max_m);
+// Walk kernel variants: Z2 with 32-bit factor words, the integer ring
+// (packed ternary words), Z2 with 16-bit factor words in LDS.
+constexpr int kVarZ2 = 0, kVarZT = 1, kVarZ2h = 2;
+
+size_t walk_lds_bytes(int variant, int max_m);
 // Sets the kernel's LDS limit and reports how many workgroups fit per CU.
 // mode: 0 walk, 1 complexity minimizer, 2 walk with proposals and logging (Z2 only).
-hipError_t prepare_walk(int ring, int max_m, int mode, int *groups_per_cu);
+hipError_t prepare_walk(int variant, int max_m, int mode, int *groups_per_cu);
 // Resets a.queue and launches `grid` persistent workgroups.
-hipError_t launch_walk(int ring, int mode, const WalkArgs &a, int grid, hipStream_t stream);
+hipError_t launch_walk(int variant, int mode, const WalkArgs &a, int grid, hipStream_t stream);
 // Current scheme of walkers dst[0..count) := best scheme of walker src.
 hipError_t launch_broadcast_best(uint32_t *cur, int32_t *m, const uint32_t *best, const int32_t *best_m, int walkers,
                                  int max_m, int src, const int32_t *dst, int count, hipStream_t stream);

@@ -93,13 +93,18 @@ FGK_HD void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_
     o0 = c0; o1 = c1; o2 = c2; o3 = c3;
 }
 
-struct LaneBase {
-    uint32_t *F;  // walker-slot-offset plane bases
+// Word: the LDS type of one factor (uint16_t for Z2 factors of up to 16
+// coefficients, uint32_t otherwise).
+template <typename WordT>
+struct LaneBaseT {
+    using Word = WordT;
+    Word *F;      // walker-slot-offset plane bases
     uint8_t *H;   // slot-offset byte base of the packed H plane
     uint8_t *N;
     int side;     // 0: index1 / component-j half of a flip, 1: index2 / component-k half
     int32_t *err;
-    int maxm;
+    int maxm;     // term capacity
+    int fs;       // LDS term stride of F (lds_term_stride)
     int hw;       // H words per component
     int n0, n1, n2;
     int nn0, nn1, nn2;
@@ -117,7 +122,7 @@ struct LaneBase {
     // Extra per-lane LDS beyond the common planes (none by default).
     FGK_HD void lane_extra(uint8_t *, int) {}
 
-    FGK_HD uint32_t &f(int c, int t) { return F[(c * maxm + t) * WS]; }
+    FGK_HD Word &f(int c, int t) { return F[(c * fs + t) * WS]; }
     // H packs terms 4q..4q+3 of a walker into one dword: byte (t & 3) of word q.
     FGK_HD uint8_t &h(int c, int t) { return H[((c * hw + (t >> 2)) * WS) * 4 + (t & 3)]; }
     FGK_HD uint32_t &hword(int c, int q) { return reinterpret_cast<uint32_t *>(H)[(c * hw + q) * WS]; }
@@ -411,10 +416,10 @@ FGK_HD void copy_best(Lane &s, const WalkArgs &A, bool want, int lane, int w) {
         mask &= mask - 1;
         const int ms = __builtin_amdgcn_readlane(s.m, src);
         const int slot = lane / kLanesPerWalker, src_slot = src / kLanesPerWalker;
-        const uint32_t *Fs = s.F - slot + src_slot;
+        const typename Lane::Word *Fs = s.F - slot + src_slot;
         uint32_t *dst = A.best + (size_t)(w - slot + src_slot) * 3 * A.max_m;
         for (int c = 0; c < 3; ++c)
-            for (int t = lane; t < ms; t += kThreadsPerGroup) dst[c * A.max_m + t] = Fs[(c * A.max_m + t) * WS];
+            for (int t = lane; t < ms; t += kThreadsPerGroup) dst[c * A.max_m + t] = Fs[(c * s.fs + t) * WS];
     }
 #else
     if (want) {
@@ -457,14 +462,18 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     const int W = A.walkers;
     const int maxm = A.max_m;
 
+    using Word = typename Lane::Word;
     Lane s;
     const int hw = h_words(maxm);
+    const int fs = lds_term_stride(maxm, (int)sizeof(Word));
+    const size_t fbytes = walk_lds_f_bytes(maxm, (int)sizeof(Word));
     const int slot = lane / kLanesPerWalker;
     s.side = lane % kLanesPerWalker;
-    s.F = reinterpret_cast<uint32_t *>(lds) + slot;
-    s.H = lds + (size_t)3 * maxm * WS * sizeof(uint32_t) + (size_t)slot * 4;
-    s.N = lds + (size_t)3 * maxm * WS * sizeof(uint32_t) + (size_t)3 * hw * WS * 4 + (size_t)slot * 4;
-    s.lane_extra(lds + walk_lds_base_bytes(maxm), slot);
+    s.F = reinterpret_cast<Word *>(lds) + slot;
+    s.H = lds + fbytes + (size_t)slot * 4;
+    s.N = lds + fbytes + (size_t)3 * hw * WS * 4 + (size_t)slot * 4;
+    s.lane_extra(lds + walk_lds_base_bytes(maxm, (int)sizeof(Word)), slot);
+    s.fs = fs;
     s.err = A.err;
     s.maxm = maxm;
     s.hw = hw;

@@ -67,6 +67,7 @@ struct fg_ctx {
     bool ready = false;
     float last_ms = 0.f;
     size_t lds_limit = 0;
+    size_t lds_per_cu = 0;
 
     uint32_t *cur = nullptr, *cur_s = nullptr, *best = nullptr, *best_s = nullptr;
     int32_t *m = nullptr, *best_m = nullptr, *best_rank = nullptr, *flips_count = nullptr, *aux_idx = nullptr;
@@ -77,6 +78,7 @@ struct fg_ctx {
     int32_t *forced = nullptr;  // [W][4] proposed first flips (learned walks)
     int nitems = 0;
     bool minimize = false;  // pool set up by fg_init_minimizer
+    int variant = 0;        // walk kernel variant (kernels.h: kVarZ2 / kVarZT / kVarZ2h)
 
     // trajectory log: ring of the last `horizon` launches, labelled rows
     struct Log {
@@ -163,6 +165,7 @@ extern "C" int fg_create(const fg_config *cfg, fg_ctx **out) {
     if (cfg->walkers <= 0) return fail("fg_create: walkers must be positive");
     if (cfg->max_iterations <= 0) return fail("fg_create: max_iterations must be positive");
     if (cfg->plus_iterations < 0) return fail("fg_create: plus_iterations must be >= 0");
+    if (cfg->word_bits != 0 && cfg->word_bits != 16 && cfg->word_bits != 32) return fail("fg_create: word_bits must be 0, 16 or 32");
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
@@ -173,6 +176,7 @@ extern "C" int fg_create(const fg_config *cfg, fg_ctx **out) {
     c->cfg = *cfg;
     if (c->cfg.zt_visit_cap <= 0) c->cfg.zt_visit_cap = 64;
     c->lds_limit = prop.sharedMemPerBlock;
+    c->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : prop.sharedMemPerBlock;
     c->cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
@@ -214,11 +218,31 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
                             " coefficients are not supported on the GPU path for this ring");
     }
     if (maxm > 255) return fail("init: more than 255 terms per scheme are not supported on the GPU path");
-    const size_t lds = ring ? fgk::zt_lds_bytes(maxm) : fgk::z2_lds_bytes(maxm);
+    // Z2 factors of up to 16 coefficients can live in 16-bit LDS words.  They
+    // cost a few percent per iteration, so they are used only when they cut
+    // the number of rounds in which the chip works through all walker blocks
+    // (4x4x4 at 131072 walkers: 4 rounds -> 3, +11% flips/s; at 65536: 2 and
+    // 2, so 32-bit words stay).
+    int nn_max = 0;
+    for (const auto &s : starts) nn_max = std::max(nn_max, std::max(s.nn[0], std::max(s.nn[1], s.nn[2])));
+    int variant = ring ? fgk::kVarZT : fgk::kVarZ2;
+    if (!ring && ctx->cfg.word_bits == 16) {
+        if (nn_max > 16) return fail("init: 16-bit factor words hold factors of up to 16 coefficients");
+        variant = fgk::kVarZ2h;
+    } else if (!ring && nn_max <= 16 && ctx->cfg.word_bits != 32) {
+        const long blocks = (W + fgk::kWalkersPerGroup - 1) / fgk::kWalkersPerGroup;
+        const size_t l32 = fgk::walk_lds_bytes(fgk::kVarZ2, maxm), l16 = fgk::walk_lds_bytes(fgk::kVarZ2h, maxm);
+        const long fit32 = (long)(ctx->lds_per_cu / l32) * ctx->cus, fit16 = (long)(ctx->lds_per_cu / l16) * ctx->cus;
+        const long rounds32 = fit32 > 0 ? (blocks + fit32 - 1) / fit32 : LONG_MAX;
+        const long rounds16 = fit16 > 0 ? (blocks + fit16 - 1) / fit16 : LONG_MAX;
+        if (rounds16 < rounds32) variant = fgk::kVarZ2h;
+    }
+    const size_t lds = fgk::walk_lds_bytes(variant, maxm);
     if (lds > ctx->lds_limit)
         return fail("init: " + std::to_string(maxm) + " terms per walker exceed the LDS of one workgroup");
 
     ctx->release();
+    ctx->variant = variant;
     ctx->W = W;
     ctx->maxm = maxm;
     const size_t plane = (size_t)3 * maxm * W;
@@ -300,10 +324,10 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
     ctx->nitems = (int)ib.size();
     HIPCHK(hipMemcpy(ctx->item_begin, ib.data(), ib.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->item_end, ie.data(), ie.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(fgk::prepare_walk(ring, maxm, minimize ? 1 : 0, &ctx->groups_per_cu));
+    HIPCHK(fgk::prepare_walk(variant, maxm, minimize ? 1 : 0, &ctx->groups_per_cu));
     if (!minimize && ring == FG_RING_Z2) {
         int g2 = 0;
-        HIPCHK(fgk::prepare_walk(ring, maxm, 2, &g2));
+        HIPCHK(fgk::prepare_walk(variant, maxm, 2, &g2));
     }
     if (ctx->groups_per_cu < 1) return fail("init: the walk kernel does not fit on a CU with this scheme size");
     ctx->grid = ctx->cfg.grid > 0 ? ctx->cfg.grid : ctx->groups_per_cu * ctx->cus;
@@ -565,7 +589,7 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced) {
     auto enqueue = [&](bool timed) -> int {
         HIPCHK(fgk::launch_fill(ctx->err, 0, 1, ctx->stream));
         if (timed) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-        HIPCHK(fgk::launch_walk(ctx->cfg.ring, mode, a, ctx->grid, ctx->stream));
+        HIPCHK(fgk::launch_walk(ctx->variant, mode, a, ctx->grid, ctx->stream));
         if (timed) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         return verify_planes(ctx, ctx->cfg.ring, ctx->cur, ctx->m, ctx->fmt, ctx->W, ctx->maxm, ctx->ok);
     };

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "kernels.h"
 #include "walk_args.h"
 #include "walk_z2_lane.h"
 #include "walk_zt_lane.h"
@@ -38,6 +39,12 @@ __global__ void __launch_bounds__(kThreadsPerGroup) walk_zt_kernel(WalkArgs A) {
     walk_persistent<lane::ZTLane, lane::kModeWalk>(A, lds);
 }
 
+// Z2 with 16-bit factor words in LDS (factors of up to 16 coefficients).
+__global__ void __launch_bounds__(kThreadsPerGroup) walk_z2h_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2Lane16, lane::kModeWalk>(A, lds);
+}
+
 // Complexity minimizer (minimizeKernel, src/entities/complexity_minimizer.cu:190-214).
 __global__ void __launch_bounds__(kThreadsPerGroup) minimize_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
@@ -49,10 +56,20 @@ __global__ void __launch_bounds__(kThreadsPerGroup) minimize_zt_kernel(WalkArgs 
     walk_persistent<lane::ZTLane, lane::kModeMinimize>(A, lds);
 }
 
+__global__ void __launch_bounds__(kThreadsPerGroup) minimize_z2h_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2Lane16, lane::kModeMinimize>(A, lds);
+}
+
 // Walk with a proposed first flip and trajectory logging (Z2).
 __global__ void __launch_bounds__(kThreadsPerGroup) learn_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::Z2Lane, lane::kModeLearn>(A, lds);
+}
+
+__global__ void __launch_bounds__(kThreadsPerGroup) learn_z2h_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2Lane16, lane::kModeLearn>(A, lds);
 }
 
 // copyTo of the minimizer's re-broadcast (complexity_minimizer.cu:130-133):
@@ -175,38 +192,39 @@ __global__ void verify_zt_kernel(VerifyArgs V) {
     if (!ok) V.ok[s] = 0;
 }
 
-size_t walk_lds_bytes(int ring, int max_m) { return ring ? zt_lds_bytes(max_m) : z2_lds_bytes(max_m); }
-
-static const void *walk_kernel(int ring, int mode) {
-    if (mode == lane::kModeMinimize)
-        return ring ? reinterpret_cast<const void *>(minimize_zt_kernel) : reinterpret_cast<const void *>(minimize_z2_kernel);
-    if (mode == lane::kModeLearn) return ring ? nullptr : reinterpret_cast<const void *>(learn_z2_kernel);
-    return ring ? reinterpret_cast<const void *>(walk_zt_kernel) : reinterpret_cast<const void *>(walk_z2_kernel);
+size_t walk_lds_bytes(int variant, int max_m) {
+    return variant == kVarZT ? zt_lds_bytes(max_m) : z2_lds_bytes(max_m, variant == kVarZ2h ? 2 : 4);
 }
 
-hipError_t prepare_walk(int ring, int max_m, int mode, int *groups_per_cu) {
-    const size_t lds = walk_lds_bytes(ring, max_m);
-    const void *fn = walk_kernel(ring, mode);
+static const void *walk_kernel(int variant, int mode) {
+    const void *k[3][3] = {
+        {reinterpret_cast<const void *>(walk_z2_kernel), reinterpret_cast<const void *>(minimize_z2_kernel),
+         reinterpret_cast<const void *>(learn_z2_kernel)},
+        {reinterpret_cast<const void *>(walk_zt_kernel), reinterpret_cast<const void *>(minimize_zt_kernel), nullptr},
+        {reinterpret_cast<const void *>(walk_z2h_kernel), reinterpret_cast<const void *>(minimize_z2h_kernel),
+         reinterpret_cast<const void *>(learn_z2h_kernel)},
+    };
+    if (variant < 0 || variant > 2 || mode < 0 || mode > 2) return nullptr;
+    return k[variant][mode];
+}
+
+hipError_t prepare_walk(int variant, int max_m, int mode, int *groups_per_cu) {
+    const size_t lds = walk_lds_bytes(variant, max_m);
+    const void *fn = walk_kernel(variant, mode);
     if (!fn) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu, fn, kThreadsPerGroup, lds);
 }
 
-hipError_t launch_walk(int ring, int mode, const WalkArgs &a, int grid, hipStream_t stream) {
-    const size_t lds = walk_lds_bytes(ring, a.max_m);
-    if (!walk_kernel(ring, mode)) return hipErrorInvalidValue;
+hipError_t launch_walk(int variant, int mode, const WalkArgs &a, int grid, hipStream_t stream) {
+    const size_t lds = walk_lds_bytes(variant, a.max_m);
+    const void *fn = walk_kernel(variant, mode);
+    if (!fn) return hipErrorInvalidValue;
     hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, stream, a.queue, 0, 1);
-    if (mode == lane::kModeMinimize) {
-        if (ring) hipLaunchKernelGGL(minimize_zt_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
-        else hipLaunchKernelGGL(minimize_z2_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
-    } else if (mode == lane::kModeLearn) {
-        hipLaunchKernelGGL(learn_z2_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
-    } else {
-        if (ring) hipLaunchKernelGGL(walk_zt_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
-        else hipLaunchKernelGGL(walk_z2_kernel, dim3(grid), dim3(kThreadsPerGroup), lds, stream, a);
-    }
-    return hipGetLastError();
+    WalkArgs arg = a;
+    void *params[] = {&arg};
+    return hipLaunchKernel(fn, dim3(grid), dim3(kThreadsPerGroup), params, lds, stream);
 }
 
 hipError_t launch_broadcast_best(uint32_t *cur, int32_t *m, const uint32_t *best, const int32_t *best_m, int walkers,

@@ -73,18 +73,39 @@ constexpr int32_t kErrCapacity = 4; // a term would exceed the LDS capacity
 constexpr int kHTermsPad = 32;
 __host__ __device__ inline int h_words(int max_m) { return ((max_m + kHTermsPad - 1) / kHTermsPad) * (kHTermsPad / 4); }
 
-// LDS bytes per workgroup of the common planes: 32-bit factor words F and
-// the packed H and N words, for every lane.
-__host__ __device__ inline size_t walk_lds_base_bytes(int max_m) {
-    return (size_t)kWalkersPerGroup * (3 * (size_t)max_m * 4 + 2 * 3 * (size_t)h_words(max_m) * 4);
+// LDS term stride of the factor planes F.  The two lanes of a walker scan
+// two different components j != k in step; padding the stride makes their
+// rows fall in different LDS bank groups: with 16-bit words a row of 32
+// walkers covers a quarter of the 64 banks, so the stride is made 1 mod 4
+// (component offsets 1, 2 or 3 quarters apart); with 32-bit words a row
+// covers half the banks and the stride is made odd.
+__host__ __device__ inline int lds_term_stride(int max_m, int word_bytes) {
+    int fs = max_m;
+    if (word_bytes == 2) {
+        while (fs % 4 != 1) ++fs;
+    } else if (fs % 2 == 0) {
+        ++fs;
+    }
+    return fs;
 }
-__host__ __device__ inline size_t z2_lds_bytes(int max_m) { return walk_lds_base_bytes(max_m); }
+
+// LDS bytes per workgroup of the factor planes F (word_bytes per factor).
+__host__ __device__ inline size_t walk_lds_f_bytes(int max_m, int word_bytes) {
+    return ((size_t)kWalkersPerGroup * 3 * lds_term_stride(max_m, word_bytes) * word_bytes + 3) & ~(size_t)3;
+}
+
+// LDS bytes per workgroup of the common planes: factor words F and the
+// packed H and N words, for every lane.
+__host__ __device__ inline size_t walk_lds_base_bytes(int max_m, int word_bytes = 4) {
+    return walk_lds_f_bytes(max_m, word_bytes) + (size_t)kWalkersPerGroup * 2 * 3 * (size_t)h_words(max_m) * 4;
+}
+__host__ __device__ inline size_t z2_lds_bytes(int max_m, int word_bytes = 4) { return walk_lds_base_bytes(max_m, word_bytes); }
 
 // Integer ring: the candidate pairs one tryFlip visits are remembered in
 // LDS (16-bit indices, kZtMaxVisits per walker).
 constexpr int kZtMaxVisits = 16;
 __host__ __device__ inline size_t zt_lds_bytes(int max_m) {
-    return walk_lds_base_bytes(max_m) + (size_t)kWalkersPerGroup * kZtMaxVisits * 2;
+    return walk_lds_base_bytes(max_m, 4) + (size_t)kWalkersPerGroup * kZtMaxVisits * 2;
 }
 
 // Brent check of schemes held in the same plane layout.

@@ -8,7 +8,39 @@
 namespace fgk {
 namespace lane {
 
-struct Z2Lane : LaneBase {
+template <typename WordT>
+struct Z2LaneT : LaneBaseT<WordT> {
+    using B = LaneBaseT<WordT>;
+    using typename B::Word;
+    using typename B::JoinInfo;
+    using B::F;
+    using B::m;
+    using B::maxm;
+    using B::fs;
+    using B::side;
+    using B::S0;
+    using B::S1;
+    using B::S2;
+    using B::sel_c;
+    using B::sel_t1;
+    using B::sel_t2;
+    using B::f;
+    using B::nx;
+    using B::copy_term;
+    using B::nonzero_term;
+    using B::remove_at;
+    using B::rebuild;
+    using B::class_walk;
+    using B::leave;
+    using B::link_join;
+    using B::Sc;
+    using B::addS;
+    using B::pop_update;
+    using B::naive_rank;
+    using B::aux32;
+    using B::perm3;
+    using B::nn;
+    using B::select_pair;
     static constexpr uint32_t kValueMask = 0xFFFFFFFFu;
 
     // removeZeroes (scheme_z2.cu:152-164), including its trailing trim and
@@ -69,7 +101,7 @@ struct Z2Lane : LaneBase {
         hf.any = -1;
         hf.t = 1 << 30;
         hf.rule = 0;
-        const uint32_t *Fc = F + mc * maxm * WS;
+        const Word *Fc = F + mc * fs * WS;
 #pragma unroll 16
         for (int t = 0; t < m; ++t) hf.any = Fc[t * WS] == nv ? t : hf.any;
         if (hf.any >= 0)
@@ -290,6 +322,9 @@ struct Z2Lane : LaneBase {
         return true;
     }
 };
+
+using Z2Lane = Z2LaneT<uint32_t>;    // Z2 factors of up to 32 coefficients
+using Z2Lane16 = Z2LaneT<uint16_t>;  // Z2 factors of up to 16 coefficients
 
 }  // namespace lane
 }  // namespace fgk

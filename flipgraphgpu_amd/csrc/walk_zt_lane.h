@@ -58,7 +58,7 @@ FGK_HD uint32_t inverse(uint32_t a) { return pack(val(a), ~sgn(a) & val(a)); }
 
 }  // namespace tern
 
-struct ZTLane : LaneBase {
+struct ZTLane : LaneBaseT<uint32_t> {
     static constexpr uint32_t kValueMask = 0xFFFFu;  // magnitudes; signs sit in bits 16-31
 
     uint16_t *V;  // lane-offset base of the visited-candidate list
@@ -161,7 +161,7 @@ struct ZTLane : LaneBase {
         hf.any = -1;
         hf.t = 1 << 30;
         hf.rule = 0;
-        const uint32_t *Fc = F + mc * maxm * WS;
+        const uint32_t *Fc = F + mc * fs * WS;
 #pragma unroll 16
         for (int t = 0; t < m; ++t) hf.any = Fc[t * WS] == nv ? t : hf.any;
         if (hf.any >= 0)

@@ -44,6 +44,7 @@ typedef struct {
     int32_t zt_visit_cap;       /* integer ring: candidates one tryFlip may visit (0 = 16) */
     int32_t use_graph;          /* 1: replay each launch (queue reset, walk, Brent check) as a captured HIP graph */
     int32_t grid;               /* persistent workgroups per launch; 0 = all that fit on the GPU at once */
+    int32_t word_bits;          /* Z2 LDS factor words: 16 or 32; 0 = 16 only when it lets more walker blocks run at once */
 } fg_config;
 
 /* Per-walker state, for tests and checkpointing. */

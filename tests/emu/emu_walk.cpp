@@ -19,7 +19,7 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                                    int use_reduce, uint32_t thr_reduce, int use_expand, uint32_t thr_expand,
                                    uint32_t thr_best_eq, int check_reduce, int launches, int visit_cap,
                                    int mode, const int32_t *forced, int32_t *log_meta, int32_t *minm,
-                                   int32_t *err_out) {
+                                   int force_wide, int32_t *err_out) {
     int maxm = 1;
     for (int q = 0; q < count; ++q) {
         maxm = std::max(maxm, std::max(recs[q].m, recs[q].best_m));
@@ -61,7 +61,12 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
         a.log_meta = lm.data();
         a.minm = mm.data();
     }
-    std::vector<uint8_t> lds(ring ? fgk::zt_lds_bytes(maxm) : fgk::z2_lds_bytes(maxm), 0xCD);
+    // same word choice as the runtime: Z2 factors of up to 16 coefficients use 16-bit LDS words
+    int nn_max = 0;
+    for (int q = 0; q < count; ++q)
+        for (int c = 0; c < 3; ++c) nn_max = std::max(nn_max, recs[q].n[c] * recs[q].n[(c + 1) % 3]);
+    const bool half = !ring && nn_max <= 16 && !force_wide;
+    std::vector<uint8_t> lds(ring ? fgk::zt_lds_bytes(maxm) : fgk::z2_lds_bytes(maxm, half ? 2 : 4), 0xCD);
     const int G = (W + fgk::kWalkersPerGroup - 1) / fgk::kWalkersPerGroup;
     for (int l = 0; l < launches; ++l) {
         if (l > 0) a.forced = nullptr;  // proposals apply to the first launch
@@ -70,14 +75,19 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                 // one call per walker: the host side of the lane code does both flip halves
                 const int lane = slot * fgk::kLanesPerWalker;
                 const int w = g * fgk::kWalkersPerGroup + slot;
-                if (mode == fgk::lane::kModeMinimize) {
-                    if (ring) fgk::lane::walk_lane<fgk::lane::ZTLane, fgk::lane::kModeMinimize>(a, lds.data(), lane, w, W);
-                    else fgk::lane::walk_lane<fgk::lane::Z2Lane, fgk::lane::kModeMinimize>(a, lds.data(), lane, w, W);
-                } else if (mode == fgk::lane::kModeLearn && !ring) {
-                    fgk::lane::walk_lane<fgk::lane::Z2Lane, fgk::lane::kModeLearn>(a, lds.data(), lane, w, W);
+                using namespace fgk::lane;
+                uint8_t *L = lds.data();
+                if (ring) {
+                    if (mode == kModeMinimize) walk_lane<ZTLane, kModeMinimize>(a, L, lane, w, W);
+                    else walk_lane<ZTLane, kModeWalk>(a, L, lane, w, W);
+                } else if (half) {
+                    if (mode == kModeMinimize) walk_lane<Z2Lane16, kModeMinimize>(a, L, lane, w, W);
+                    else if (mode == kModeLearn) walk_lane<Z2Lane16, kModeLearn>(a, L, lane, w, W);
+                    else walk_lane<Z2Lane16, kModeWalk>(a, L, lane, w, W);
                 } else {
-                    if (ring) fgk::lane::walk_lane<fgk::lane::ZTLane, fgk::lane::kModeWalk>(a, lds.data(), lane, w, W);
-                    else fgk::lane::walk_lane<fgk::lane::Z2Lane, fgk::lane::kModeWalk>(a, lds.data(), lane, w, W);
+                    if (mode == kModeMinimize) walk_lane<Z2Lane, kModeMinimize>(a, L, lane, w, W);
+                    else if (mode == kModeLearn) walk_lane<Z2Lane, kModeLearn>(a, L, lane, w, W);
+                    else walk_lane<Z2Lane, kModeWalk>(a, L, lane, w, W);
                 }
             }
     }

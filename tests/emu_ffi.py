@@ -20,14 +20,17 @@ def lib():
         L.emu_walk_records.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64] + \
             [ctypes.c_int] * 4 + [ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
 
 
-def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=False):
+def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=False, wide=False):
     """Emulated launches; with learn (Z2) the first launch takes the proposed
-    pairs `forced` [walkers][4] and the result adds (log_meta, minm)."""
+    pairs `forced` [walkers][4] and the result adds (log_meta, minm).  Z2
+    factors of up to 16 coefficients run the 16-bit-word lane code unless
+    wide is set."""
     recs = np.ascontiguousarray(recs.copy())
     err = ctypes.c_int32()
     p = params
@@ -39,7 +42,7 @@ def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=Fals
                                 p.random_iterations, p.plus_iterations, p.use_reduce, p.thr_reduce,
                                 p.use_expand, p.thr_expand, p.thr_best_eq, p.check_reduce, launches,
                                 p.zt_visit_cap, mode, None if fz is None else fz.ctypes.data,
-                                lm.ctypes.data, mm.ctypes.data, ctypes.byref(err))
+                                lm.ctypes.data, mm.ctypes.data, int(wide), ctypes.byref(err))
     assert rc == 0
     if learn:
         return recs, err.value, lm, mm

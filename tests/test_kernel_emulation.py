@@ -61,3 +61,15 @@ def test_emulated_integer_kernel_matches_oracle(n, walkers, iterations):
 @pytest.mark.parametrize("n", [(2, 2, 2), (3, 3, 3)])
 def test_emulated_integer_kernel_plus_expand_paths(n):
     _check(n, 64, 2000, seed=8, launches=2, plus=150, p_expand=200.0, ring=O.RING_ZT, p_reduce=50.0)
+
+
+@pytest.mark.parametrize("n", [(2, 2, 2), (3, 3, 3), (4, 4, 4)])
+def test_emulated_wide_words_match_oracle(n):
+    # the 32-bit-word lane code, which formats with wider factors use,
+    # on formats that otherwise run with 16-bit words
+    start = O.naive_records(*n, 64)
+    p = O.make_params(iterations=1500, plus_iterations=200, p_expand=100.0)
+    emu, err = E.walk(start, p, 77, launches=2, wide=True)
+    ora = O.walk_records(start, p, 77, launches=2, nthreads=8)
+    assert err == 0
+    assert first_mismatch(emu, ora) is None, first_mismatch(emu, ora)

@@ -42,11 +42,12 @@ def test_scorer_forward_backward_match_oracle(count, M, n):
         assert np.array_equal(s2, s_gpu)
 
 
-def test_forced_walk_matches_oracle():
-    print("[learned] forced walk 4x4x4", flush=True)
+@pytest.mark.parametrize("bits", [32, 16])
+def test_forced_walk_matches_oracle(bits):
+    print(f"[learned] forced walk 4x4x4 bits={bits}", flush=True)
     rng = np.random.default_rng(7)
     W = 256
-    with fg.Engine(W, 21, max_iterations=300, plus_iterations=300) as e:
+    with fg.Engine(W, 21, max_iterations=300, plus_iterations=300, word_bits=bits) as e:
         e.init_naive(4, 4, 4)
         assert e.walk(300) == 0
         recs = e.read_walkers()

@@ -18,16 +18,17 @@ def _blob_of(schemes):
     return np.concatenate([np.array([len(schemes)], dtype=np.int32)] + list(schemes))
 
 
-@pytest.mark.parametrize("ring,n,walkers,iterations", [
-    (O.RING_Z2, (3, 3, 3), 1024, 3000),
-    (O.RING_Z2, (4, 4, 4), 256, 2000),
-    (O.RING_ZT, (3, 3, 3), 512, 3000),
+@pytest.mark.parametrize("ring,n,walkers,iterations,bits", [
+    (O.RING_Z2, (3, 3, 3), 1024, 3000, 0),
+    (O.RING_Z2, (4, 4, 4), 256, 2000, 0),
+    (O.RING_Z2, (3, 3, 3), 256, 2000, 16),
+    (O.RING_ZT, (3, 3, 3), 512, 3000, 0),
 ])
-def test_minimizer_bit_identical_to_oracle(ring, n, walkers, iterations):
-    print(f"[minimizer] ring={ring} n={n} walkers={walkers} iterations={iterations}", flush=True)
+def test_minimizer_bit_identical_to_oracle(ring, n, walkers, iterations, bits):
+    print(f"[minimizer] ring={ring} n={n} walkers={walkers} iterations={iterations} bits={bits}", flush=True)
     schemes = walked_schemes(n, 24, 2000, 5, ring)
     seed = 2024
-    with fg.Engine(walkers, seed, ring=ring, max_iterations=iterations, check_reduce=False) as e:
+    with fg.Engine(walkers, seed, ring=ring, max_iterations=iterations, check_reduce=False, word_bits=bits) as e:
         e.init_minimizer(_blob_of(schemes))
         init = e.read_walkers()
         # initializeKernel: round-robin copies, best = start, best complexity = global minimum

@@ -13,12 +13,12 @@ pytestmark = pytest.mark.gpu
 
 
 def _run_both(n, walkers, iterations, seed, launches=1, plus=8000, p_expand=0.0, random_iterations=False,
-              ring=O.RING_Z2, p_reduce=0.0):
+              ring=O.RING_Z2, p_reduce=0.0, word_bits=0):
     print(f"[parity] ring={ring} n={n} walkers={walkers} iterations={iterations} launches={launches} seed={seed} "
           f"plus={plus} p_expand={p_expand} p_reduce={p_reduce} random={random_iterations}", flush=True)
     with fg.Engine(walkers, seed, ring=ring, max_iterations=iterations, plus_iterations=plus,
                    expand_probability=p_expand, reduce_probability=p_reduce,
-                   random_iterations=random_iterations, zt_visit_cap=16) as e:
+                   random_iterations=random_iterations, zt_visit_cap=16, word_bits=word_bits) as e:
         e.init_naive(*n)
         init = e.read_walkers()
         ora0 = O.naive_records(*n, walkers, ring=ring)
@@ -49,6 +49,13 @@ def test_walk_bit_identical_to_oracle(n, walkers, iterations):
 def test_walk_with_plus_and_expand_bit_identical(n):
     # small plateau and a high expand rate exercise plus / split / rebuild paths
     gpu, ora = _run_both(n, 128, 2500, seed=7, launches=2, plus=150, p_expand=200.0)
+    assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
+
+
+@pytest.mark.parametrize("n", [(2, 2, 2), (3, 3, 3), (2, 3, 4), (4, 4, 4)])
+def test_walk_16bit_words_bit_identical(n):
+    # Z2 factors of up to 16 coefficients held in 16-bit LDS words
+    gpu, ora = _run_both(n, 128, 2000, seed=31, launches=2, plus=200, p_expand=100.0, word_bits=16)
     assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
 
 
