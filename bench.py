@@ -42,6 +42,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "ttt"],
+                    help="c2: the throughput bench (default); c1: 2x2x2 time to rank 7; c3: batched uneven formats; "
+                         "c4: integer-ring walks + additions reducer + complexity minimizer; ttt: learned vs uniform")
+    ap.add_argument("--launches", type=int, default=0, help="launch budget of the c1/c3/c4 workloads (0: default)")
     tt = ap.add_argument_group("time to target (learned vs uniform proposals, 4x4x4 Z2)")
     tt.add_argument("--ttt", action="store_true", help="run the time-to-target study instead of the throughput bench")
     tt.add_argument("--seeds", type=int, default=10)
@@ -201,8 +205,164 @@ def time_to_target(a):
     print(json.dumps(out))
 
 
+def workload_c1(a):
+    """2x2x2 Z2 from the naive rank-8 scheme, 1024 walkers, plus after a
+    10000-flip plateau: wall time to a verified rank-7 scheme (flip budget 1e6
+    per walker)."""
+    import flipgraphgpu_amd as fg
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+
+    e = fg.Engine(1024, a.seed, max_iterations=1000, plus_iterations=10000, expand_probability=0.0)
+    e.init_naive(2, 2, 2)
+    c0 = e.counters()
+    t0 = time.perf_counter()
+    launches, best = 0, 8
+    while launches < 1000:
+        if e.walk(1000):
+            raise RuntimeError("invalid schemes")
+        launches += 1
+        blob, best, _ = e.best()
+        if best <= 7 and fg.verify([blob])[0] and O.validate(blob) == 1:
+            break
+    dt = time.perf_counter() - t0
+    flips = e.counters()["flips"] - c0["flips"]
+    e.close()
+    print(json.dumps({"metric": "time to verified 2x2x2 Z2 rank 7", "value": dt, "unit": "s",
+                      "higher_is_better": False, "n_gpus": 1, "best_rank": int(best), "launches": launches,
+                      "flips": int(flips), "flips_per_s": flips / dt,
+                      "config": {"workload": "2x2x2 Z2, naive rank-8 start, 1024 walkers, plus-iterations 10000, "
+                                             "launches of 1000 iterations (budget 1e6 per walker)"}}))
+
+
+def workload_c3(a):
+    """Five uneven Z2 formats in one persistent pool: 262144 walkers split
+    1:2:4:8:16, autotuned grid, per-format best ranks after a launch budget
+    against target = naive rank - 20%."""
+    import numpy as np
+
+    import flipgraphgpu_amd as fg
+
+    formats = [(2, 2, 3), (2, 3, 4), (3, 3, 4), (3, 4, 5), (4, 4, 4)]
+    weights = [1, 2, 4, 8, 16]
+    W, L = 262144, 1000
+    launches = a.launches or 40
+    e = fg.Engine(W, a.seed, max_iterations=L, plus_iterations=20000, expand_probability=0.0)
+    e.init_batched(formats, weights)
+    grid = e.autotune(200)
+    c0 = e.counters()
+    t0 = time.perf_counter()
+    kernel_ms = 0.0
+    for _ in range(launches):
+        if e.walk(L):
+            raise RuntimeError("invalid schemes")
+        kernel_ms += e.last_walk_ms()
+    dt = time.perf_counter() - t0
+    c1 = e.counters()
+    summ = e.summary()
+    n = summ["n"]
+    per = {}
+    for f in formats:
+        sel = np.all(n == np.array(f), axis=1)
+        naive = f[0] * f[1] * f[2]
+        best = int(summ["best_m"][sel].min())
+        per["x".join(map(str, f))] = {"walkers": int(sel.sum()), "naive": naive, "target": round(naive * 0.8, 1),
+                                     "best": best, "reached": best <= naive * 0.8}
+    e.close()
+    flips = c1["flips"] - c0["flips"]
+    print(json.dumps({"metric": "flips/s (batched uneven Z2 formats, 262144 walkers)", "value": flips / dt,
+                      "unit": "flips/s", "n_gpus": 1, "launches": launches, "seconds": dt,
+                      "kernel_ms_per_launch": kernel_ms / launches, "autotuned_grid": grid, "formats": per,
+                      "config": {"workload": "Z2 formats 2x2x3/2x3x4/3x3x4/3x4x5/4x4x4 from naive, walkers 1:2:4:8:16 "
+                                             "of 262144 (largest remainder), plus-iterations 20000, launches of 1000 "
+                                             "iterations, one persistent launch for all formats"}}))
+
+
+def workload_c4(a):
+    """Integer-ring ({-1,0,1}) walks of 3x3x3 and 4x4x4, 65536 walkers each,
+    from the naive scheme; the best scheme found goes to the additions
+    reducer and the walkers' schemes of that rank to the complexity
+    minimizer.  Reports additions before and after."""
+    import numpy as np
+
+    import flipgraphgpu_amd as fg
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+
+    out = {"metric": "additions of integer-ring schemes before / after reduction", "n_gpus": 1, "formats": {}}
+    for n, launches in (((3, 3, 3), a.launches or 30), ((4, 4, 4), a.launches or 30)):
+        W = 65536
+        e = fg.Engine(W, a.seed, ring=fg.RING_ZT, max_iterations=1000, plus_iterations=20000,
+                      expand_probability=0.0)
+        e.init_naive(*n)
+        c0 = e.counters()
+        t0 = time.perf_counter()
+        for _ in range(launches):
+            if e.walk(1000):
+                raise RuntimeError("invalid schemes")
+        dt = time.perf_counter() - t0
+        flips = e.counters()["flips"] - c0["flips"]
+        blob, rank, _ = e.best()
+        assert fg.verify([blob], ring=fg.RING_ZT)[0] and O.validate(blob, O.RING_ZT) == 1
+        recs = e.read_walkers()
+        cands = [w for w in range(W) if recs[w]["best_m"] == rank][:256]
+        blobs = [e.walker_scheme(w, best=True) for w in cands]
+        e.close()
+        naive_cx = O.complexity(O.naive_blob(*n, ring=O.RING_ZT), O.RING_ZT)
+        cx_found = min(O.complexity(b, O.RING_ZT) for b in blobs)
+        # complexity minimizer over the walkers' best schemes of that rank
+        t1 = time.perf_counter()
+        z = fg.Engine(4096, a.seed, ring=fg.RING_ZT, max_iterations=10000, check_reduce=False)
+        z.init_minimizer(np.concatenate([np.array([len(blobs)], dtype=np.int32)] + blobs))
+        best_cx = None
+        for it in range(8):
+            if z.minimize():
+                raise RuntimeError("invalid schemes in the minimizer")
+            bc, _ = z.complexities()
+            bw = int(np.argmin(bc))
+            if best_cx is not None and bc[bw] >= best_cx:
+                break
+            best_cx = int(bc[bw])
+            z.broadcast_best(bw, list(range(0, 4096, 2)))
+        mblob, mcx, _ = z.best()
+        z.close()
+        t_min = time.perf_counter() - t1
+        # additions reducer on the minimized scheme
+        t2 = time.perf_counter()
+        r = fg.Reducer(mblob, count=512, seed=a.seed)
+        no_imp = 0
+        while no_imp < 3:
+            r.run()
+            no_imp = 0 if r.update() else no_imp + 1
+        b = r.best()
+        r.close()
+        t_red = time.perf_counter() - t2
+        out["formats"]["x".join(map(str, n))] = {
+            "walkers": W, "launches": launches, "walk_seconds": round(dt, 2), "flips_per_s": flips / dt,
+            "best_rank": int(rank), "naive_rank": n[0] * n[1] * n[2],
+            "additions_naive_scheme": int(naive_cx),
+            "additions_found_scheme": int(cx_found),
+            "additions_after_minimizer": int(mcx), "minimizer_seconds": round(t_min, 2),
+            "reducer_naive_additions": int(sum(b["naive"])), "reducer_reduced_additions": int(b["total"]),
+            "reducer_fresh_variables": int(b["total_fresh"]), "reducer_seconds": round(t_red, 2),
+        }
+        print(f"[c4] {n}: rank {rank}, additions {cx_found} -> {mcx} (minimizer) -> {b['total']} (reducer)",
+              file=sys.stderr, flush=True)
+    out["config"] = {"workload": "integer ring {-1,0,1}, 65536 walkers per format from naive, launches of 1000 "
+                                 "iterations; best rank's schemes -> complexity minimizer (4096 walkers) -> additions "
+                                 "reducer (512 reducers, until 3 passes without improvement)"}
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
+    if a.workload in ("c1", "c3", "c4") and a.gpus == 1:
+        {"c1": workload_c1, "c3": workload_c3, "c4": workload_c4}[a.workload](a)
+        return
+    if a.workload == "ttt":
+        a.ttt = True
     if a.calibrate_copy:
         calibrate_copy(a.calibrate_copy)
         return
