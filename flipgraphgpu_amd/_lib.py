@@ -76,6 +76,8 @@ def load_library():
     L.fg_init_batched.argtypes = [vp, ctypes.c_int, i32p, i32p]
     L.fg_init_minimizer.argtypes = [vp, i32p, ctypes.c_int64]
     L.fg_minimize.argtypes = [vp]
+    L.fg_init_flips.argtypes = [vp, i32p, ctypes.c_int64]
+    L.fg_flip_schemes.argtypes = [vp, ctypes.c_int]
     L.fg_complexities.argtypes = [vp, i32p, i32p]
     L.fg_broadcast_best.argtypes = [vp, ctypes.c_int, i32p, ctypes.c_int]
     L.fg_autotune.argtypes = [vp, ctypes.c_int, i32p]
@@ -197,6 +199,18 @@ class Engine:
         """Minimizer pool from a count-prefixed blob of schemes of one format."""
         b, p = _i32(blob)
         _check(self.L.fg_init_minimizer(self.ctx, p, len(b)), "fg_init_minimizer")
+
+    def init_flips(self, blob):
+        """Scheme pool of the additions reducer's --max-flips mode: every walker
+        starts from the single integer-ring scheme `blob` (no count prefix)."""
+        b = np.concatenate([[1], np.asarray(blob, dtype=np.int32)]).astype(np.int32)
+        p = b.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        _check(self.L.fg_init_flips(self.ctx, p, len(b)), "fg_init_flips")
+
+    def flip_schemes(self, max_flips):
+        """flipSchemesKernel: walkers >= 1 restart from walker 0's scheme and take
+        up to max_flips flips; returns the number of schemes failing the Brent check."""
+        return _check(self.L.fg_flip_schemes(self.ctx, int(max_flips)), "fg_flip_schemes")
 
     def minimize(self):
         """One minimizer launch; returns the number of current schemes failing the GPU Brent check."""

@@ -443,6 +443,7 @@ FGK_HD bool wave_any(bool v) {
 constexpr int kModeWalk = 0;      // randomWalkKernel
 constexpr int kModeMinimize = 1;  // minimizeKernel
 constexpr int kModeLearn = 2;     // walk + proposed first flip + trajectory logging
+constexpr int kModeFlips = 3;     // flipSchemesKernel of the additions reducer
 
 // The walk of one lane (randomWalkKernel body, src/entities/flip_graph.cu:520-573),
 // or in kModeMinimize the complexity minimizer's (minimizeKernel,
@@ -452,7 +453,11 @@ constexpr int kModeLearn = 2;     // walk + proposed first flip + trajectory log
 // first iteration of a launch: a proposed pair (A.forced) replaces the
 // uniform pair draw when it is still a flip, and logged walkers record their
 // scheme and the pair they flip (A.log_words / A.log_meta); the lowest rank
-// of the launch goes to A.minm.  `lds` is the workgroup's LDS block, `lane`
+// of the launch goes to A.minm.  kModeFlips is the additions reducer's
+// scheme flips (flipSchemesKernel, src/entities/scheme_additions_reducer.cu:448-460):
+// every walker but walker 0 draws a count in [0, A.iterations] from its aux
+// stream and applies that many flips with reductions, stopping at the first
+// that finds no pair.  `lds` is the workgroup's LDS block, `lane`
 // the lane in the workgroup (walker slot lane / 2, flip side lane % 2), `w`
 // the walker index.  The host emulation calls it once per walker with an
 // even lane and does both sides.
@@ -536,6 +541,25 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
                 }
             }
             copy_best(s, A, copy, lane, w);
+        }
+    } else if constexpr (kMode == kModeFlips) {
+        int count = 0;
+        if (active && w != 0) count = (int)(s.aux32() % (uint32_t)(A.iterations + 1));  // :454
+        const int trips = wave_max(count);
+        bool live = active;
+        for (int iter = 0; iter < trips && wave_any(live); ++iter) {
+            if (live && iter < count) {
+                uint32_t b0, b1, b2, b3;
+                philox10((uint32_t)it, (uint32_t)(it >> 32), (uint32_t)w, 0u, s.k0, s.k1, b0, b1, b2, b3);
+                ++it;
+                const int rank = s.m;
+                if (!s.try_flip(b0, b1, true)) {  // :458-459
+                    live = false;
+                } else {
+                    ++total_flips;
+                    sum_m += (uint64_t)rank;
+                }
+            }
         }
     } else {
     int iters = 0;

@@ -12,6 +12,7 @@
 
 #include "fg.h"
 #include "reducer_args.h"
+#include "runtime_internal.h"
 #include "scheme_host.h"
 
 namespace fgk {
@@ -67,7 +68,10 @@ struct fg_reducer {
     fg::Scheme scheme;
     int n[3] = {0, 0, 0}, nn[3] = {0, 0, 0}, m = 0;
     int e_cap = 0, l_cap = 0, f_cap = 500, p_cap = 0, h_bits = 6;
-    size_t lds = 0;
+    size_t lds = 0, lds_limit = 0;
+    int S = 1;               // schemes the reducers read (reducer r takes scheme r % S)
+    int max_nn = 0;
+    fg_ctx *pool = nullptr;  // --max-flips > 0: the flipped schemes (walker w = scheme w)
 
     int8_t *coef = nullptr;
     int32_t *sm = nullptr, *best_fresh_d = nullptr, *info = nullptr, *err = nullptr;
@@ -88,6 +92,7 @@ struct fg_reducer {
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
+        if (pool) fg_destroy(pool);
     }
 };
 
@@ -166,6 +171,71 @@ int max_subexpressions(const fg::Scheme &s, int c) {
             }
     }
     return total;
+}
+
+// Pair occurrences of one component of scheme q of the coefficient array
+// (the same count as occurrences() on a host scheme).
+long long coef_occurrences(const fg_reducer *R, const std::vector<int8_t> &coef, int q, int m, int c) {
+    const int8_t *base = coef.data() + (size_t)q * 3 * R->m * R->max_nn;
+    long long p = 0;
+    if (c < 2) {
+        for (int t = 0; t < m; ++t) {
+            long long z = 0;
+            for (int k = 0; k < R->nn[c]; ++k) z += base[((size_t)c * R->m + t) * R->max_nn + k] != 0;
+            p += z * (z - 1) / 2;
+        }
+    } else {
+        for (int k = 0; k < R->nn[2]; ++k) {
+            long long z = 0;
+            for (int t = 0; t < m; ++t) z += base[((size_t)2 * R->m + t) * R->max_nn + k] != 0;
+            p += z * (z - 1) / 2;
+        }
+    }
+    return p;
+}
+
+// Pair-table capacity and LDS of one pass for the largest occurrence count.
+int size_pairs(fg_reducer *R, long long occ) {
+    R->p_cap = (int)std::max<long long>(occ, 64);
+    R->h_bits = 6;
+    while ((1LL << R->h_bits) < 2LL * R->p_cap) ++R->h_bits;
+    R->lds = fgk::reducer_lds_bytes(R->e_cap, R->l_cap, R->f_cap, R->p_cap, R->h_bits);
+    if (R->lds > R->lds_limit)
+        return rfail("fg_reducer: a scheme needs " + std::to_string(R->lds) +
+                     " bytes of LDS per reducer component, more than one workgroup has");
+    return 0;
+}
+
+// flipSchemesKernel (scheme_additions_reducer.cu:448-460) on the flip pool,
+// then the pool's schemes become the reducers' inputs.
+int flip_and_load(fg_reducer *R) {
+    const int bad = fg_flip_schemes(R->pool, R->max_flips);
+    if (bad < 0) return bad;
+    if (bad > 0) return rfail("fg_reducer_run: " + std::to_string(bad) + " flipped schemes fail the Brent check", -3);
+    fgrt::PoolView v{};
+    if (!fgrt::pool_view(R->pool, &v)) return rfail("fg_reducer_run: flip pool not initialised");
+    std::vector<uint32_t> words((size_t)3 * v.maxm * v.W);
+    std::vector<int32_t> m(v.W);
+    RCHK(hipMemcpy(words.data(), v.cur, words.size() * 4, hipMemcpyDeviceToHost));
+    RCHK(hipMemcpy(m.data(), v.m, m.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<int8_t> coef((size_t)R->S * 3 * R->m * R->max_nn, 0);
+    long long occ = 0;
+    for (int q = 0; q < R->S; ++q) {
+        if (m[q] < 1 || m[q] > R->m) return rfail("fg_reducer_run: flipped scheme with a bad rank");
+        for (int c = 0; c < 3; ++c)
+            for (int t = 0; t < m[q]; ++t) {
+                const uint32_t x = words[((size_t)c * v.maxm + t) * v.W + q];
+                for (int k = 0; k < R->nn[c]; ++k)
+                    if ((x >> k) & 1u)
+                        coef[(((size_t)q * 3 + c) * R->m + t) * R->max_nn + k] = ((x >> (16 + k)) & 1u) ? -1 : 1;
+            }
+        for (int c = 0; c < 3; ++c) occ = std::max(occ, coef_occurrences(R, coef, q, m[q], c));
+    }
+    const int rc = size_pairs(R, occ);
+    if (rc) return rc;
+    RCHK(hipMemcpy(R->coef, coef.data(), coef.size(), hipMemcpyHostToDevice));
+    RCHK(hipMemcpy(R->sm, m.data(), (size_t)R->S * 4, hipMemcpyHostToDevice));
+    return 0;
 }
 
 int read_back(fg_reducer *R) {
@@ -258,7 +328,6 @@ extern "C" int fg_reducer_create(const int32_t *blob, int64_t len, int count, in
     if (!blob || !out || len < 4) return rfail("fg_reducer_create: bad arguments");
     *out = nullptr;
     if (count < 1 || schemes_count < 1 || max_flips < 0) return rfail("fg_reducer_create: counts must be positive");
-    if (max_flips > 0) return rfail("fg_reducer_create: scheme flips between passes (--max-flips > 0) are not built");
     const int m = blob[3], n1 = blob[0], n2 = blob[1], n3 = blob[2];
     if (n1 < 1 || n2 < 1 || n3 < 1 || m < 1) return rfail("fg_reducer_create: bad scheme header");
     // SchemeAdditionsReducer::read (scheme_additions_reducer.cu:29-101)
@@ -302,10 +371,6 @@ extern "C" int fg_reducer_create(const int32_t *blob, int64_t len, int count, in
     R->l_cap = std::max(std::max(R->nn[0], R->nn[1]), m);
     long long p = 0;
     for (int c = 0; c < 3; ++c) p = std::max(p, occurrences(s, c));
-    R->p_cap = (int)std::max<long long>(p, 64);
-    R->h_bits = 6;
-    while ((1LL << R->h_bits) < 2LL * R->p_cap) ++R->h_bits;
-    R->lds = fgk::reducer_lds_bytes(R->e_cap, R->l_cap, R->f_cap, R->p_cap, R->h_bits);
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
@@ -316,20 +381,23 @@ extern "C" int fg_reducer_create(const int32_t *blob, int64_t len, int count, in
         delete R;
         return rfail(std::string("fg_reducer_create: this build targets gfx950 (MI355X), found ") + prop.gcnArchName);
     }
-    if (R->lds > prop.sharedMemPerBlock) {
-        const size_t need = R->lds;
+    R->lds_limit = prop.sharedMemPerBlock;
+    if (size_pairs(R, p)) {
         delete R;
-        return rfail("fg_reducer_create: the scheme needs " + std::to_string(need) +
-                     " bytes of LDS per reducer component, more than one workgroup has");
+        return -1;
     }
     // the reducers read scheme r % schemes; without flips every scheme is a
     // copy of the input (initializeKernel, :433-446), so one copy suffices
-    const int S = 1;
+    const int S = max_flips > 0 ? schemes_count : 1;
     const int max_nn = std::max(std::max(R->nn[0], R->nn[1]), R->nn[2]);
+    R->S = S;
+    R->max_nn = max_nn;
     std::vector<int8_t> coef((size_t)S * 3 * m * max_nn, 0);
-    for (int c = 0; c < 3; ++c)
-        for (int t = 0; t < m; ++t)
-            for (int k = 0; k < R->nn[c]; ++k) coef[((size_t)c * m + t) * max_nn + k] = (int8_t)coefficient(s, c, t, k);
+    for (int q = 0; q < S; ++q)
+        for (int c = 0; c < 3; ++c)
+            for (int t = 0; t < m; ++t)
+                for (int k = 0; k < R->nn[c]; ++k)
+                    coef[(((size_t)q * 3 + c) * m + t) * max_nn + k] = (int8_t)coefficient(s, c, t, k);
     std::vector<int32_t> sm(S, m);
     const size_t rc = (size_t)count * 3;
     if (hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -347,6 +415,24 @@ extern "C" int fg_reducer_create(const int32_t *blob, int64_t len, int count, in
         hipMemcpy(R->words, w0.data(), w0.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
         delete R;
         return rfail("fg_reducer_create: upload failed");
+    }
+    if (max_flips > 0) {
+        // the flipped schemes live in a walker pool of the integer-ring walk
+        fg_config pc{};
+        pc.ring = FG_RING_ZT;
+        pc.walkers = schemes_count;
+        pc.max_iterations = max_flips;
+        pc.plus_iterations = 1 << 30;
+        pc.check_reduce = 1;
+        pc.seed = seed;
+        std::vector<int32_t> one(1 + (size_t)len);
+        one[0] = 1;
+        std::copy(blob, blob + len, one.begin() + 1);
+        if (fg_create(&pc, &R->pool) < 0 || fg_init_flips(R->pool, one.data(), (int64_t)one.size()) < 0) {
+            const std::string e = fg_last_error();
+            delete R;
+            return rfail("fg_reducer_create: flip pool: " + e);
+        }
     }
     // initialize (:144-175): the best reducers start as the plain expressions
     for (int c = 0; c < 3; ++c) {
@@ -409,11 +495,14 @@ extern "C" int fg_reducer_scheme_stats(fg_reducer *r, int32_t *out) {
 
 extern "C" int fg_reducer_run(fg_reducer *r, int iteration) {
     if (!r) return rfail("fg_reducer_run: null reducer");
-    (void)iteration;
+    if (r->max_flips > 0 && iteration > 1) {  // reduceIteration (:172-176)
+        const int rc = flip_and_load(r);
+        if (rc) return rc;
+    }
     fgk::ReduceArgs a{};
     a.coef = r->coef;
     a.sm = r->sm;
-    a.schemes = 1;
+    a.schemes = r->S;
     a.n1 = r->n[0];
     a.n2 = r->n[1];
     a.n3 = r->n[2];

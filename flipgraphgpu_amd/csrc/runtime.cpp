@@ -78,6 +78,7 @@ struct fg_ctx {
     int32_t *forced = nullptr;  // [W][4] proposed first flips (learned walks)
     int nitems = 0;
     bool minimize = false;  // pool set up by fg_init_minimizer
+    bool flips = false;     // pool set up by fg_init_flips
     int variant = 0;        // walk kernel variant (kernels.h: kVarZ2 / kVarZT / kVarZ2h)
 
     // trajectory log: ring of the last `horizon` launches, labelled rows
@@ -132,6 +133,7 @@ struct fg_ctx {
         log.release();
         nitems = 0;
         minimize = false;
+        flips = false;
         ready = false;
     }
 };
@@ -202,9 +204,10 @@ namespace {
 // Fill the pool from start schemes (walker w gets starts[w % count]) and
 // set bestRanks as FlipGraph::updateRanks does (flip_graph.cu:249-277): the
 // smallest rank per sorted format key, seeded with the naive rank of the
-// configured format.
+// configured format.  mode: the walk kernel mode the pool runs (0 walk,
+// 1 complexity minimizer, 3 the additions reducer's scheme flips).
 int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vector<int> &assign, int n1, int n2,
-              int n3, bool minimize = false) {
+              int n3, int mode = 0) {
     if (starts.empty()) return fail("init: no schemes");
     const int W = ctx->cfg.walkers;
     int maxm = ctx->cfg.max_rank;
@@ -320,12 +323,13 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
     HIPCHK(dalloc(&ctx->forced, (size_t)4 * W));
     HIPCHK(fgk::launch_fill(ctx->forced, -1, 4 * W, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    ctx->minimize = minimize;
+    ctx->minimize = mode == 1;
+    ctx->flips = mode == 3;
     ctx->nitems = (int)ib.size();
     HIPCHK(hipMemcpy(ctx->item_begin, ib.data(), ib.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->item_end, ie.data(), ie.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(fgk::prepare_walk(variant, maxm, minimize ? 1 : 0, &ctx->groups_per_cu));
-    if (!minimize && ring == FG_RING_Z2) {
+    HIPCHK(fgk::prepare_walk(variant, maxm, mode, &ctx->groups_per_cu));
+    if (mode == 0 && ring == FG_RING_Z2) {
         int g2 = 0;
         HIPCHK(fgk::prepare_walk(variant, maxm, 2, &g2));
     }
@@ -419,11 +423,43 @@ extern "C" int fg_init_minimizer(fg_ctx *ctx, const int32_t *blob, int64_t len) 
     std::vector<int> assign(ctx->cfg.walkers);
     for (int w = 0; w < ctx->cfg.walkers; ++w) assign[w] = w % (int)starts.size();
     const fg::Scheme &s0 = starts[0];
-    const int rc = init_pool(ctx, starts, assign, s0.n[0], s0.n[1], s0.n[2], true);
+    const int rc = init_pool(ctx, starts, assign, s0.n[0], s0.n[1], s0.n[2], 1);
     if (rc) return rc;
     std::vector<int32_t> bc(ctx->W, best_c);
     HIPCHK(hipMemcpy(ctx->best_rank, bc.data(), bc.size() * 4, hipMemcpyHostToDevice));
     return 0;
+}
+
+namespace {
+int walk_launch(fg_ctx *ctx, int iterations, bool use_forced);
+}
+
+extern "C" int fg_init_flips(fg_ctx *ctx, const int32_t *blob, int64_t len) {
+    if (!ctx || !blob || len < 1) return fail("fg_init_flips: bad arguments");
+    if (ctx->cfg.ring != FG_RING_ZT) return fail("fg_init_flips: scheme flips are built for the integer ring");
+    if (blob[0] != 1) return fail("fg_init_flips: exactly one scheme expected");
+    fg::Scheme s;
+    std::string err;
+    int64_t pos = 1;
+    if (!fg::scheme_from_blob(ctx->cfg.ring, blob, len, &pos, &s, &err)) return fail("fg_init_flips: " + err);
+    if (!fg::validate(s)) return fail("fg_init_flips: the scheme fails the Brent check", -2);
+    std::vector<fg::Scheme> starts{s};
+    return init_pool(ctx, starts, std::vector<int>(ctx->cfg.walkers, 0), s.n[0], s.n[1], s.n[2], 3);
+}
+
+extern "C" int fg_flip_schemes(fg_ctx *ctx, int max_flips) {
+    if (!ctx || !ctx->ready) return fail("fg_flip_schemes: context not initialised");
+    if (!ctx->flips) return fail("fg_flip_schemes: the pool was not set up by fg_init_flips");
+    if (max_flips < 1) return fail("fg_flip_schemes: max_flips must be positive");
+    // schemes[0].copyTo(schemes[idx]) (:456): walker 0's scheme never changes
+    // and is its best (start) scheme, so the copy is a broadcast of that best
+    if (ctx->W > 1) {
+        std::vector<int32_t> dst(ctx->W - 1);
+        for (int w = 1; w < ctx->W; ++w) dst[w - 1] = w;
+        const int rc = fg_broadcast_best(ctx, 0, dst.data(), (int)dst.size());
+        if (rc) return rc;
+    }
+    return walk_launch(ctx, max_flips, false);
 }
 
 extern "C" int fg_minimize(fg_ctx *ctx) {
@@ -492,7 +528,10 @@ namespace {
 int walk_launch(fg_ctx *ctx, int iterations, bool use_forced);
 }
 
-extern "C" int fg_walk(fg_ctx *ctx, int iterations) { return walk_launch(ctx, iterations, false); }
+extern "C" int fg_walk(fg_ctx *ctx, int iterations) {
+    if (ctx && ctx->flips) return fail("fg_walk: a scheme-flips pool runs fg_flip_schemes");
+    return walk_launch(ctx, iterations, false);
+}
 
 namespace fgrt {
 int walk_forced(fg_ctx *ctx, int iterations) { return walk_launch(ctx, iterations, true); }
@@ -529,7 +568,7 @@ bool log_view(fg_ctx *ctx, LogView *v) {
 namespace {
 int walk_launch(fg_ctx *ctx, int iterations, bool use_forced) {
     if (!ctx || !ctx->ready) return fail("fg_walk: context not initialised");
-    const bool learn = !ctx->minimize && (use_forced || ctx->log.on);
+    const bool learn = !ctx->minimize && !ctx->flips && (use_forced || ctx->log.on);
     if (learn && ctx->cfg.ring != FG_RING_Z2) return fail("fg_walk: proposals and trajectory logs are built for Z2 walks");
     fgk::WalkArgs a{};
     a.cur = ctx->cur;
@@ -565,7 +604,7 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced) {
     a.nitems = ctx->nitems;
     a.queue = ctx->queue;
     a.cx = ctx->cx;
-    const int mode = ctx->minimize ? 1 : (learn ? 2 : 0);
+    const int mode = ctx->minimize ? 1 : (ctx->flips ? 3 : (learn ? 2 : 0));
     const int slot = ctx->log.on ? (int)(ctx->log.launches % ctx->log.horizon) : 0;
     if (learn) {
         a.forced = use_forced ? ctx->forced : nullptr;
@@ -582,6 +621,12 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced) {
         a.random_iterations = 0;
         a.use_reduce = a.use_expand = 0;
         a.check_reduce = 0;
+    }
+    if (ctx->flips) {
+        // flipSchemesKernel: up to `iterations` flips with reductions, nothing else
+        a.random_iterations = 0;
+        a.use_reduce = a.use_expand = 0;
+        a.check_reduce = 1;
     }
     // one launch: error flag reset, queue reset + persistent walk (timed),
     // then the Brent check of every current scheme

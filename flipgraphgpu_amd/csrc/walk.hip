@@ -61,6 +61,13 @@ __global__ void __launch_bounds__(kThreadsPerGroup) minimize_z2h_kernel(WalkArgs
     walk_persistent<lane::Z2Lane16, lane::kModeMinimize>(A, lds);
 }
 
+// Scheme flips of the additions reducer (flipSchemesKernel,
+// src/entities/scheme_additions_reducer.cu:448-460), integer ring.
+__global__ void __launch_bounds__(kThreadsPerGroup) flips_zt_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::ZTLane, lane::kModeFlips>(A, lds);
+}
+
 // Walk with a proposed first flip and trajectory logging (Z2).
 __global__ void __launch_bounds__(kThreadsPerGroup) learn_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
@@ -197,14 +204,15 @@ size_t walk_lds_bytes(int variant, int max_m) {
 }
 
 static const void *walk_kernel(int variant, int mode) {
-    const void *k[3][3] = {
+    const void *k[3][4] = {
         {reinterpret_cast<const void *>(walk_z2_kernel), reinterpret_cast<const void *>(minimize_z2_kernel),
-         reinterpret_cast<const void *>(learn_z2_kernel)},
-        {reinterpret_cast<const void *>(walk_zt_kernel), reinterpret_cast<const void *>(minimize_zt_kernel), nullptr},
+         reinterpret_cast<const void *>(learn_z2_kernel), nullptr},
+        {reinterpret_cast<const void *>(walk_zt_kernel), reinterpret_cast<const void *>(minimize_zt_kernel), nullptr,
+         reinterpret_cast<const void *>(flips_zt_kernel)},
         {reinterpret_cast<const void *>(walk_z2h_kernel), reinterpret_cast<const void *>(minimize_z2h_kernel),
-         reinterpret_cast<const void *>(learn_z2h_kernel)},
+         reinterpret_cast<const void *>(learn_z2h_kernel), nullptr},
     };
-    if (variant < 0 || variant > 2 || mode < 0 || mode > 2) return nullptr;
+    if (variant < 0 || variant > 2 || mode < 0 || mode > 3) return nullptr;
     return k[variant][mode];
 }
 

@@ -102,6 +102,21 @@ FG_API int fg_init_minimizer(fg_ctx *ctx, const int32_t *blob, int64_t len);
  * every current scheme; returns the number of invalid ones or < 0. */
 FG_API int fg_minimize(fg_ctx *ctx);
 
+/* Scheme pool of the additions reducer's --max-flips mode
+ * (SchemeAdditionsReducer::initialize + initializeKernel,
+ * src/entities/scheme_additions_reducer.cu:144-175,433-446): blob holds one
+ * integer-ring scheme (count 1, no reduction); every walker starts from it
+ * and walker 0 keeps it.  Returns -2 if it fails the Brent check. */
+FG_API int fg_init_flips(fg_ctx *ctx, const int32_t *blob, int64_t len);
+
+/* flipSchemesKernel (src/entities/scheme_additions_reducer.cu:448-460): every
+ * walker w >= 1 restarts from walker 0's scheme, draws a count in
+ * [0, max_flips] from its own random stream and applies that many flips with
+ * reductions (tryFlip), stopping at the first that finds no pair; walker 0
+ * is left alone.  Runs the GPU Brent check of every scheme; returns the
+ * number of invalid ones or < 0. */
+FG_API int fg_flip_schemes(fg_ctx *ctx, int max_flips);
+
 /* bestComplexities[] (minimizer pools only; best may be null otherwise) and
  * getComplexity() of every walker's current scheme after the last launch
  * (src/entities/complexity_minimizer.cu:108-128).  Each non-null output

@@ -821,9 +821,34 @@ static void minimize(fgo_walker *w, const fgo_walk_params *p) {
     }
 }
 
+/* flipSchemesKernel (src/entities/scheme_additions_reducer.cu:448-460):
+ * every walker but walker 0 draws a count in [0, iterations] (:454, from
+ * the aux stream) and applies that many flips with reductions (:457-459),
+ * stopping at the first that finds no pair. */
+static void flip_schemes(fgo_walker *w, const fgo_walk_params *p) {
+    if (w->rng.walker == 0) return; /* :451 */
+    fgo_scheme *s = &w->cur;
+    const int count = (int)(fgo_rng_aux32(&w->rng) % (uint32_t)(p->iterations + 1));
+    fgo_walk_params q = *p;
+    q.check_reduce = 1;
+    for (int it = 0; it < count; it++) {
+        uint32_t b[4];
+        fgo_rng_main_block(&w->rng, b);
+        int rank = s->m;
+        int ok = s->ring == FGO_RING_Z2 ? try_flip_z2(s, b, &q) : try_flip_zt(s, b, &w->rng, &q);
+        if (!ok) break;
+        w->total_flips++;
+        w->sum_m += (uint64_t)rank;
+    }
+}
+
 void fgo_walk(fgo_walker *w, const fgo_walk_params *p) {
     if (p->mode == 1) {
         minimize(w, p);
+        return;
+    }
+    if (p->mode == 3) {
+        flip_schemes(w, p);
         return;
     }
     fgo_scheme *s = &w->cur;

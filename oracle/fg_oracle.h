@@ -84,7 +84,8 @@ typedef struct {
     int select_mode;       /* FGO_SELECT_* */
     int check_reduce;      /* tryFlip(checkReduce) */
     int zt_visit_cap;      /* max candidates one integer tryFlip visits (at most 16) */
-    int mode;              /* 0: flip-graph walk; 1: complexity minimizer (best_rank holds the complexity) */
+    int mode;              /* 0: flip-graph walk; 1: complexity minimizer (best_rank holds the complexity);
+                              3: the additions reducer's scheme flips (iterations = max flips) */
 } fgo_walk_params;
 
 typedef struct {

@@ -79,6 +79,7 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                 uint8_t *L = lds.data();
                 if (ring) {
                     if (mode == kModeMinimize) walk_lane<ZTLane, kModeMinimize>(a, L, lane, w, W);
+                    else if (mode == kModeFlips) walk_lane<ZTLane, kModeFlips>(a, L, lane, w, W);
                     else walk_lane<ZTLane, kModeWalk>(a, L, lane, w, W);
                 } else if (half) {
                     if (mode == kModeMinimize) walk_lane<Z2Lane16, kModeMinimize>(a, L, lane, w, W);

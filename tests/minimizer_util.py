@@ -73,3 +73,43 @@ def walked_schemes(n, count, iterations, seed, ring=O.RING_Z2, same_rank=True):
         common = max(set(ranks), key=ranks.count)
         blobs = [b for b in blobs if int(b[3]) == common]
     return blobs
+
+
+def records_from_blob(blob, count, ring=O.RING_ZT):
+    """`count` walker records whose current and best scheme is `blob` (coefficients
+    in {-1, 0, 1}), with fresh random streams: the state fg_init_flips sets up."""
+    n1, n2, n3, m = (int(x) for x in blob[:4])
+    nn = (n1 * n2, n2 * n3, n3 * n1)
+    recs = np.zeros(count, dtype=O.RECORD_DTYPE)
+    recs["n"] = (n1, n2, n3)
+    recs["m"] = m
+    recs["best_m"] = m
+    recs["best_rank"] = m
+    recs["aux_idx"] = 4
+    p = 4
+    for c in range(3):
+        for t in range(m):
+            row = [int(x) for x in blob[p:p + nn[c]]]
+            p += nn[c]
+            v = sum(1 << j for j, x in enumerate(row) if x)
+            s = sum(1 << j for j, x in enumerate(row) if x < 0) if ring == O.RING_ZT else 0
+            recs["cur_v"][:, c, t] = v
+            recs["cur_s"][:, c, t] = s
+            recs["best_v"][:, c, t] = v
+            recs["best_s"][:, c, t] = s
+    return recs
+
+
+def flip_launches(blob, count, max_flips, seed, launches, run):
+    """Oracle-side driver of the additions reducer's scheme flips: before every
+    launch walkers 1.. restart from walker 0's scheme (its best, which it never
+    leaves); `run(recs, params)` performs one flips launch.  Returns the
+    records after each launch."""
+    recs = records_from_blob(blob, count)
+    params = O.make_params(iterations=max_flips, mode=3, zt_visit_cap=16)
+    out = []
+    for _ in range(launches):
+        recs = broadcast(recs, 0, range(1, count))
+        recs = run(recs, params)
+        out.append(recs)
+    return out

@@ -9,6 +9,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import oracle_ffi as O
@@ -84,3 +85,34 @@ def test_reducer_saves_valid_reductions(tmp_path):
             fresh += len(d[name + "_fresh"])
             total += len(d[name + "_fresh"]) + sum(len(e) - 1 for e in d[name])
         assert total == int(mm.group(1)) and fresh == int(mm.group(2))
+
+
+@pytest.mark.gpu
+def test_reducer_with_scheme_flips_saves_valid_schemes(tmp_path):
+    # --max-flips > 0: the reducers read flipped copies of the input
+    # (flipSchemesKernel, scheme_additions_reducer.cu:448-460) and the best is
+    # taken over whole reducers (updateBestTogether, :253-285), so each saved
+    # file is the reduced form of one valid, possibly flipped, scheme
+    from minimizer_util import walked_schemes
+    print("[cli] additions_reducer --max-flips", flush=True)
+    blob = walked_schemes((3, 3, 3), 8, 4000, 3, O.RING_ZT)[0]
+    m = int(blob[3])
+    inp = tmp_path / "s.txt"
+    inp.write_text(" ".join(str(int(x)) for x in blob[:4]) + "\n" + " ".join(str(int(x)) for x in blob[4:]) + "\n")
+    out = tmp_path / "red"
+    r = run("-i", str(inp), "-o", str(out), "--count", "64", "--schemes-count", "8", "--max-flips", "5",
+            "--seed", "4", "--max-no-improvements", "2", timeout=180)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "- schemes count: 8" in r.stdout and "- max flips: 5" in r.stdout
+    assert "No improvements for 2 iterations" in r.stdout
+    files = sorted(glob.glob(str(out / "*.json")))
+    assert files
+    for f in files:
+        d = json.load(open(f))
+        mm = len(d["u"])  # rank of the scheme the best reducer read
+        mats = [_json_forms(d[name], d[name + "_fresh"], (9, 9, mm)[c]) for c, name in enumerate("uvw")]
+        assert len(mats[1]) == mm and mm <= m
+        U = np.array([[fm.get(k + 1, 0) for k in range(9)] for fm in mats[0]])
+        V = np.array([[fm.get(k + 1, 0) for k in range(9)] for fm in mats[1]])
+        W = np.array([[fm.get(t + 1, 0) for t in range(mm)] for fm in mats[2]]).T
+        assert S.brent_ok((3, 3, 3), U, V, W, ring="ZT"), f

@@ -73,4 +73,4 @@ def test_reducer_rejects_invalid_scheme():
     with pytest.raises(fg.FgError):
         fg.Reducer(blob, count=4)
     with pytest.raises(fg.FgError):
-        fg.Reducer(S.to_blob(n, U, V, W), count=4, max_flips=3)
+        fg.Reducer(S.to_blob(n, U, V, W), count=4, schemes_count=0, max_flips=3)
