@@ -31,6 +31,8 @@ RECORD_DTYPE = np.dtype([
     ("cur_s", np.uint64, (3, REC_MAX_M)),
     ("best_v", np.uint64, (3, REC_MAX_M)),
     ("best_s", np.uint64, (3, REC_MAX_M)),
+    ("best_n", np.int32, (3,)),
+    ("reserved", np.int32),
 ])
 
 
@@ -77,6 +79,9 @@ def load_library():
     L.fg_init_minimizer.argtypes = [vp, i32p, ctypes.c_int64]
     L.fg_minimize.argtypes = [vp]
     L.fg_init_flips.argtypes = [vp, i32p, ctypes.c_int64]
+    L.fg_resize.argtypes = [vp, ctypes.c_double]
+    L.fg_snapshot_best.argtypes = [vp, i32p, ctypes.c_int]
+    L.fg_set_best_ranks.argtypes = [vp, i32p]
     L.fg_flip_schemes.argtypes = [vp, ctypes.c_int]
     L.fg_complexities.argtypes = [vp, i32p, i32p]
     L.fg_broadcast_best.argtypes = [vp, ctypes.c_int, i32p, ctypes.c_int]
@@ -211,6 +216,20 @@ class Engine:
         """flipSchemesKernel: walkers >= 1 restart from walker 0's scheme and take
         up to max_flips flips; returns the number of schemes failing the Brent check."""
         return _check(self.L.fg_flip_schemes(self.ctx, int(max_flips)), "fg_flip_schemes")
+
+    def resize(self, probability):
+        """resizeKernel on every walker; returns the number of schemes failing the Brent check."""
+        return _check(self.L.fg_resize(self.ctx, float(probability)), "fg_resize")
+
+    def snapshot_best(self, walkers):
+        """The current scheme of each listed walker becomes its best (updateRanks)."""
+        w = np.ascontiguousarray(walkers, dtype=np.int32)
+        _check(self.L.fg_snapshot_best(self.ctx, w.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(w)),
+               "fg_snapshot_best")
+
+    def set_best_ranks(self, ranks):
+        r = np.ascontiguousarray(ranks, dtype=np.int32)
+        _check(self.L.fg_set_best_ranks(self.ctx, r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "fg_set_best_ranks")
 
     def minimize(self):
         """One minimizer launch; returns the number of current schemes failing the GPU Brent check."""

@@ -5,7 +5,8 @@
 // Extensions (not in the reference): --ring picks Z2 or ZT at run time
 // (the reference fixes it at compile time, src/config.cuh:6);
 // --max-launches and --target-rank give the otherwise endless loop
-// (src/entities/flip_graph.cu:286) a stop condition.
+// (src/entities/flip_graph.cu:286) a stop condition; --max-rank sets the
+// term capacity that resized schemes may grow to.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -32,10 +33,51 @@ struct Run {
     int walkers = 0;
     std::string path;
     std::map<std::string, int> best_by_key;  // n2bestRank
+    std::string main_key;                    // format of -n1 -n2 -n3
 };
 
+// FlipGraph::updateRanks(iteration, true) (src/entities/flip_graph.cu:249-277)
+// after a resize pass: a walker whose rank is below its format's best so far
+// makes its current scheme its best; every walker's target rank becomes its
+// format's best; each format that improved saves the scheme that improved it.
+int update_ranks(Run &r, int iteration) {
+    const int W = r.walkers;
+    std::vector<int32_t> m(W), n((size_t)3 * W), ranks(W);
+    if (fg_summary(r.ctx, m.data(), nullptr, nullptr, nullptr, n.data()) < 0) return -1;
+    std::vector<std::string> keys(W);
+    std::map<std::string, int> best_index;  // n2bestIndex
+    std::vector<int32_t> snap;
+    for (int w = 0; w < W; ++w) {
+        keys[w] = fg::format_key(n[3 * w], n[3 * w + 1], n[3 * w + 2]);
+        auto it = r.best_by_key.find(keys[w]);
+        if (it == r.best_by_key.end() || m[w] < it->second) {
+            r.best_by_key[keys[w]] = m[w];
+            best_index[keys[w]] = w;
+            snap.push_back(w);
+        }
+    }
+    if (fg_snapshot_best(r.ctx, snap.data(), (int)snap.size()) < 0) return -1;
+    for (int w = 0; w < W; ++w) ranks[w] = r.best_by_key[keys[w]];
+    if (fg_set_best_ranks(r.ctx, ranks.data()) < 0) return -1;
+    for (const auto &kv : best_index) {
+        fg::Scheme s;
+        if (!fgcli::fetch_scheme(r.ctx, r.ring, kv.second, 0, &s) || !fgcli::verified(r.ring, s)) {
+            std::cout << "Scheme of walker " << kv.second << " failed verification, not saved" << std::endl;
+            continue;
+        }
+        const std::string p = fgcli::save_path(r.path, r.ring, s, iteration, kv.second);
+        fgcli::make_dirs(r.path);
+        fg::save_json(s, p);
+        std::cout << "Best rank of " << kv.first << " was improved to " << r.best_by_key[kv.first]
+                  << " (known: " << fgcli::known_rank(kv.first, r.ring) << ")! Scheme saved to \"" << p << "\""
+                  << std::endl;
+    }
+    if (!best_index.empty()) std::cout << std::endl;
+    return 0;
+}
+
 // FlipGraph::report (src/entities/flip_graph.cu:303-380).  Returns the best
-// rank seen for the first format key.
+// rank seen for the format of -n1 -n2 -n3 (-1 when no walker has it).
 int report(Run &r, std::chrono::steady_clock::time_point start, int iteration, const std::vector<double> &times,
            int log_period, int count = 3) {
     const int W = r.walkers;
@@ -46,7 +88,6 @@ int report(Run &r, std::chrono::steady_clock::time_point start, int iteration, c
     }
     std::map<std::string, std::vector<int>> by_key;
     for (int w = 0; w < W; ++w) by_key[fg::format_key(n[3 * w], n[3 * w + 1], n[3 * w + 2])].push_back(w);
-    int first_best = -1;
     for (auto &kv : by_key) {
         std::vector<int> &idx = kv.second;
         const int k = std::min<int>(count, (int)idx.size());
@@ -68,7 +109,6 @@ int report(Run &r, std::chrono::steady_clock::time_point start, int iteration, c
                 std::cout << "Best scheme of walker " << w << " failed verification, not saved" << std::endl;
             }
         }
-        if (first_best < 0) first_best = r.best_by_key.count(kv.first) ? r.best_by_key[kv.first] : br[w];
     }
     if (log_period == 0 || iteration % log_period == 0) {
         const double last = times.back();
@@ -106,7 +146,8 @@ int report(Run &r, std::chrono::steady_clock::time_point start, int iteration, c
                   << std::endl
                   << std::endl;
     }
-    return first_best;
+    auto mk = r.best_by_key.find(r.main_key);
+    return mk == r.best_by_key.end() ? -1 : mk->second;
 }
 
 }  // namespace
@@ -132,6 +173,7 @@ int main(int argc, char *argv[]) {
     parser.add("--ring", ArgType::String, "RING", "coefficient ring, Z2 or ZT", "Z2");
     parser.add("--max-launches", ArgType::Natural, "INT", "stop after this many walk launches (0: run forever)", "0");
     parser.add("--target-rank", ArgType::Natural, "INT", "stop once a verified scheme of at most this rank is saved (0: never)", "0");
+    parser.add("--max-rank", ArgType::Natural, "INT", "terms a resized scheme may grow to (0: min(255, max(64, 2 n1 n2 n3)))", "0");
 
     if (!parser.parse(argc, argv)) return 0;
 
@@ -152,6 +194,7 @@ int main(int argc, char *argv[]) {
     const std::string ring_s = parser.get("--ring");
     const int max_launches = std::stoi(parser.get("--max-launches"));
     const int target_rank = std::stoi(parser.get("--target-rank"));
+    int max_rank = std::stoi(parser.get("--max-rank"));
     if (ring_s != "Z2" && ring_s != "ZT") {
         std::cout << "Unknown ring \"" << ring_s << "\" (expected Z2 or ZT)" << std::endl;
         return 0;
@@ -193,10 +236,16 @@ int main(int argc, char *argv[]) {
     std::cout << "  - resize: " << p_resize << std::endl;
     std::cout << "- log period: " << log_period << std::endl;
     std::cout << "- seed: " << seed << std::endl;
-    if (p_resize > 0)
-        std::cout << "- note: resize edges (project/extend/merge/product) are not built; a positive resize probability "
-                     "only randomises the per-launch iteration count, as in the reference"
+    const bool resize = p_resize > 0 && ring == FG_RING_Z2;
+    if (resize) {
+        if (max_rank == 0) max_rank = std::min(255, std::max(64, 2 * n1 * n2 * n3));
+        if (max_rank > 255) max_rank = 255;
+        std::cout << "- resized schemes grow to at most " << max_rank << " terms" << std::endl;
+    } else if (p_resize > 0) {
+        std::cout << "- note: resize edges are built for Z2; with ZT a positive resize probability only randomises "
+                     "the per-launch iteration count"
                   << std::endl;
+    }
 
     fg_config cfg{};
     cfg.ring = ring;
@@ -208,10 +257,15 @@ int main(int argc, char *argv[]) {
     cfg.expand_probability = p_expand;
     cfg.reduce_probability = p_reduce;
     cfg.seed = (uint64_t)seed;
+    if (resize) {
+        cfg.max_rank = max_rank;
+        cfg.word_bits = 32;  // resized factors may exceed 16 coefficients
+    }
     Run r;
     r.ring = ring;
     r.walkers = walkers;
     r.path = path;
+    r.main_key = fg::format_key(n1, n2, n3);
     if (fg_create(&cfg, &r.ctx) < 0) {
         std::cout << "Error: " << fg_last_error() << std::endl;
         return -1;
@@ -265,6 +319,15 @@ int main(int argc, char *argv[]) {
         times.push_back(std::chrono::duration<double>(t2 - t1).count());
         const int best = report(r, start, iteration + 1, times, log_period);
         t1 = std::chrono::steady_clock::now();
+        if (resize) {  // flip_graph.cu:296-299
+            const int rbad = fg_resize(r.ctx, p_resize);
+            if (rbad < 0 || update_ranks(r, iteration) < 0) {
+                std::cout << "Error: " << fg_last_error() << std::endl;
+                fg_destroy(r.ctx);
+                return -1;
+            }
+            if (rbad > 0) std::cout << "invalid (" << rbad << ") schemes (resize)" << std::endl;
+        }
         if (target_rank > 0 && best > 0 && best <= target_rank) {
             std::cout << "Target rank " << target_rank << " reached" << std::endl;
             break;

@@ -14,13 +14,17 @@ constexpr int kVarZ2 = 0, kVarZT = 1, kVarZ2h = 2;
 size_t walk_lds_bytes(int variant, int max_m);
 // Sets the kernel's LDS limit and reports how many workgroups fit per CU.
 // mode: 0 walk, 1 complexity minimizer, 2 walk with proposals and logging (Z2 only),
-// 3 the additions reducer's scheme flips (integer ring only).
+// 3 the additions reducer's scheme flips (integer ring only), 4 resize (Z2 with 32-bit words).
 hipError_t prepare_walk(int variant, int max_m, int mode, int *groups_per_cu);
 // Resets a.queue and launches `grid` persistent workgroups.
 hipError_t launch_walk(int variant, int mode, const WalkArgs &a, int grid, hipStream_t stream);
 // Current scheme of walkers dst[0..count) := best scheme of walker src.
 hipError_t launch_broadcast_best(uint32_t *cur, int32_t *m, const uint32_t *best, const int32_t *best_m, int walkers,
                                  int max_m, int src, const int32_t *dst, int count, hipStream_t stream);
+// Best scheme, rank and format of walkers list[0..count) := their current ones.
+hipError_t launch_snapshot_best(const uint32_t *cur, uint32_t *best, const int32_t *m, int32_t *best_m,
+                                const int32_t *fmt, int32_t *bfmt, int walkers, int max_m, const int32_t *list,
+                                int count, hipStream_t stream);
 hipError_t launch_verify(int ring, const VerifyArgs &v, hipStream_t stream);
 hipError_t launch_fill(int32_t *p, int32_t v, int n, hipStream_t stream);
 // Labels one logged batch of the trajectory ring into the log rows.

@@ -415,7 +415,15 @@ FGK_HD void copy_best(Lane &s, const WalkArgs &A, bool want, int lane, int w) {
         const int src = __builtin_ctzll(mask);
         mask &= mask - 1;
         const int ms = __builtin_amdgcn_readlane(s.m, src);
+        const int f0 = __builtin_amdgcn_readlane(s.n0, src), f1 = __builtin_amdgcn_readlane(s.n1, src),
+                  f2 = __builtin_amdgcn_readlane(s.n2, src);
         const int slot = lane / kLanesPerWalker, src_slot = src / kLanesPerWalker;
+        if (A.bfmt && lane == 0) {
+            int32_t *bf = A.bfmt + 4 * (size_t)(w - slot + src_slot);
+            bf[0] = f0;
+            bf[1] = f1;
+            bf[2] = f2;
+        }
         const typename Lane::Word *Fs = s.F - slot + src_slot;
         uint32_t *dst = A.best + (size_t)(w - slot + src_slot) * 3 * A.max_m;
         for (int c = 0; c < 3; ++c)
@@ -423,6 +431,11 @@ FGK_HD void copy_best(Lane &s, const WalkArgs &A, bool want, int lane, int w) {
     }
 #else
     if (want) {
+        if (A.bfmt) {
+            A.bfmt[4 * (size_t)w] = s.n0;
+            A.bfmt[4 * (size_t)w + 1] = s.n1;
+            A.bfmt[4 * (size_t)w + 2] = s.n2;
+        }
         uint32_t *dst = A.best + (size_t)w * 3 * A.max_m;
         for (int c = 0; c < 3; ++c)
             for (int t = 0; t < s.m; ++t) dst[c * A.max_m + t] = s.f(c, t);
@@ -444,6 +457,12 @@ constexpr int kModeWalk = 0;      // randomWalkKernel
 constexpr int kModeMinimize = 1;  // minimizeKernel
 constexpr int kModeLearn = 2;     // walk + proposed first flip + trajectory logging
 constexpr int kModeFlips = 3;     // flipSchemesKernel of the additions reducer
+constexpr int kModeResize = 4;    // resizeKernel (Z2)
+
+// Resize draw thresholds: a uniform compare u < q is x < floor(q * 2^32).
+constexpr uint32_t kThrHalf = 2147483648u, kThr005 = 214748364u, kThr055 = 2362232012u, kThr085 = 3650722201u;
+constexpr int kMinProjectN = 2;    // src/config.cuh:34
+constexpr int kMaxExtensionN = 16; // src/config.cuh:37
 
 // The walk of one lane (randomWalkKernel body, src/entities/flip_graph.cu:520-573),
 // or in kModeMinimize the complexity minimizer's (minimizeKernel,
@@ -457,7 +476,11 @@ constexpr int kModeFlips = 3;     // flipSchemesKernel of the additions reducer
 // scheme flips (flipSchemesKernel, src/entities/scheme_additions_reducer.cu:448-460):
 // every walker but walker 0 draws a count in [0, A.iterations] from its aux
 // stream and applies that many flips with reductions, stopping at the first
-// that finds no pair.  `lds` is the workgroup's LDS block, `lane`
+// that finds no pair.  kModeResize is resizeKernel
+// (src/entities/flip_graph.cu:575-613): with probability thr_resize / 2^32 a
+// walker swaps two dimensions (half the time), then tries to merge with up
+// to three random walkers' best schemes, else projects, multiplies by a
+// random best scheme, doubles a dimension or extends one.  `lds` is the workgroup's LDS block, `lane`
 // the lane in the workgroup (walker slot lane / 2, flip side lane % 2), `w`
 // the walker index.  The host emulation calls it once per walker with an
 // even lane and does both sides.
@@ -541,6 +564,31 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
                 }
             }
             copy_best(s, A, copy, lane, w);
+        }
+    } else if constexpr (kMode == kModeResize) {
+        if (active && (uint64_t)s.aux32() < A.thr_resize) {  // :579
+            const int R = A.resize_max_rank, C = A.resize_max_coef;
+            if (s.aux32() < kThrHalf) s.swap_size_random();  // :584-585
+            bool merged = false;
+            for (int i = 0; i < 3 && !merged; ++i) {  // :587-592
+                const int x = (int)(s.aux32() % (uint32_t)W);
+                const int32_t *bf = A.bfmt + 4 * (size_t)x;
+                merged = s.try_merge(A.best + (size_t)x * 3 * maxm, maxm, A.best_m[x], bf[0], bf[1], bf[2], R, C);
+            }
+            if (!merged) {  // :594-609
+                const uint32_t u = s.aux32();
+                if (u < kThr005) {
+                    s.try_project();
+                } else if (u < kThr055) {
+                    const int x = (int)(s.aux32() % (uint32_t)W);
+                    const int32_t *bf = A.bfmt + 4 * (size_t)x;
+                    s.try_product_scheme(A.best + (size_t)x * 3 * maxm, maxm, A.best_m[x], bf[0], bf[1], bf[2], R, C);
+                } else if (u < kThr085) {
+                    s.try_product_dim(R, C);
+                } else {
+                    s.try_extend(R, C);
+                }
+            }
         }
     } else if constexpr (kMode == kModeFlips) {
         int count = 0;
@@ -641,6 +689,11 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
         A.total_flips[w] = total_flips;
         A.sum_m[w] = sum_m;
         if (A.cx) A.cx[w] = s.complexity();
+        if constexpr (kMode == kModeResize) {
+            A.fmt[4 * w] = s.n0;
+            A.fmt[4 * w + 1] = s.n1;
+            A.fmt[4 * w + 2] = s.n2;
+        }
         for (int c = 0; c < 3; ++c)
             for (int t = 0; t < s.m; ++t) A.cur[((size_t)c * maxm + t) * W + w] = s.f(c, t);
     }

@@ -71,7 +71,7 @@ struct fg_ctx {
 
     uint32_t *cur = nullptr, *cur_s = nullptr, *best = nullptr, *best_s = nullptr;
     int32_t *m = nullptr, *best_m = nullptr, *best_rank = nullptr, *flips_count = nullptr, *aux_idx = nullptr;
-    int32_t *fmt = nullptr, *err = nullptr, *ok = nullptr;
+    int32_t *fmt = nullptr, *bfmt = nullptr, *err = nullptr, *ok = nullptr;
     uint64_t *it = nullptr, *aux = nullptr, *total_flips = nullptr, *sum_m = nullptr;
     int32_t *item_begin = nullptr, *item_end = nullptr, *queue = nullptr;
     int32_t *cx = nullptr, *dst = nullptr;
@@ -122,12 +122,12 @@ struct fg_ctx {
     }
     void release() {
         drop_graph();
-        void *ptrs[] = {cur, cur_s, best, best_s, m, best_m, best_rank, flips_count, aux_idx,
-                        fmt, err, ok, it, aux, total_flips, sum_m, item_begin, item_end, queue, cx, dst, forced};
+        void *ptrs[] = {cur, cur_s, best, best_s, m, best_m, best_rank, flips_count, aux_idx, fmt, bfmt,
+                        err, ok, it, aux, total_flips, sum_m, item_begin, item_end, queue, cx, dst, forced};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         cur = cur_s = best = best_s = nullptr;
-        m = best_m = best_rank = flips_count = aux_idx = fmt = err = ok = nullptr;
+        m = best_m = best_rank = flips_count = aux_idx = fmt = bfmt = err = ok = nullptr;
         it = aux = total_flips = sum_m = nullptr;
         item_begin = item_end = queue = cx = dst = forced = nullptr;
         log.release();
@@ -309,6 +309,7 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
     HIPCHK(dalloc(&ctx->flips_count, W));
     HIPCHK(dalloc(&ctx->aux_idx, W));
     HIPCHK(dalloc(&ctx->fmt, (size_t)4 * W));
+    HIPCHK(dalloc(&ctx->bfmt, (size_t)4 * W));
     HIPCHK(dalloc(&ctx->err, 1));
     HIPCHK(dalloc(&ctx->ok, W));
     HIPCHK(dalloc(&ctx->it, W));
@@ -332,6 +333,7 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
     if (mode == 0 && ring == FG_RING_Z2) {
         int g2 = 0;
         HIPCHK(fgk::prepare_walk(variant, maxm, 2, &g2));
+        if (variant == fgk::kVarZ2) HIPCHK(fgk::prepare_walk(variant, maxm, 4, &g2));
     }
     if (ctx->groups_per_cu < 1) return fail("init: the walk kernel does not fit on a CU with this scheme size");
     ctx->grid = ctx->cfg.grid > 0 ? ctx->cfg.grid : ctx->groups_per_cu * ctx->cus;
@@ -344,6 +346,7 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
     HIPCHK(hipMemcpy(ctx->flips_count, zeros.data(), W * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->aux_idx, aidx.data(), W * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->fmt, fmt.data(), (size_t)16 * W, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->bfmt, fmt.data(), (size_t)16 * W, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->it, z64.data(), W * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->aux, z64.data(), W * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->total_flips, z64.data(), W * 8, hipMemcpyHostToDevice));
@@ -431,7 +434,7 @@ extern "C" int fg_init_minimizer(fg_ctx *ctx, const int32_t *blob, int64_t len) 
 }
 
 namespace {
-int walk_launch(fg_ctx *ctx, int iterations, bool use_forced);
+int walk_launch(fg_ctx *ctx, int iterations, bool use_forced, double resize = 0.0);
 }
 
 extern "C" int fg_init_flips(fg_ctx *ctx, const int32_t *blob, int64_t len) {
@@ -524,13 +527,37 @@ extern "C" int fg_init_batched(fg_ctx *ctx, int nformats, const int32_t *formats
     return init_pool(ctx, starts, assign, formats[0], formats[1], formats[2]);
 }
 
-namespace {
-int walk_launch(fg_ctx *ctx, int iterations, bool use_forced);
-}
 
 extern "C" int fg_walk(fg_ctx *ctx, int iterations) {
     if (ctx && ctx->flips) return fail("fg_walk: a scheme-flips pool runs fg_flip_schemes");
     return walk_launch(ctx, iterations, false);
+}
+
+extern "C" int fg_resize(fg_ctx *ctx, double probability) {
+    if (!ctx || !ctx->ready) return fail("fg_resize: context not initialised");
+    if (ctx->cfg.ring != FG_RING_Z2 || ctx->variant != fgk::kVarZ2 || ctx->minimize || ctx->flips)
+        return fail("fg_resize: resize edges are built for Z2 walk pools with 32-bit factor words (word_bits 32)");
+    if (!(probability > 0)) return 0;
+    return walk_launch(ctx, 1, false, probability);
+}
+
+extern "C" int fg_snapshot_best(fg_ctx *ctx, const int32_t *walkers, int count) {
+    if (!ctx || !ctx->ready) return fail("fg_snapshot_best: context not initialised");
+    if (count < 0 || count > ctx->W || (count > 0 && !walkers)) return fail("fg_snapshot_best: bad walker list");
+    for (int q = 0; q < count; ++q)
+        if (walkers[q] < 0 || walkers[q] >= ctx->W) return fail("fg_snapshot_best: walker out of range");
+    if (count == 0) return 0;
+    HIPCHK(hipMemcpyAsync(ctx->dst, walkers, (size_t)count * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(fgk::launch_snapshot_best(ctx->cur, ctx->best, ctx->m, ctx->best_m, ctx->fmt, ctx->bfmt, ctx->W, ctx->maxm,
+                                     ctx->dst, count, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+extern "C" int fg_set_best_ranks(fg_ctx *ctx, const int32_t *best_rank) {
+    if (!ctx || !ctx->ready || !best_rank) return fail("fg_set_best_ranks: bad arguments");
+    HIPCHK(hipMemcpy(ctx->best_rank, best_rank, (size_t)ctx->W * 4, hipMemcpyHostToDevice));
+    return 0;
 }
 
 namespace fgrt {
@@ -566,9 +593,9 @@ bool log_view(fg_ctx *ctx, LogView *v) {
 }  // namespace fgrt
 
 namespace {
-int walk_launch(fg_ctx *ctx, int iterations, bool use_forced) {
+int walk_launch(fg_ctx *ctx, int iterations, bool use_forced, double resize) {
     if (!ctx || !ctx->ready) return fail("fg_walk: context not initialised");
-    const bool learn = !ctx->minimize && !ctx->flips && (use_forced || ctx->log.on);
+    const bool learn = resize <= 0 && !ctx->minimize && !ctx->flips && (use_forced || ctx->log.on);
     if (learn && ctx->cfg.ring != FG_RING_Z2) return fail("fg_walk: proposals and trajectory logs are built for Z2 walks");
     fgk::WalkArgs a{};
     a.cur = ctx->cur;
@@ -604,7 +631,15 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced) {
     a.nitems = ctx->nitems;
     a.queue = ctx->queue;
     a.cx = ctx->cx;
-    const int mode = ctx->minimize ? 1 : (ctx->flips ? 3 : (learn ? 2 : 0));
+    const int mode = resize > 0 ? 4 : (ctx->minimize ? 1 : (ctx->flips ? 3 : (learn ? 2 : 0)));
+    a.bfmt = ctx->bfmt;
+    if (resize > 0) {
+        // resizeKernel: MAX_RANK and MAX_MATRIX_ELEMENTS become the pool's
+        // term capacity and the 32-bit factor word
+        a.thr_resize = resize >= 1.0 ? (1ull << 32) : (uint64_t)std::floor(resize * 4294967296.0);
+        a.resize_max_rank = std::min(ctx->maxm, 350);
+        a.resize_max_coef = 32;
+    }
     const int slot = ctx->log.on ? (int)(ctx->log.launches % ctx->log.horizon) : 0;
     if (learn) {
         a.forced = use_forced ? ctx->forced : nullptr;
@@ -638,7 +673,7 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced) {
         if (timed) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         return verify_planes(ctx, ctx->cfg.ring, ctx->cur, ctx->m, ctx->fmt, ctx->W, ctx->maxm, ctx->ok);
     };
-    if (ctx->cfg.use_graph && mode != 2) {
+    if (ctx->cfg.use_graph && mode != 2 && mode != 4) {
         if (ctx->graph_iterations != a.iterations) {
             ctx->drop_graph();
             HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -856,6 +891,8 @@ extern "C" int fg_read_walkers(fg_ctx *ctx, int begin, int count, fg_walker_reco
     HIPCHK(hipMemcpy(fc.data(), ctx->flips_count, W * 4, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(ai.data(), ctx->aux_idx, W * 4, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(fmt.data(), ctx->fmt, (size_t)16 * W, hipMemcpyDeviceToHost));
+    std::vector<int32_t> bfmt((size_t)4 * W);
+    HIPCHK(hipMemcpy(bfmt.data(), ctx->bfmt, (size_t)16 * W, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(it.data(), ctx->it, W * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(aux.data(), ctx->aux, W * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(tf.data(), ctx->total_flips, W * 8, hipMemcpyDeviceToHost));
@@ -865,6 +902,7 @@ extern "C" int fg_read_walkers(fg_ctx *ctx, int begin, int count, fg_walker_reco
         fg_walker_record &r = out[q];
         memset(&r, 0, sizeof r);
         for (int c = 0; c < 3; ++c) r.n[c] = fmt[4 * w + c];
+        for (int c = 0; c < 3; ++c) r.best_n[c] = bfmt[4 * w + c];
         r.m = m[w];
         r.best_m = bm[w];
         r.best_rank = br[w];
@@ -899,10 +937,12 @@ extern "C" int fg_write_walkers(fg_ctx *ctx, int begin, int count, const fg_walk
         const fg_walker_record &r = in[q];
         if (r.m < 0 || r.m > maxm || r.best_m < 0 || r.best_m > maxm)
             return fail("fg_write_walkers: record rank exceeds the pool capacity");
-        if (r.n[0] * r.n[1] * r.n[2] > maxm) return fail("fg_write_walkers: format exceeds the pool capacity");
-        for (int c = 0; c < 3; ++c)
-            if (r.n[c] * r.n[(c + 1) % 3] > max_coefficients(ctx->cfg.ring))
+        for (int c = 0; c < 3; ++c) {
+            const int *bn = r.best_n[0] > 0 ? r.best_n : r.n;
+            if (r.n[c] < 1 || r.n[c] * r.n[(c + 1) % 3] > max_coefficients(ctx->cfg.ring) ||
+                bn[c] * bn[(c + 1) % 3] > max_coefficients(ctx->cfg.ring))
                 return fail("fg_write_walkers: factor wider than the GPU word of this ring");
+        }
     }
     const size_t plane = (size_t)3 * maxm * W;
     std::vector<uint32_t> cur(plane), best(plane);
@@ -916,6 +956,8 @@ extern "C" int fg_write_walkers(fg_ctx *ctx, int begin, int count, const fg_walk
     HIPCHK(hipMemcpy(fc.data(), ctx->flips_count, W * 4, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(ai.data(), ctx->aux_idx, W * 4, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(fmt.data(), ctx->fmt, (size_t)16 * W, hipMemcpyDeviceToHost));
+    std::vector<int32_t> bfmt((size_t)4 * W);
+    HIPCHK(hipMemcpy(bfmt.data(), ctx->bfmt, (size_t)16 * W, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(it.data(), ctx->it, W * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(aux.data(), ctx->aux, W * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(tf.data(), ctx->total_flips, W * 8, hipMemcpyDeviceToHost));
@@ -924,6 +966,7 @@ extern "C" int fg_write_walkers(fg_ctx *ctx, int begin, int count, const fg_walk
         const int w = begin + q;
         const fg_walker_record &r = in[q];
         for (int c = 0; c < 3; ++c) fmt[4 * w + c] = r.n[c];
+        for (int c = 0; c < 3; ++c) bfmt[4 * w + c] = r.best_n[0] > 0 ? r.best_n[c] : r.n[c];
         m[w] = r.m;
         bm[w] = r.best_m;
         br[w] = r.best_rank;
@@ -948,6 +991,7 @@ extern "C" int fg_write_walkers(fg_ctx *ctx, int begin, int count, const fg_walk
     HIPCHK(hipMemcpy(ctx->flips_count, fc.data(), W * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->aux_idx, ai.data(), W * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->fmt, fmt.data(), (size_t)16 * W, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->bfmt, bfmt.data(), (size_t)16 * W, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->it, it.data(), W * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->aux, aux.data(), W * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->total_flips, tf.data(), W * 8, hipMemcpyHostToDevice));

@@ -68,6 +68,12 @@ __global__ void __launch_bounds__(kThreadsPerGroup) flips_zt_kernel(WalkArgs A) 
     walk_persistent<lane::ZTLane, lane::kModeFlips>(A, lds);
 }
 
+// resizeKernel (src/entities/flip_graph.cu:575-613), Z2 with 32-bit words.
+__global__ void __launch_bounds__(kThreadsPerGroup) resize_z2_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2Lane, lane::kModeResize>(A, lds);
+}
+
 // Walk with a proposed first flip and trajectory logging (Z2).
 __global__ void __launch_bounds__(kThreadsPerGroup) learn_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
@@ -95,6 +101,30 @@ __global__ void broadcast_best_kernel(uint32_t *cur, int32_t *m, const uint32_t 
     const int ms = best_m[src];
     if (t < ms) cur[((size_t)c * max_m + t) * walkers + d] = best[((size_t)src * 3 + c) * max_m + t];
     if (c == 0 && t == 0) m[d] = ms;
+}
+
+// schemes[i].copyTo(schemesBest[i]) of updateRanks (src/entities/flip_graph.cu:249-262)
+// for walkers list[0..count): the current scheme, its rank and its format
+// become the walker's best.  One thread per (walker, component, term).
+__global__ void snapshot_best_kernel(const uint32_t *cur, uint32_t *best, const int32_t *m, int32_t *best_m,
+                                     const int32_t *fmt, int32_t *bfmt, int walkers, int max_m, const int32_t *list,
+                                     int count) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long per = 3L * max_m;
+    if (g >= per * count) return;
+    const int q = (int)(g / per);
+    const int c = (int)((g % per) / max_m);
+    const int t = (int)(g % max_m);
+    const int w = list[q];
+    if (w < 0 || w >= walkers) return;
+    const int mw = m[w];
+    if (t < mw) best[((size_t)w * 3 + c) * max_m + t] = cur[((size_t)c * max_m + t) * walkers + w];
+    if (c == 0 && t == 0) {
+        best_m[w] = mw;
+        bfmt[4 * w] = fmt[4 * w];
+        bfmt[4 * w + 1] = fmt[4 * w + 1];
+        bfmt[4 * w + 2] = fmt[4 * w + 2];
+    }
 }
 
 // Labels the flips logged `horizon` launches ago: one thread per log slot;
@@ -204,15 +234,15 @@ size_t walk_lds_bytes(int variant, int max_m) {
 }
 
 static const void *walk_kernel(int variant, int mode) {
-    const void *k[3][4] = {
+    const void *k[3][5] = {
         {reinterpret_cast<const void *>(walk_z2_kernel), reinterpret_cast<const void *>(minimize_z2_kernel),
-         reinterpret_cast<const void *>(learn_z2_kernel), nullptr},
+         reinterpret_cast<const void *>(learn_z2_kernel), nullptr, reinterpret_cast<const void *>(resize_z2_kernel)},
         {reinterpret_cast<const void *>(walk_zt_kernel), reinterpret_cast<const void *>(minimize_zt_kernel), nullptr,
-         reinterpret_cast<const void *>(flips_zt_kernel)},
+         reinterpret_cast<const void *>(flips_zt_kernel), nullptr},
         {reinterpret_cast<const void *>(walk_z2h_kernel), reinterpret_cast<const void *>(minimize_z2h_kernel),
-         reinterpret_cast<const void *>(learn_z2h_kernel), nullptr},
+         reinterpret_cast<const void *>(learn_z2h_kernel), nullptr, nullptr},
     };
-    if (variant < 0 || variant > 2 || mode < 0 || mode > 3) return nullptr;
+    if (variant < 0 || variant > 2 || mode < 0 || mode > 4) return nullptr;
     return k[variant][mode];
 }
 
@@ -242,6 +272,17 @@ hipError_t launch_broadcast_best(uint32_t *cur, int32_t *m, const uint32_t *best
     const int block = 256;
     hipLaunchKernelGGL(broadcast_best_kernel, dim3((unsigned)((threads + block - 1) / block)), dim3(block), 0, stream,
                        cur, m, best, best_m, walkers, max_m, src, dst, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_snapshot_best(const uint32_t *cur, uint32_t *best, const int32_t *m, int32_t *best_m,
+                                const int32_t *fmt, int32_t *bfmt, int walkers, int max_m, const int32_t *list,
+                                int count, hipStream_t stream) {
+    const long threads = 3L * max_m * count;
+    if (threads == 0) return hipSuccess;
+    const int block = 256;
+    hipLaunchKernelGGL(snapshot_best_kernel, dim3((unsigned)((threads + block - 1) / block)), dim3(block), 0, stream,
+                       cur, best, m, best_m, fmt, bfmt, walkers, max_m, list, count);
     return hipGetLastError();
 }
 

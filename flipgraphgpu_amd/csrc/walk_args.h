@@ -31,7 +31,7 @@ struct WalkArgs {
     uint64_t *aux;
     uint64_t *total_flips;
     uint64_t *sum_m;
-    const int32_t *fmt;   // [walkers][4]: n1 n2 n3 unused
+    int32_t *fmt;         // [walkers][4]: n1 n2 n3 unused (written by resize launches)
     int32_t walkers;
     int32_t max_m;
     uint32_t key0;
@@ -61,6 +61,11 @@ struct WalkArgs {
     uint32_t *log_words;    // [walkers / log_every][3][max_m] scheme before that flip
     int32_t *log_meta;      // [walkers / log_every][4]: component (-1: no flip), index1, index2, rank
     int32_t *minm;          // [walkers] lowest rank during the launch
+    // resize launches (Z2, resizeKernel)
+    int32_t *bfmt;            // [walkers][4] format of each walker's best scheme (null: same as fmt)
+    uint64_t thr_resize;      // a walker resizes when its first aux draw is below this (at most 2^32)
+    int32_t resize_max_rank;  // rank limit of a resized scheme (the reference's MAX_RANK)
+    int32_t resize_max_coef;  // coefficients per factor (the reference's MAX_MATRIX_ELEMENTS)
 };
 
 // Error bits written by the walk kernels.

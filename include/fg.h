@@ -63,6 +63,8 @@ typedef struct {
     uint64_t cur_s[3][FG_REC_MAX_M];  /* ZT signs (0 for Z2) */
     uint64_t best_v[3][FG_REC_MAX_M];
     uint64_t best_s[3][FG_REC_MAX_M];
+    int32_t best_n[3];    /* format of the best scheme (0 on write: same as n) */
+    int32_t reserved;
 } fg_walker_record;
 
 FG_API const char *fg_last_error(void);
@@ -150,6 +152,26 @@ FG_API int fg_autotune(fg_ctx *ctx, int iterations, int32_t *grid);
  * GPU Brent check of every walker's current scheme afterwards; returns the
  * number of invalid schemes found (0 expected) or < 0 on error. */
 FG_API int fg_walk(fg_ctx *ctx, int iterations);
+
+/* resizeKernel (src/entities/flip_graph.cu:575-613), Z2 pools with 32-bit
+ * factor words: each walker resizes with the given probability -- half the
+ * time it first swaps two dimensions (swapSize), then it tries to merge with
+ * the best scheme of up to three random walkers (merge), else it projects
+ * out a row (project, 5%), multiplies by a random walker's best scheme
+ * (product of schemes, 50%), doubles a dimension (product, 30%) or extends
+ * one (extend, 15%) (src/schemes/scheme_z2.cu:643-979,1165-1324).  The
+ * reference's MAX_RANK and MAX_MATRIX_ELEMENTS are the pool's term capacity
+ * and 32.  Runs the GPU Brent check of every scheme; returns the number of
+ * invalid ones or < 0. */
+FG_API int fg_resize(fg_ctx *ctx, double probability);
+
+/* schemes[i].copyTo(schemesBest[i]) of FlipGraph::updateRanks
+ * (src/entities/flip_graph.cu:249-262) for the listed walkers: the current
+ * scheme, rank and format become the walker's best. */
+FG_API int fg_snapshot_best(fg_ctx *ctx, const int32_t *walkers, int count);
+
+/* bestRanks[] := best_rank[walkers] (updateRanks, :264-265). */
+FG_API int fg_set_best_ranks(fg_ctx *ctx, const int32_t *best_rank);
 
 /* Best scheme over all walkers (smallest best rank; lowest walker index
  * among ties), as a single-scheme blob (no count prefix). */

@@ -908,6 +908,7 @@ void fgo_walker_to_record(const fgo_walker *w, fgo_walker_record *rec) {
     rec->aux = w->rng.aux;
     rec->total_flips = w->total_flips;
     rec->sum_m = w->sum_m;
+    for (int c = 0; c < 3; c++) rec->best_n[c] = w->best.n[c];
     for (int c = 0; c < 3; c++) {
         for (int t = 0; t < w->cur.m && t < FGO_REC_MAX_M; t++) {
             rec->cur_v[c][t] = w->cur.v[c][t];
@@ -941,7 +942,7 @@ int fgo_walker_from_record(fgo_walker *w, int ring, const fgo_walker_record *rec
     w->log[0] = -1;
     scheme_from_words(&w->cur, ring, rec->n, rec->m, rec->cur_v, rec->cur_s);
     fgo_init_flips(&w->cur);
-    scheme_from_words(&w->best, ring, rec->n, rec->best_m, rec->best_v, rec->best_s);
+    scheme_from_words(&w->best, ring, rec->best_n[0] > 0 ? rec->best_n : rec->n, rec->best_m, rec->best_v, rec->best_s);
     w->best_rank = rec->best_rank;
     w->flips_count = rec->flips_count;
     fgo_rng_init(&w->rng, seed, walker);
@@ -1214,4 +1215,443 @@ int fgo_check_flipset_walk(int ring, int n1, int n2, int n3, uint64_t seed, int 
     }
     free(w); free(s0);
     return bad;
+}
+
+/* ------------------------------------------------------------------ */
+/* resize and meta operators (Z2): src/schemes/scheme_z2.cu:184-310,   */
+/* 643-979,1165-1324 and resizeKernel (src/entities/flip_graph.cu:575-613) */
+/* ------------------------------------------------------------------ */
+
+/* Bit (i, j) of an a x b factor sits at i * b + j. */
+static uint64_t bit_at(uint64_t x, int pos) { return (x >> pos) & 1u; }
+
+/* excludeRow (scheme_z2.cu:205-224): drop row `row` of every factor of
+ * component `matrix` (n[matrix] x n[matrix + 1]). */
+static void exclude_row(fgo_scheme *s, int matrix, int row) {
+    const int a = s->n[matrix], b = s->n[(matrix + 1) % 3];
+    for (int t = 0; t < s->m; t++) {
+        uint64_t x = s->v[matrix][t], y = 0;
+        int ii = 0;
+        for (int i = 0; i < a; i++) {
+            if (i == row) continue;
+            for (int j = 0; j < b; j++) y |= bit_at(x, i * b + j) << (ii * b + j);
+            ii++;
+        }
+        s->v[matrix][t] = y;
+    }
+}
+
+/* excludeColumn (scheme_z2.cu:184-203) */
+static void exclude_column(fgo_scheme *s, int matrix, int column) {
+    const int a = s->n[matrix], b = s->n[(matrix + 1) % 3];
+    for (int t = 0; t < s->m; t++) {
+        uint64_t x = s->v[matrix][t], y = 0;
+        for (int i = 0; i < a; i++) {
+            int jj = 0;
+            for (int j = 0; j < b; j++) {
+                if (j == column) continue;
+                y |= bit_at(x, i * b + j) << (i * (b - 1) + jj);
+                jj++;
+            }
+        }
+        s->v[matrix][t] = y;
+    }
+}
+
+/* addColumn (scheme_z2.cu:226-239): an empty last column; addRow
+ * (:241-254) leaves the bit positions unchanged. */
+static void add_column(fgo_scheme *s, int matrix) {
+    const int a = s->n[matrix], b = s->n[(matrix + 1) % 3];
+    for (int t = 0; t < s->m; t++) {
+        uint64_t x = s->v[matrix][t], y = 0;
+        for (int i = 0; i < a; i++)
+            for (int j = 0; j < b; j++) y |= bit_at(x, i * b + j) << (i * (b + 1) + j);
+        s->v[matrix][t] = y;
+    }
+}
+
+static void set_nn(fgo_scheme *s) {
+    for (int c = 0; c < 3; c++) s->nn[c] = s->n[c] * s->n[(c + 1) % 3];
+}
+
+static void z2_triplet(fgo_scheme *s, uint64_t u, uint64_t v, uint64_t w) {
+    tv a = {u, 0, 1}, b = {v, 0, 1}, c = {w, 0, 1};
+    add_triplet(s, 0, 1, 2, a, b, c);
+}
+
+/* project (scheme_z2.cu:643-653) */
+void fgo_project(fgo_scheme *s, int p, int q) {
+    exclude_row(s, p, q);
+    exclude_column(s, (p + 2) % 3, q);
+    s->n[p]--;
+    set_nn(s);
+    remove_zeroes(s);
+    fgo_init_flips(s);
+}
+
+/* extend (scheme_z2.cu:655-687) */
+void fgo_extend(fgo_scheme *s, int p) {
+    const int n0 = s->n[0], n1 = s->n[1], n2 = s->n[2];
+    if (p == 0) {
+        add_column(s, 2);
+        for (int i = 0; i < n2; i++)
+            for (int j = 0; j < n1; j++)
+                z2_triplet(s, 1ull << (n0 * n1 + j), 1ull << (j * n2 + i), 1ull << (i * (n0 + 1) + n0));
+    } else if (p == 1) {
+        add_column(s, 0);
+        for (int i = 0; i < n0; i++)
+            for (int j = 0; j < n2; j++)
+                z2_triplet(s, 1ull << (i * (n1 + 1) + n1), 1ull << (n1 * n2 + j), 1ull << (j * n0 + i));
+    } else {
+        add_column(s, 1);
+        for (int i = 0; i < n0; i++)
+            for (int j = 0; j < n1; j++)
+                z2_triplet(s, 1ull << (i * n1 + j), 1ull << (j * (n2 + 1) + n2), 1ull << (n2 * n0 + i));
+    }
+    s->n[p]++;
+    set_nn(s);
+    fgo_init_flips(s);
+}
+
+/* Factor x of an a x b block placed at offset (di, dj) of a rows x nb
+ * matrix. */
+static uint64_t embed(uint64_t x, int a, int b, int nb, int di, int dj) {
+    uint64_t y = 0;
+    for (int i = 0; i < a; i++)
+        for (int j = 0; j < b; j++) y |= bit_at(x, i * b + j) << ((i + di) * nb + j + dj);
+    return y;
+}
+
+/* product (scheme_z2.cu:689-747): dimension p doubled, every term copied
+ * into the second diagonal block. */
+void fgo_product_dim(fgo_scheme *s, int p) {
+    int nn_[3] = {s->n[0], s->n[1], s->n[2]}, d[3];
+    nn_[p] *= 2;
+    for (int i = 0; i < 3; i++) d[i] = p == i ? s->n[i] : 0;
+    const int m = s->m;
+    for (int t = 0; t < m; t++) {
+        for (int c = 0; c < 3; c++) {
+            const int c1 = (c + 1) % 3;
+            const uint64_t x = s->v[c][t];
+            s->v[c][t] = embed(x, s->n[c], s->n[c1], nn_[c1], 0, 0);
+            s->v[c][t + m] = embed(x, s->n[c], s->n[c1], nn_[c1], d[c], d[c1]);
+            s->s[c][t + m] = 0;
+            s->ok[c][t + m] = 1;
+        }
+    }
+    for (int c = 0; c < 3; c++) s->n[c] = nn_[c];
+    set_nn(s);
+    s->m = 2 * m;
+    fgo_init_flips(s);
+}
+
+/* merge (scheme_z2.cu:749-817): `o` joins as the second block along p. */
+void fgo_merge(fgo_scheme *s, const fgo_scheme *o, int p) {
+    int nn_[3], d[3];
+    for (int i = 0; i < 3; i++) {
+        nn_[i] = i == p ? s->n[i] + o->n[i] : s->n[i];
+        d[i] = i == p ? s->n[i] : 0;
+    }
+    for (int t = 0; t < s->m; t++)
+        for (int c = 0; c < 3; c++) {
+            const int c1 = (c + 1) % 3;
+            s->v[c][t] = embed(s->v[c][t], s->n[c], s->n[c1], nn_[c1], 0, 0);
+        }
+    for (int t = 0; t < o->m; t++)
+        for (int c = 0; c < 3; c++) {
+            const int c1 = (c + 1) % 3;
+            s->v[c][s->m + t] = embed(o->v[c][t], o->n[c], o->n[c1], nn_[c1], d[c], d[c1]);
+            s->s[c][s->m + t] = 0;
+            s->ok[c][s->m + t] = 1;
+        }
+    for (int c = 0; c < 3; c++) s->n[c] = nn_[c];
+    set_nn(s);
+    s->m += o->m;
+    fgo_init_flips(s);
+}
+
+/* product (scheme_z2.cu:819-862): Kronecker product of two schemes, term
+ * index1 * m2 + index2. */
+void fgo_product_scheme(fgo_scheme *s, const fgo_scheme *o) {
+    fgo_scheme *a = (fgo_scheme *)malloc(sizeof(fgo_scheme));
+    fgo_copy(a, s, 0);
+    for (int c = 0; c < 3; c++) s->n[c] = a->n[c] * o->n[c];
+    set_nn(s);
+    s->m = a->m * o->m;
+    for (int t1 = 0; t1 < a->m; t1++)
+        for (int t2 = 0; t2 < o->m; t2++) {
+            const int t = t1 * o->m + t2;
+            for (int c = 0; c < 3; c++) {
+                const int c1 = (c + 1) % 3;
+                uint64_t y = 0;
+                for (int i = 0; i < a->nn[c]; i++) {
+                    if (!bit_at(a->v[c][t1], i)) continue;
+                    const int r1 = i / a->n[c1], q1 = i % a->n[c1];
+                    for (int j = 0; j < o->nn[c]; j++) {
+                        if (!bit_at(o->v[c][t2], j)) continue;
+                        const int r2 = j / o->n[c1], q2 = j % o->n[c1];
+                        y |= 1ull << ((r1 * o->n[c] + r2) * s->n[c1] + q1 * o->n[c1] + q2);
+                    }
+                }
+                s->v[c][t] = y;
+                s->s[c][t] = 0;
+                s->ok[c][t] = 1;
+            }
+        }
+    free(a);
+    fgo_init_flips(s);
+}
+
+/* swapSize (scheme_z2.cu:916-979): the scheme of the transposed problem
+ * obtained by exchanging dimensions p1 and p2. */
+void fgo_swap_size(fgo_scheme *s, int p1, int p2) {
+    if (p1 == p2) return;
+    if (p1 > p2) { int t = p1; p1 = p2; p2 = t; }
+    int idx[3], nn_[3];
+    if (p1 == 0 && p2 == 1) { idx[0] = 0; idx[1] = 2; idx[2] = 1; }
+    else if (p1 == 0 && p2 == 2) { idx[0] = 1; idx[1] = 0; idx[2] = 2; }
+    else { idx[0] = 2; idx[1] = 1; idx[2] = 0; }
+    for (int i = 0; i < 3; i++) nn_[i] = s->n[(idx[i] + 1) % 3];
+    for (int t = 0; t < s->m; t++) {
+        uint64_t y[3];
+        for (int c = 0; c < 3; c++) {
+            const int a = nn_[c], b = nn_[(c + 1) % 3];
+            const uint64_t x = s->v[idx[c]][t];
+            y[c] = 0;
+            for (int i = 0; i < a; i++)
+                for (int j = 0; j < b; j++) y[c] |= bit_at(x, j * a + i) << (i * b + j);
+        }
+        for (int c = 0; c < 3; c++) s->v[c][t] = y[c];
+    }
+    for (int c = 0; c < 3; c++) s->n[c] = nn_[c];
+    set_nn(s);
+    fgo_init_flips(s);
+}
+
+/* swapBasisRows / swapBasisColumns (scheme_z2.cu:864-914) */
+void fgo_swap_basis_rows(fgo_scheme *s, int i1, int i2) {
+    const int n0 = s->n[0], n1 = s->n[1], n2 = s->n[2];
+    int rows[64];
+    for (int r = 0; r < n0; r++) rows[r] = r;
+    rows[i1] = i2;
+    rows[i2] = i1;
+    for (int t = 0; t < s->m; t++) {
+        uint64_t u = 0, w = 0;
+        for (int i = 0; i < n0; i++)
+            for (int j = 0; j < n1; j++) u |= bit_at(s->v[0][t], rows[i] * n1 + j) << (i * n1 + j);
+        for (int i = 0; i < n2; i++)
+            for (int j = 0; j < n0; j++) w |= bit_at(s->v[2][t], i * n0 + rows[j]) << (i * n0 + j);
+        s->v[0][t] = u;
+        s->v[2][t] = w;
+    }
+}
+
+void fgo_swap_basis_columns(fgo_scheme *s, int j1, int j2) {
+    const int n0 = s->n[0], n1 = s->n[1], n2 = s->n[2];
+    int cols[64];
+    for (int c = 0; c < n2; c++) cols[c] = c;
+    cols[j1] = j2;
+    cols[j2] = j1;
+    for (int t = 0; t < s->m; t++) {
+        uint64_t v = 0, w = 0;
+        for (int i = 0; i < n1; i++)
+            for (int j = 0; j < n2; j++) v |= bit_at(s->v[1][t], i * n2 + cols[j]) << (i * n2 + j);
+        for (int i = 0; i < n2; i++)
+            for (int j = 0; j < n0; j++) w |= bit_at(s->v[2][t], cols[i] * n0 + j) << (i * n0 + j);
+        s->v[1][t] = v;
+        s->v[2][t] = w;
+    }
+}
+
+/* isValidProject / isValidExtension / isValidProduct (scheme_z2.cu:256-294),
+ * with this build's rank capacity and factor width in place of MAX_RANK and
+ * MAX_MATRIX_ELEMENTS. */
+static int valid_project(const fgo_scheme *s, int i) {
+    return s->n[i] - 1 >= FGO_MIN_PROJECT_N && s->n[(i + 1) % 3] >= FGO_MIN_PROJECT_N &&
+           s->n[(i + 2) % 3] >= FGO_MIN_PROJECT_N;
+}
+
+static int valid_sizes(const int nn_[3], const fgo_resize_caps *k) {
+    for (int p = 0; p < 3; p++) {
+        if (nn_[p] * nn_[(p + 1) % 3] > k->max_coef) return 0;
+        if (nn_[p] > FGO_MAX_EXTENSION_N) return 0;
+    }
+    return 1;
+}
+
+static int valid_extension(const fgo_scheme *s, int i, const fgo_resize_caps *k) {
+    int nn_[3] = {s->n[0], s->n[1], s->n[2]};
+    nn_[i] += 1;
+    if (s->m + nn_[(i + 1) % 3] * nn_[(i + 2) % 3] > k->max_rank) return 0;
+    return valid_sizes(nn_, k);
+}
+
+static int valid_product(const fgo_scheme *s, int i, const fgo_resize_caps *k) {
+    int nn_[3] = {s->n[0], s->n[1], s->n[2]};
+    nn_[i] *= 2;
+    if (s->m * 2 > k->max_rank) return 0;
+    return valid_sizes(nn_, k);
+}
+
+static uint32_t aux_below(fgo_rng *r, uint32_t n) { return fgo_rng_aux32(r) % n; }
+
+/* tryProject / tryExtend / tryProduct (scheme_z2.cu:1165-1223) */
+static int try_project(fgo_scheme *s, fgo_rng *r) {
+    int idx[3], size = 0;
+    for (int i = 0; i < 3; i++)
+        if (valid_project(s, i)) idx[size++] = i;
+    if (!size) return 0;
+    const int p = idx[aux_below(r, (uint32_t)size)];
+    const int q = (int)aux_below(r, (uint32_t)s->n[p]);
+    fgo_project(s, p, q);
+    while (try_reduce_mode(s, FGO_SELECT_CANONICAL)) {
+    }
+    return 1;
+}
+
+static int try_extend(fgo_scheme *s, fgo_rng *r, const fgo_resize_caps *k) {
+    int idx[3], size = 0;
+    for (int i = 0; i < 3; i++)
+        if (valid_extension(s, i, k)) idx[size++] = i;
+    if (!size) return 0;
+    fgo_extend(s, idx[aux_below(r, (uint32_t)size)]);
+    while (try_reduce_mode(s, FGO_SELECT_CANONICAL)) {
+    }
+    return 1;
+}
+
+static int try_product_dim(fgo_scheme *s, fgo_rng *r, const fgo_resize_caps *k) {
+    int idx[3], size = 0;
+    for (int i = 0; i < 3; i++)
+        if (valid_product(s, i, k)) idx[size++] = i;
+    if (!size) return 0;
+    fgo_product_dim(s, idx[aux_below(r, (uint32_t)size)]);
+    return 1;
+}
+
+/* tryMerge (scheme_z2.cu:1225-1254) */
+static int try_merge(fgo_scheme *s, const fgo_scheme *o, fgo_rng *r, const fgo_resize_caps *k) {
+    if (s->m + o->m > k->max_rank) return 0;
+    const int e0 = s->n[0] == o->n[0], e1 = s->n[1] == o->n[1], e2 = s->n[2] == o->n[2];
+    const int a0 = s->n[0] + o->n[0], a1 = s->n[1] + o->n[1], a2 = s->n[2] + o->n[2];
+    int p[3], size = 0;
+    if (a0 <= FGO_MAX_EXTENSION_N && a0 * s->n[1] <= k->max_coef && a0 * s->n[2] <= k->max_coef && e1 && e2) p[size++] = 0;
+    if (e0 && a1 <= FGO_MAX_EXTENSION_N && s->n[0] * a1 <= k->max_coef && s->n[2] * a1 <= k->max_coef && e2) p[size++] = 1;
+    if (e0 && e1 && a2 <= FGO_MAX_EXTENSION_N && s->n[0] * a2 <= k->max_coef && s->n[1] * a2 <= k->max_coef) p[size++] = 2;
+    if (!size) return 0;
+    fgo_merge(s, o, p[aux_below(r, (uint32_t)size)]);
+    return 1;
+}
+
+/* tryProduct(scheme) (scheme_z2.cu:1256-1275) */
+static int try_product_scheme(fgo_scheme *s, const fgo_scheme *o, const fgo_resize_caps *k) {
+    if (s->m * o->m > k->max_rank) return 0;
+    int sz[3];
+    for (int i = 0; i < 3; i++) sz[i] = s->n[i] * o->n[i];
+    for (int i = 0; i < 3; i++) {
+        if (sz[i] > FGO_MAX_EXTENSION_N) return 0;
+        if (sz[i] * sz[(i + 1) % 3] > k->max_coef) return 0;
+    }
+    fgo_product_scheme(s, o);
+    return 1;
+}
+
+/* swapSize(state) (scheme_z2.cu:1315-1324); the redraw loop is bounded. */
+static void swap_size_random(fgo_scheme *s, fgo_rng *r) {
+    for (int tries = 0; tries < FGO_PLUS_MAX_TRIES; tries++) {
+        const int p1 = (int)aux_below(r, 3u), p2 = (int)aux_below(r, 3u);
+        if (p1 != p2) {
+            fgo_swap_size(s, p1, p2);
+            return;
+        }
+    }
+}
+
+/* resizeKernel body (src/entities/flip_graph.cu:575-613) for walker w:
+ * every uniform compare u < q becomes aux32() < floor(q * 2^32) and every
+ * curand() % n becomes aux32() % n, all from the walker's aux stream.
+ * bests[i] is walker i's best scheme as the launch left it. */
+void fgo_resize_walker(fgo_walker *w, const fgo_scheme *const *bests, int count, uint64_t thr_resize,
+                       const fgo_resize_caps *k) {
+    fgo_scheme *s = &w->cur;
+    if ((uint64_t)fgo_rng_aux32(&w->rng) >= thr_resize) return; /* :579 */
+    if (fgo_rng_aux32(&w->rng) < FGO_THR_HALF) swap_size_random(s, &w->rng); /* :585-586 */
+    int merged = 0;
+    for (int i = 0; i < 3 && !merged; i++) { /* :588-592 */
+        const int index = (int)aux_below(&w->rng, (uint32_t)count);
+        merged |= try_merge(s, bests[index], &w->rng, k);
+    }
+    if (!merged) { /* :594-609 */
+        const uint32_t x = fgo_rng_aux32(&w->rng);
+        if (x < FGO_THR_005) {
+            try_project(s, &w->rng);
+        } else if (x < FGO_THR_055) {
+            const int index = (int)aux_below(&w->rng, (uint32_t)count);
+            try_product_scheme(s, bests[index], k);
+        } else if (x < FGO_THR_085) {
+            try_product_dim(s, &w->rng, k);
+        } else {
+            try_extend(s, &w->rng, k);
+        }
+    }
+}
+
+/* One resize pass over records (walkers walker_base + q): every walker
+ * reads the best schemes the records hold before the pass. */
+int fgo_resize_records(fgo_walker_record *recs, int count, uint32_t walker_base, uint64_t seed, uint64_t thr_resize,
+                       int max_rank, int max_coef) {
+    fgo_resize_caps k = {max_rank, max_coef};
+    fgo_walker *ws = (fgo_walker *)malloc(sizeof(fgo_walker) * (size_t)count);
+    const fgo_scheme **bests = (const fgo_scheme **)malloc(sizeof(fgo_scheme *) * (size_t)count);
+    if (!ws || !bests) { free(ws); free(bests); return -2; }
+    for (int q = 0; q < count; q++) {
+        if (fgo_walker_from_record(&ws[q], FGO_RING_Z2, &recs[q], seed, walker_base + (uint32_t)q)) {
+            free(ws); free(bests);
+            return -1;
+        }
+        bests[q] = &ws[q].best;
+    }
+    /* the bests are not touched by the pass, so walkers can go in order */
+    fgo_scheme *snap = (fgo_scheme *)malloc(sizeof(fgo_scheme) * (size_t)count);
+    if (!snap) { free(ws); free(bests); return -2; }
+    for (int q = 0; q < count; q++) {
+        fgo_copy(&snap[q], &ws[q].best, 0);
+        bests[q] = &snap[q];
+    }
+    for (int q = 0; q < count; q++) {
+        fgo_resize_walker(&ws[q], bests, count, thr_resize, &k);
+        fgo_walker_to_record(&ws[q], &recs[q]);
+    }
+    free(snap);
+    free(ws);
+    free(bests);
+    return 0;
+}
+
+/* Single operators on blobs (tests): op 0 project(p=a, q=b), 1 extend(a),
+ * 2 product(a), 3 merge(other, a), 4 product(other), 5 swapSize(a, b),
+ * 6 swapBasisRows(a, b), 7 swapBasisColumns(a, b).  Returns the output
+ * blob length. */
+int fgo_blob_resize_op(const int32_t *in, const int32_t *other, int op, int a, int b, int32_t *out) {
+    fgo_scheme *s = (fgo_scheme *)malloc(sizeof(fgo_scheme)), *o = (fgo_scheme *)malloc(sizeof(fgo_scheme));
+    int len = -1;
+    if (!s || !o) goto done;
+    if (fgo_from_blob(s, FGO_RING_Z2, in, 0) != 1) goto done;
+    if (other && fgo_from_blob(o, FGO_RING_Z2, other, 0) != 1) goto done;
+    switch (op) {
+    case 0: fgo_project(s, a, b); break;
+    case 1: fgo_extend(s, a); break;
+    case 2: fgo_product_dim(s, a); break;
+    case 3: fgo_merge(s, o, a); break;
+    case 4: fgo_product_scheme(s, o); break;
+    case 5: fgo_swap_size(s, a, b); break;
+    case 6: fgo_swap_basis_rows(s, a, b); break;
+    case 7: fgo_swap_basis_columns(s, a, b); break;
+    default: goto done;
+    }
+    len = fgo_to_blob(s, out);
+done:
+    free(s);
+    free(o);
+    return len;
 }

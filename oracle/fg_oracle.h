@@ -119,6 +119,8 @@ typedef struct {
     uint64_t cur_s[3][FGO_REC_MAX_M];
     uint64_t best_v[3][FGO_REC_MAX_M];
     uint64_t best_s[3][FGO_REC_MAX_M];
+    int32_t best_n[3]; /* format of the best scheme (0: same as n) */
+    int32_t reserved;
 } fgo_walker_record;
 
 /* ---- RNG ---- */
@@ -182,6 +184,33 @@ int fgo_blob_replay(int ring, const int32_t *in, const int32_t *ops, int nops, i
 int fgo_blob_pairs(int ring, const int32_t *blob, int mode, uint32_t *out, int cap);
 int fgo_check_flipset_walk(int ring, int n1, int n2, int n3, uint64_t seed, int iterations, int walkers,
                            int plus_iterations);
+
+/* Resize and meta operators (Z2).  MAX_RANK and MAX_MATRIX_ELEMENTS of the
+ * reference (src/config.cuh:10-11) are replaced by the pool's rank capacity
+ * and factor width; the other limits are the reference's (:34-37). */
+#define FGO_MIN_PROJECT_N 2
+#define FGO_MAX_EXTENSION_N 16
+#define FGO_THR_HALF 2147483648u   /* floor(0.5 * 2^32) */
+#define FGO_THR_005 214748364u     /* floor(0.05 * 2^32) */
+#define FGO_THR_055 2362232012u    /* floor(0.55 * 2^32) */
+#define FGO_THR_085 3650722201u    /* floor(0.85 * 2^32) */
+typedef struct {
+    int max_rank;
+    int max_coef;
+} fgo_resize_caps;
+void fgo_project(fgo_scheme *s, int p, int q);
+void fgo_extend(fgo_scheme *s, int p);
+void fgo_product_dim(fgo_scheme *s, int p);
+void fgo_merge(fgo_scheme *s, const fgo_scheme *o, int p);
+void fgo_product_scheme(fgo_scheme *s, const fgo_scheme *o);
+void fgo_swap_size(fgo_scheme *s, int p1, int p2);
+void fgo_swap_basis_rows(fgo_scheme *s, int i1, int i2);
+void fgo_swap_basis_columns(fgo_scheme *s, int j1, int j2);
+void fgo_resize_walker(fgo_walker *w, const fgo_scheme *const *bests, int count, uint64_t thr_resize,
+                       const fgo_resize_caps *k);
+int fgo_resize_records(fgo_walker_record *recs, int count, uint32_t walker_base, uint64_t seed, uint64_t thr_resize,
+                       int max_rank, int max_coef);
+int fgo_blob_resize_op(const int32_t *in, const int32_t *other, int op, int a, int b, int32_t *out);
 
 /* Additions reducer (fg_reducer.c). */
 typedef struct fgo_reducer fgo_reducer;

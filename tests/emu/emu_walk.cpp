@@ -19,21 +19,24 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                                    int use_reduce, uint32_t thr_reduce, int use_expand, uint32_t thr_expand,
                                    uint32_t thr_best_eq, int check_reduce, int launches, int visit_cap,
                                    int mode, const int32_t *forced, int32_t *log_meta, int32_t *minm,
-                                   int force_wide, int32_t *err_out) {
+                                   int force_wide, uint64_t thr_resize, int resize_max_rank, int resize_max_coef,
+                                   int32_t *err_out) {
     int maxm = 1;
     for (int q = 0; q < count; ++q) {
         maxm = std::max(maxm, std::max(recs[q].m, recs[q].best_m));
         maxm = std::max(maxm, recs[q].n[0] * recs[q].n[1] * recs[q].n[2]);
     }
+    if (mode == fgk::lane::kModeResize) maxm = std::max(maxm, resize_max_rank);  // room for grown schemes
     const int W = count;
     const size_t plane = (size_t)3 * maxm * W;
     std::vector<uint32_t> cur(plane), best(plane);
-    std::vector<int32_t> m(W), bm(W), br(W), fc(W), ai(W), fmt((size_t)4 * W);
+    std::vector<int32_t> m(W), bm(W), br(W), fc(W), ai(W), fmt((size_t)4 * W), bfmt((size_t)4 * W);
     std::vector<uint64_t> it(W), aux(W), tf(W), sm(W);
     std::vector<int32_t> cx(W);
     for (int w = 0; w < W; ++w) {
         const fg_walker_record &r = recs[w];
         for (int c = 0; c < 3; ++c) fmt[4 * w + c] = r.n[c];
+        for (int c = 0; c < 3; ++c) bfmt[4 * w + c] = r.best_n[0] > 0 ? r.best_n[c] : r.n[c];
         m[w] = r.m; bm[w] = r.best_m; br[w] = r.best_rank; fc[w] = r.flips_count; ai[w] = r.aux_idx;
         it[w] = r.it; aux[w] = r.aux; tf[w] = r.total_flips; sm[w] = r.sum_m;
         for (int c = 0; c < 3; ++c) {
@@ -52,6 +55,10 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
     a.use_expand = use_expand; a.thr_expand = thr_expand; a.thr_best_eq = thr_best_eq;
     a.check_reduce = check_reduce; a.zt_visit_cap = visit_cap; a.err = &err;
     a.cx = cx.data();
+    a.bfmt = bfmt.data();
+    a.thr_resize = thr_resize;
+    a.resize_max_rank = resize_max_rank;
+    a.resize_max_coef = resize_max_coef;
     std::vector<uint32_t> log_words((size_t)W * 3 * maxm);
     std::vector<int32_t> lm((size_t)W * 4, -1), mm(W, 0);
     if (mode == fgk::lane::kModeLearn) {
@@ -65,7 +72,7 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
     int nn_max = 0;
     for (int q = 0; q < count; ++q)
         for (int c = 0; c < 3; ++c) nn_max = std::max(nn_max, recs[q].n[c] * recs[q].n[(c + 1) % 3]);
-    const bool half = !ring && nn_max <= 16 && !force_wide;
+    const bool half = !ring && nn_max <= 16 && !force_wide && mode != fgk::lane::kModeResize;
     std::vector<uint8_t> lds(ring ? fgk::zt_lds_bytes(maxm) : fgk::z2_lds_bytes(maxm, half ? 2 : 4), 0xCD);
     const int G = (W + fgk::kWalkersPerGroup - 1) / fgk::kWalkersPerGroup;
     for (int l = 0; l < launches; ++l) {
@@ -81,6 +88,8 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                     if (mode == kModeMinimize) walk_lane<ZTLane, kModeMinimize>(a, L, lane, w, W);
                     else if (mode == kModeFlips) walk_lane<ZTLane, kModeFlips>(a, L, lane, w, W);
                     else walk_lane<ZTLane, kModeWalk>(a, L, lane, w, W);
+                } else if (mode == kModeResize) {
+                    walk_lane<Z2Lane, kModeResize>(a, L, lane, w, W);
                 } else if (half) {
                     if (mode == kModeMinimize) walk_lane<Z2Lane16, kModeMinimize>(a, L, lane, w, W);
                     else if (mode == kModeLearn) walk_lane<Z2Lane16, kModeLearn>(a, L, lane, w, W);
@@ -100,6 +109,8 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
         std::memset(r.best_v, 0, sizeof r.best_v);
         std::memset(r.cur_s, 0, sizeof r.cur_s);
         std::memset(r.best_s, 0, sizeof r.best_s);
+        for (int c = 0; c < 3; ++c) r.n[c] = fmt[4 * w + c];
+        for (int c = 0; c < 3; ++c) r.best_n[c] = bfmt[4 * w + c];
         r.m = m[w]; r.best_m = bm[w]; r.best_rank = br[w]; r.flips_count = fc[w]; r.aux_idx = ai[w];
         r.it = it[w]; r.aux = aux[w]; r.total_flips = tf[w]; r.sum_m = sm[w];
         for (int c = 0; c < 3; ++c) {

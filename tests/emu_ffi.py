@@ -21,20 +21,23 @@ def lib():
             [ctypes.c_int] * 4 + [ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                  ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
 
 
-def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=False, wide=False):
+def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=False, wide=False, resize=None):
     """Emulated launches; with learn (Z2) the first launch takes the proposed
     pairs `forced` [walkers][4] and the result adds (log_meta, minm).  Z2
     factors of up to 16 coefficients run the 16-bit-word lane code unless
-    wide is set."""
+    wide is set.  resize = (thr_resize, max_rank, max_coef) runs the resize
+    mode instead of the walk."""
     recs = np.ascontiguousarray(recs.copy())
     err = ctypes.c_int32()
     p = params
-    mode = 2 if learn else p.mode
+    mode = 2 if learn else (4 if resize is not None else p.mode)
+    thr, rmax, cmax = resize if resize is not None else (0, 0, 0)
     fz = None if forced is None else np.ascontiguousarray(forced, dtype=np.int32)
     lm = np.zeros((len(recs), 4), dtype=np.int32)
     mm = np.zeros(len(recs), dtype=np.int32)
@@ -42,7 +45,7 @@ def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=Fals
                                 p.random_iterations, p.plus_iterations, p.use_reduce, p.thr_reduce,
                                 p.use_expand, p.thr_expand, p.thr_best_eq, p.check_reduce, launches,
                                 p.zt_visit_cap, mode, None if fz is None else fz.ctypes.data,
-                                lm.ctypes.data, mm.ctypes.data, int(wide), ctypes.byref(err))
+                                lm.ctypes.data, mm.ctypes.data, int(wide), thr, rmax, cmax, ctypes.byref(err))
     assert rc == 0
     if learn:
         return recs, err.value, lm, mm

@@ -80,3 +80,31 @@ def test_flip_graph_input_path_roundtrip(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Start reading 1 schemes" in r.stdout
     assert re.search(r"\|\s+\d+ \|  4x4x4 \|  4x4x4 \|    47 \|   49 \|", r.stdout), r.stdout[-2000:]
+
+
+@pytest.mark.gpu
+def test_flip_graph_resize_edges_save_valid_schemes_of_new_formats(tmp_path):
+    # the reference's default --resize-probability 0.2: after each launch some
+    # walkers change format (resizeKernel) and updateRanks saves every format
+    # whose best rank improved (src/entities/flip_graph.cu:249-277, 296-299)
+    out = tmp_path / "schemes"
+    print("[cli] flip_graph 3x3x3 with resize edges", flush=True)
+    r = run("-n1", "3", "-n2", "3", "-n3", "3", "--schemes", "1024", "--max-iterations", "400", "--max-launches", "4",
+            "--seed", "2", "--path", str(out), timeout=180)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "- resized schemes grow to at most 64 terms" in r.stdout
+    assert "invalid" not in r.stdout
+    improved = re.findall(r"Best rank of (\d+x\d+x\d+) was improved to (\d+) \(known: -?\d+\)!", r.stdout)
+    keys = {k for k, _ in improved}
+    assert len(keys - {"3x3x3"}) >= 2, improved
+    files = glob.glob(str(out / "*.json"))
+    assert files
+    for f in files:
+        d = json.load(open(f))
+        n = tuple(d["n"])
+        name = os.path.basename(f)
+        assert re.fullmatch(r"\d+x\d+x\d+_m\d+_c-?\d+_iteration\d+_run\d+_" + "x".join(map(str, n)) + r"_Z2\.json",
+                            name), name
+        assert name.startswith("x".join(map(str, sorted(n))) + f"_m{d['m']}_")
+        blob = S.to_blob(n, d["u"], d["v"], d["w"])
+        assert O.validate(blob) == 1, name

@@ -37,7 +37,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_record_layout_matches_oracle():
     assert fg.RECORD_DTYPE == O.RECORD_DTYPE
-    assert fg.RECORD_DTYPE.itemsize == 64 + 4 * 3 * 256 * 8
+    assert fg.RECORD_DTYPE.itemsize == 64 + 4 * 3 * 256 * 8 + 16
 
 
 def test_last_error_is_callable_without_gpu():
