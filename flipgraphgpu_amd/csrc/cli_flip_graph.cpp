@@ -210,8 +210,8 @@ int main(int argc, char *argv[]) {
         std::cout << "Error sizes: schemes of more than 255 terms are not supported (now: 255)" << std::endl;
         return 0;
     }
-    if (p_sandwich > 0 || p_basis > 0) {
-        std::cout << "Sandwiching and basis-change edges are not supported by this engine; use probability 0" << std::endl;
+    if ((p_sandwich > 0 || p_basis > 0) && ring != FG_RING_Z2) {
+        std::cout << "Sandwiching and basis-change edges are built for Z2; use probability 0 with ZT" << std::endl;
         return 0;
     }
 
@@ -261,6 +261,9 @@ int main(int argc, char *argv[]) {
         cfg.max_rank = max_rank;
         cfg.word_bits = 32;  // resized factors may exceed 16 coefficients
     }
+    cfg.sandwiching_probability = p_sandwich;
+    cfg.basis_probability = p_basis;
+    if (p_sandwich > 0 || p_basis > 0) cfg.word_bits = 32;
     Run r;
     r.ring = ring;
     r.walkers = walkers;

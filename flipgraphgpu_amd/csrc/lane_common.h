@@ -458,6 +458,7 @@ constexpr int kModeMinimize = 1;  // minimizeKernel
 constexpr int kModeLearn = 2;     // walk + proposed first flip + trajectory logging
 constexpr int kModeFlips = 3;     // flipSchemesKernel of the additions reducer
 constexpr int kModeResize = 4;    // resizeKernel (Z2)
+constexpr int kModeMeta = 5;      // walk with the sandwiching and basis-change edges (Z2)
 
 // Resize draw thresholds: a uniform compare u < q is x < floor(q * 2^32).
 constexpr uint32_t kThrHalf = 2147483648u, kThr005 = 214748364u, kThr055 = 2362232012u, kThr085 = 3650722201u;
@@ -480,7 +481,9 @@ constexpr int kMaxExtensionN = 16; // src/config.cuh:37
 // (src/entities/flip_graph.cu:575-613): with probability thr_resize / 2^32 a
 // walker swaps two dimensions (half the time), then tries to merge with up
 // to three random walkers' best schemes, else projects, multiplies by a
-// random best scheme, doubles a dimension or extends one.  `lds` is the workgroup's LDS block, `lane`
+// random best scheme, doubles a dimension or extends one.  kModeMeta is the
+// walk with the sandwiching and basis-change checks of randomWalkKernel
+// (:558-562), kept out of the plain walk kernel.  `lds` is the workgroup's LDS block, `lane`
 // the lane in the workgroup (walker slot lane / 2, flip side lane % 2), `w`
 // the walker index.  The host emulation calls it once per walker with an
 // even lane and does both sides.
@@ -670,6 +673,10 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
             if (A.use_reduce && s.aux32() < A.thr_reduce) s.try_reduce();  // :550-551
             if (A.use_expand && b3 < A.thr_expand && s.m <= best_m + 2) {   // :553-556
                 if (s.try_expand(1 + (int)(s.aux32() % 2u))) flips_count = 0;
+            }
+            if constexpr (kMode == kModeMeta) {
+                if (A.use_sandwich && s.aux32() < A.thr_sandwich) s.sandwiching();  // :558-559
+                if (A.use_basis && s.aux32() < A.thr_basis) s.swap_basis();         // :561-562
             }
             if (flips_count >= A.plus_iterations && s.try_plus()) flips_count = 0;  // :564-565
         }

@@ -229,10 +229,13 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
     int nn_max = 0;
     for (const auto &s : starts) nn_max = std::max(nn_max, std::max(s.nn[0], std::max(s.nn[1], s.nn[2])));
     int variant = ring ? fgk::kVarZT : fgk::kVarZ2;
+    const bool meta = ctx->cfg.sandwiching_probability > 0 || ctx->cfg.basis_probability > 0;
+    if (meta && ring) return fail("init: sandwiching and basis-change edges are built for Z2");
+    if (meta && ctx->cfg.word_bits == 16) return fail("init: sandwiching and basis-change edges need 32-bit factor words");
     if (!ring && ctx->cfg.word_bits == 16) {
         if (nn_max > 16) return fail("init: 16-bit factor words hold factors of up to 16 coefficients");
         variant = fgk::kVarZ2h;
-    } else if (!ring && nn_max <= 16 && ctx->cfg.word_bits != 32) {
+    } else if (!ring && nn_max <= 16 && ctx->cfg.word_bits != 32 && !meta) {
         const long blocks = (W + fgk::kWalkersPerGroup - 1) / fgk::kWalkersPerGroup;
         const size_t l32 = fgk::walk_lds_bytes(fgk::kVarZ2, maxm), l16 = fgk::walk_lds_bytes(fgk::kVarZ2h, maxm);
         const long fit32 = (long)(ctx->lds_per_cu / l32) * ctx->cus, fit16 = (long)(ctx->lds_per_cu / l16) * ctx->cus;
@@ -334,6 +337,7 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
         int g2 = 0;
         HIPCHK(fgk::prepare_walk(variant, maxm, 2, &g2));
         if (variant == fgk::kVarZ2) HIPCHK(fgk::prepare_walk(variant, maxm, 4, &g2));
+        if (variant == fgk::kVarZ2) HIPCHK(fgk::prepare_walk(variant, maxm, 5, &g2));
     }
     if (ctx->groups_per_cu < 1) return fail("init: the walk kernel does not fit on a CU with this scheme size");
     ctx->grid = ctx->cfg.grid > 0 ? ctx->cfg.grid : ctx->groups_per_cu * ctx->cus;
@@ -631,7 +635,13 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced, double resize) {
     a.nitems = ctx->nitems;
     a.queue = ctx->queue;
     a.cx = ctx->cx;
-    const int mode = resize > 0 ? 4 : (ctx->minimize ? 1 : (ctx->flips ? 3 : (learn ? 2 : 0)));
+    const bool meta = ctx->cfg.sandwiching_probability > 0 || ctx->cfg.basis_probability > 0;
+    if (meta && learn) return fail("fg_walk: proposals and trajectory logs do not combine with basis-change edges");
+    const int mode = resize > 0 ? 4 : (ctx->minimize ? 1 : (ctx->flips ? 3 : (learn ? 2 : (meta ? 5 : 0))));
+    a.use_sandwich = ctx->cfg.sandwiching_probability > 0;
+    a.thr_sandwich = threshold(ctx->cfg.sandwiching_probability, ctx->cfg.max_iterations);
+    a.use_basis = ctx->cfg.basis_probability > 0;
+    a.thr_basis = threshold(ctx->cfg.basis_probability, ctx->cfg.max_iterations);
     a.bfmt = ctx->bfmt;
     if (resize > 0) {
         // resizeKernel: MAX_RANK and MAX_MATRIX_ELEMENTS become the pool's

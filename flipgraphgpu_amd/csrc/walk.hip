@@ -74,6 +74,12 @@ __global__ void __launch_bounds__(kThreadsPerGroup) resize_z2_kernel(WalkArgs A)
     walk_persistent<lane::Z2Lane, lane::kModeResize>(A, lds);
 }
 
+// Walk with the sandwiching and basis-change edges (Z2 with 32-bit words).
+__global__ void __launch_bounds__(kThreadsPerGroup) walk_meta_z2_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2Lane, lane::kModeMeta>(A, lds);
+}
+
 // Walk with a proposed first flip and trajectory logging (Z2).
 __global__ void __launch_bounds__(kThreadsPerGroup) learn_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
@@ -234,15 +240,16 @@ size_t walk_lds_bytes(int variant, int max_m) {
 }
 
 static const void *walk_kernel(int variant, int mode) {
-    const void *k[3][5] = {
+    const void *k[3][6] = {
         {reinterpret_cast<const void *>(walk_z2_kernel), reinterpret_cast<const void *>(minimize_z2_kernel),
-         reinterpret_cast<const void *>(learn_z2_kernel), nullptr, reinterpret_cast<const void *>(resize_z2_kernel)},
+         reinterpret_cast<const void *>(learn_z2_kernel), nullptr, reinterpret_cast<const void *>(resize_z2_kernel),
+         reinterpret_cast<const void *>(walk_meta_z2_kernel)},
         {reinterpret_cast<const void *>(walk_zt_kernel), reinterpret_cast<const void *>(minimize_zt_kernel), nullptr,
-         reinterpret_cast<const void *>(flips_zt_kernel), nullptr},
+         reinterpret_cast<const void *>(flips_zt_kernel), nullptr, nullptr},
         {reinterpret_cast<const void *>(walk_z2h_kernel), reinterpret_cast<const void *>(minimize_z2h_kernel),
-         reinterpret_cast<const void *>(learn_z2h_kernel), nullptr, nullptr},
+         reinterpret_cast<const void *>(learn_z2h_kernel), nullptr, nullptr, nullptr},
     };
-    if (variant < 0 || variant > 2 || mode < 0 || mode > 4) return nullptr;
+    if (variant < 0 || variant > 2 || mode < 0 || mode > 5) return nullptr;
     return k[variant][mode];
 }
 

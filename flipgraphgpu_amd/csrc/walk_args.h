@@ -66,6 +66,11 @@ struct WalkArgs {
     uint64_t thr_resize;      // a walker resizes when its first aux draw is below this (at most 2^32)
     int32_t resize_max_rank;  // rank limit of a resized scheme (the reference's MAX_RANK)
     int32_t resize_max_coef;  // coefficients per factor (the reference's MAX_MATRIX_ELEMENTS)
+    // basis-change edges of the walk (Z2, meta-walk kernel only)
+    int32_t use_sandwich;
+    uint32_t thr_sandwich;
+    int32_t use_basis;
+    uint32_t thr_basis;
 };
 
 // Error bits written by the walk kernels.

@@ -65,6 +65,57 @@ FGK_HD uint32_t rz_kron(uint32_t x, int xa, int xb, uint32_t y, int ya, int yb) 
 }
 
 
+// GF(2) square matrices of up to 16 x 16 as row bitmasks (bit j of r[i] is
+// entry (i, j)), for sandwiching.
+struct Gf2Mat {
+    uint32_t r[16];
+};
+// inverseMatrixZ2 (scheme_z2.cu:444-486): Gauss-Jordan on [a | I]; false
+// when a is singular.
+FGK_HD bool gf2_invert(int n, const Gf2Mat &a, Gf2Mat &inv) {
+    uint32_t aug[16];
+    const uint32_t mask = (1u << n) - 1u;
+    for (int i = 0; i < 16; ++i) aug[i] = i < n ? ((a.r[i] & mask) | (1u << (n + i))) : 0u;
+    for (int col = 0; col < n; ++col) {
+        int piv = -1;
+        for (int row = col; row < n; ++row)
+            if (piv < 0 && ((aug[row] >> col) & 1u)) piv = row;
+        if (piv < 0) return false;
+        const uint32_t t = aug[col];
+        aug[col] = aug[piv];
+        aug[piv] = t;
+        for (int row = 0; row < n; ++row)
+            if (row != col && ((aug[row] >> col) & 1u)) aug[row] ^= aug[col];
+    }
+    for (int i = 0; i < 16; ++i) inv.r[i] = i < n ? (aug[i] >> n) & mask : 0u;
+    return true;
+}
+// L * X for an a x b factor X and an a x a matrix L.
+FGK_HD uint32_t gf2_left(const Gf2Mat &L, uint32_t x, int a, int b) {
+    const uint32_t rm = (1u << b) - 1u;
+    uint32_t y = 0;
+    for (int i = 0; i < a; ++i) {
+        uint32_t row = 0;
+        for (int k = 0; k < a; ++k)
+            if ((L.r[i] >> k) & 1u) row ^= (x >> (k * b)) & rm;
+        y |= row << (i * b);
+    }
+    return y;
+}
+// X * R for an a x b factor X and a b x b matrix R.
+FGK_HD uint32_t gf2_right(uint32_t x, const Gf2Mat &R, int a, int b) {
+    const uint32_t rm = (1u << b) - 1u;
+    uint32_t y = 0;
+    for (int i = 0; i < a; ++i) {
+        const uint32_t xi = (x >> (i * b)) & rm;
+        uint32_t row = 0;
+        for (int k = 0; k < b; ++k)
+            if ((xi >> k) & 1u) row ^= R.r[k] & rm;
+        y |= row << (i * b);
+    }
+    return y;
+}
+
 template <typename WordT>
 struct Z2LaneT : LaneBaseT<WordT> {
     using B = LaneBaseT<WordT>;
@@ -576,6 +627,79 @@ struct Z2LaneT : LaneBaseT<WordT> {
         }
     }
     FGK_HD static int pick3(const int *v, uint32_t q) { return q == 0 ? v[0] : (q == 1 ? v[1] : v[2]); }
+
+    // ---- basis changes (scheme_z2.cu:864-914, 1277-1313) ----
+    // invertibleMatrixZ2 (:488-492): a random n x n matrix (row i = the low n
+    // bits of one aux draw, random.cuh:24-32) until it is invertible; the
+    // redraws are bounded.
+    FGK_HD bool random_invertible(int n, Gf2Mat &a, Gf2Mat &inv) {
+        const uint32_t mask = (1u << n) - 1u;
+        for (int tries = 0; tries < kPlusMaxTries; ++tries) {
+            for (int i = 0; i < 16; ++i) a.r[i] = i < n ? (aux32() & mask) : 0u;
+            if (gf2_invert(n, a, inv)) return true;
+        }
+        return false;
+    }
+    // sandwiching (:1277-1300): U -> A U B^-1, V -> B V C^-1, W -> C W A^-1
+    // for random invertible A (n0), B (n1), C (n2), drawn in that order.  Each
+    // matrix is applied as soon as it is drawn (the products associate), so
+    // only one matrix and its inverse are live.
+    FGK_HD void sandwiching() {
+        Gf2Mat a, ai;
+        if (!random_invertible(n0, a, ai)) return;
+        for (int t = 0; t < m; ++t) {
+            put(0, t, gf2_left(a, f(0, t), n0, n1));
+            put(2, t, gf2_right(f(2, t), ai, n2, n0));
+        }
+        if (!random_invertible(n1, a, ai)) {
+            rebuild();
+            return;
+        }
+        for (int t = 0; t < m; ++t) {
+            put(0, t, gf2_right(f(0, t), ai, n0, n1));
+            put(1, t, gf2_left(a, f(1, t), n1, n2));
+        }
+        if (random_invertible(n2, a, ai)) {
+            for (int t = 0; t < m; ++t) {
+                put(1, t, gf2_right(f(1, t), ai, n1, n2));
+                put(2, t, gf2_left(a, f(2, t), n2, n0));
+            }
+        }
+        rebuild();
+    }
+    // swapBasis (:1302-1313): swaps two rows of A (rows of U, columns of W)
+    // or two columns of B (columns of V, rows of W).  A permutation keeps
+    // equal factors equal and coefficient counts unchanged, so the partner
+    // classes stay as they are, as the reference keeps its FlipSets.
+    FGK_HD void swap_basis() {
+        if (aux32() % 2u) {
+            const int i1 = (int)(aux32() % (uint32_t)n0), i2 = (int)(aux32() % (uint32_t)n0);
+            for (int t = 0; t < m; ++t) {
+                put(0, t, swap_rows(f(0, t), n0, n1, i1, i2));
+                put(2, t, swap_cols(f(2, t), n2, n0, i1, i2));
+            }
+        } else {
+            const int j1 = (int)(aux32() % (uint32_t)n2), j2 = (int)(aux32() % (uint32_t)n2);
+            for (int t = 0; t < m; ++t) {
+                put(1, t, swap_cols(f(1, t), n1, n2, j1, j2));
+                put(2, t, swap_rows(f(2, t), n2, n0, j1, j2));
+            }
+        }
+    }
+    FGK_HD static uint32_t swap_rows(uint32_t x, int a, int b, int i1, int i2) {
+        const uint32_t rm = (1u << b) - 1u;
+        const uint32_t r1 = (x >> (i1 * b)) & rm, r2 = (x >> (i2 * b)) & rm;
+        x &= ~((rm << (i1 * b)) | (rm << (i2 * b)));
+        return x | (r2 << (i1 * b)) | (r1 << (i2 * b));
+    }
+    FGK_HD static uint32_t swap_cols(uint32_t x, int a, int b, int j1, int j2) {
+        uint32_t y = x;
+        for (int i = 0; i < a; ++i) {
+            const uint32_t c1 = (x >> (i * b + j1)) & 1u, c2 = (x >> (i * b + j2)) & 1u;
+            y = (y & ~((1u << (i * b + j1)) | (1u << (i * b + j2)))) | (c2 << (i * b + j1)) | (c1 << (i * b + j2));
+        }
+        return y;
+    }
 
     // A given pair (component c, index1 t1 < index2 t2) flipped with the
     // orientation coins of tryFlip; false when the factors differ.

@@ -45,6 +45,8 @@ typedef struct {
     int32_t use_graph;          /* 1: replay each launch (queue reset, walk, Brent check) as a captured HIP graph */
     int32_t grid;               /* persistent workgroups per launch; 0 = all that fit on the GPU at once */
     int32_t word_bits;          /* Z2 LDS factor words: 16 or 32; 0 = 16 only when it lets more walker blocks run at once */
+    double sandwiching_probability; /* --sandwiching-probability (divided by max iterations; Z2, 32-bit words) */
+    double basis_probability;       /* --basis-probability (divided by max iterations; Z2, 32-bit words) */
 } fg_config;
 
 /* Per-walker state, for tests and checkpointing. */

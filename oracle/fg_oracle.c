@@ -842,6 +842,9 @@ static void flip_schemes(fgo_walker *w, const fgo_walk_params *p) {
     }
 }
 
+static void sandwiching(fgo_scheme *s, fgo_rng *r);
+static void swap_basis(fgo_scheme *s, fgo_rng *r);
+
 void fgo_walk(fgo_walker *w, const fgo_walk_params *p) {
     if (p->mode == 1) {
         minimize(w, p);
@@ -887,6 +890,10 @@ void fgo_walk(fgo_walker *w, const fgo_walk_params *p) {
         if (p->use_reduce && fgo_rng_aux32(&w->rng) < p->thr_reduce) try_reduce_mode(s, p->select_mode); /* :550-551 */
         if (p->use_expand && b[3] < p->thr_expand && s->m <= w->best.m + 2) { /* :553-556 */
             if (try_expand(s, aux_randint(&w->rng, 1, 2), &w->rng)) w->flips_count = 0;
+        }
+        if (s->ring == FGO_RING_Z2) {
+            if (p->use_sandwich && fgo_rng_aux32(&w->rng) < p->thr_sandwich) sandwiching(s, &w->rng); /* :558-559 */
+            if (p->use_basis && fgo_rng_aux32(&w->rng) < p->thr_basis) swap_basis(s, &w->rng);       /* :561-562 */
         }
         if (w->flips_count >= p->plus_iterations && try_plus(s, &w->rng)) w->flips_count = 0; /* :564-565 */
         if (s->m < w->min_m) w->min_m = s->m;
@@ -1654,4 +1661,106 @@ done:
     free(s);
     free(o);
     return len;
+}
+
+/* inverseMatrixZ2 (scheme_z2.cu:444-486) on 0/1 int matrices */
+static int gf2_inverse(int n, const int *a, int *inv) {
+    int aug[16 * 32];
+    const int n2 = 2 * n;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            aug[i * n2 + j] = a[i * n + j];
+            aug[i * n2 + n + j] = i == j;
+        }
+    for (int col = 0; col < n; col++) {
+        int piv = -1;
+        for (int row = col; row < n; row++)
+            if (aug[row * n2 + col]) { piv = row; break; }
+        if (piv < 0) return 0;
+        if (piv != col)
+            for (int i = 0; i < n2; i++) { int t = aug[col * n2 + i]; aug[col * n2 + i] = aug[piv * n2 + i]; aug[piv * n2 + i] = t; }
+        for (int row = 0; row < n; row++)
+            if (row != col && aug[row * n2 + col])
+                for (int j = col; j < n2; j++) aug[row * n2 + j] ^= aug[col * n2 + j];
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) inv[i * n + j] = aug[i * n2 + n + j];
+    return 1;
+}
+
+/* invertibleMatrixZ2 (scheme_z2.cu:488-492) with randomMatrixZ2
+ * (src/utils/random.cuh:24-32): row i is the low n bits of one aux draw;
+ * the redraws are bounded. */
+static int random_invertible(fgo_rng *r, int n, int *a, int *inv) {
+    for (int tries = 0; tries < FGO_PLUS_MAX_TRIES; tries++) {
+        for (int i = 0; i < n; i++) {
+            const uint32_t rnd = fgo_rng_aux32(r);
+            for (int j = 0; j < n; j++) a[i * n + j] = (int)((rnd >> j) & 1u);
+        }
+        if (gf2_inverse(n, a, inv)) return 1;
+    }
+    return 0;
+}
+
+/* matmul (scheme_z2.cu:494-528): left (n1 x n1) . X (n1 x n2) . right (n2 x n2) */
+static uint64_t gf2_matmul(uint64_t x, const int *left, const int *right, int n1, int n2) {
+    int r1[64], r2[64];
+    for (int i = 0; i < n1 * n2; i++) r1[i] = r2[i] = 0;
+    for (int i = 0; i < n1; i++)
+        for (int j = 0; j < n2; j++)
+            for (int k = 0; k < n1; k++) r1[i * n2 + j] ^= left[i * n1 + k] && (int)((x >> (k * n2 + j)) & 1u);
+    for (int i = 0; i < n1; i++)
+        for (int j = 0; j < n2; j++)
+            for (int k = 0; k < n2; k++) r2[i * n2 + j] ^= r1[i * n2 + k] && right[k * n2 + j];
+    uint64_t y = 0;
+    for (int i = 0; i < n1 * n2; i++)
+        if (r2[i]) y |= 1ull << i;
+    return y;
+}
+
+static void identity(int n, int *a) {
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) a[i * n + j] = i == j;
+}
+
+/* sandwiching (scheme_z2.cu:1277-1300): U -> A U B^-1, V -> B V C^-1,
+ * W -> C W A^-1 for random invertible A, B, C drawn in that order; each is
+ * applied once drawn (the products associate), and a matrix whose bounded
+ * redraws all fail ends the edge there. */
+static void sandwiching(fgo_scheme *s, fgo_rng *r) {
+    int a[256], ai[256], i0[256], i1[256], i2[256];
+    const int n0 = s->n[0], n1 = s->n[1], n2 = s->n[2];
+    identity(n0, i0);
+    identity(n1, i1);
+    identity(n2, i2);
+    if (!random_invertible(r, n0, a, ai)) return;
+    for (int t = 0; t < s->m; t++) {
+        s->v[0][t] = gf2_matmul(s->v[0][t], a, i1, n0, n1);
+        s->v[2][t] = gf2_matmul(s->v[2][t], i2, ai, n2, n0);
+    }
+    if (random_invertible(r, n1, a, ai)) {
+        for (int t = 0; t < s->m; t++) {
+            s->v[0][t] = gf2_matmul(s->v[0][t], i0, ai, n0, n1);
+            s->v[1][t] = gf2_matmul(s->v[1][t], a, i2, n1, n2);
+        }
+        if (random_invertible(r, n2, a, ai)) {
+            for (int t = 0; t < s->m; t++) {
+                s->v[1][t] = gf2_matmul(s->v[1][t], i1, ai, n1, n2);
+                s->v[2][t] = gf2_matmul(s->v[2][t], a, i0, n2, n0);
+            }
+        }
+    }
+    fgo_init_flips(s);
+}
+
+/* swapBasis (scheme_z2.cu:1302-1313); the partner lists stay valid, as a
+ * permutation keeps equal factors equal. */
+static void swap_basis(fgo_scheme *s, fgo_rng *r) {
+    if (fgo_rng_aux32(r) % 2u) {
+        const int i1 = (int)(fgo_rng_aux32(r) % (uint32_t)s->n[0]), i2 = (int)(fgo_rng_aux32(r) % (uint32_t)s->n[0]);
+        fgo_swap_basis_rows(s, i1, i2);
+    } else {
+        const int j1 = (int)(fgo_rng_aux32(r) % (uint32_t)s->n[2]), j2 = (int)(fgo_rng_aux32(r) % (uint32_t)s->n[2]);
+        fgo_swap_basis_columns(s, j1, j2);
+    }
 }

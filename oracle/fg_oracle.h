@@ -86,6 +86,10 @@ typedef struct {
     int zt_visit_cap;      /* max candidates one integer tryFlip visits (at most 16) */
     int mode;              /* 0: flip-graph walk; 1: complexity minimizer (best_rank holds the complexity);
                               3: the additions reducer's scheme flips (iterations = max flips) */
+    int use_sandwich;      /* sandwiching edge (Z2): P = thr_sandwich / 2^32 per applied flip */
+    uint32_t thr_sandwich;
+    int use_basis;         /* basis-change edge (Z2) */
+    uint32_t thr_basis;
 } fgo_walk_params;
 
 typedef struct {

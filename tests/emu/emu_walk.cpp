@@ -20,6 +20,7 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                                    uint32_t thr_best_eq, int check_reduce, int launches, int visit_cap,
                                    int mode, const int32_t *forced, int32_t *log_meta, int32_t *minm,
                                    int force_wide, uint64_t thr_resize, int resize_max_rank, int resize_max_coef,
+                                   int use_sandwich, uint32_t thr_sandwich, int use_basis, uint32_t thr_basis,
                                    int32_t *err_out) {
     int maxm = 1;
     for (int q = 0; q < count; ++q) {
@@ -59,6 +60,10 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
     a.thr_resize = thr_resize;
     a.resize_max_rank = resize_max_rank;
     a.resize_max_coef = resize_max_coef;
+    a.use_sandwich = use_sandwich;
+    a.thr_sandwich = thr_sandwich;
+    a.use_basis = use_basis;
+    a.thr_basis = thr_basis;
     std::vector<uint32_t> log_words((size_t)W * 3 * maxm);
     std::vector<int32_t> lm((size_t)W * 4, -1), mm(W, 0);
     if (mode == fgk::lane::kModeLearn) {
@@ -72,7 +77,8 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
     int nn_max = 0;
     for (int q = 0; q < count; ++q)
         for (int c = 0; c < 3; ++c) nn_max = std::max(nn_max, recs[q].n[c] * recs[q].n[(c + 1) % 3]);
-    const bool half = !ring && nn_max <= 16 && !force_wide && mode != fgk::lane::kModeResize;
+    const bool half = !ring && nn_max <= 16 && !force_wide && mode != fgk::lane::kModeResize &&
+                      mode != fgk::lane::kModeMeta;
     std::vector<uint8_t> lds(ring ? fgk::zt_lds_bytes(maxm) : fgk::z2_lds_bytes(maxm, half ? 2 : 4), 0xCD);
     const int G = (W + fgk::kWalkersPerGroup - 1) / fgk::kWalkersPerGroup;
     for (int l = 0; l < launches; ++l) {
@@ -90,6 +96,8 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                     else walk_lane<ZTLane, kModeWalk>(a, L, lane, w, W);
                 } else if (mode == kModeResize) {
                     walk_lane<Z2Lane, kModeResize>(a, L, lane, w, W);
+                } else if (mode == kModeMeta) {
+                    walk_lane<Z2Lane, kModeMeta>(a, L, lane, w, W);
                 } else if (half) {
                     if (mode == kModeMinimize) walk_lane<Z2Lane16, kModeMinimize>(a, L, lane, w, W);
                     else if (mode == kModeLearn) walk_lane<Z2Lane16, kModeLearn>(a, L, lane, w, W);

@@ -22,6 +22,7 @@ def lib():
                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                   ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32,
                                   ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
@@ -37,6 +38,8 @@ def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=Fals
     err = ctypes.c_int32()
     p = params
     mode = 2 if learn else (4 if resize is not None else p.mode)
+    if mode == 0 and (p.use_sandwich or p.use_basis):
+        mode = 5  # the meta-walk kernel
     thr, rmax, cmax = resize if resize is not None else (0, 0, 0)
     fz = None if forced is None else np.ascontiguousarray(forced, dtype=np.int32)
     lm = np.zeros((len(recs), 4), dtype=np.int32)
@@ -45,7 +48,8 @@ def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=Fals
                                 p.random_iterations, p.plus_iterations, p.use_reduce, p.thr_reduce,
                                 p.use_expand, p.thr_expand, p.thr_best_eq, p.check_reduce, launches,
                                 p.zt_visit_cap, mode, None if fz is None else fz.ctypes.data,
-                                lm.ctypes.data, mm.ctypes.data, int(wide), thr, rmax, cmax, ctypes.byref(err))
+                                lm.ctypes.data, mm.ctypes.data, int(wide), thr, rmax, cmax,
+                                p.use_sandwich, p.thr_sandwich, p.use_basis, p.thr_basis, ctypes.byref(err))
     assert rc == 0
     if learn:
         return recs, err.value, lm, mm

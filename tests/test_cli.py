@@ -108,3 +108,18 @@ def test_flip_graph_resize_edges_save_valid_schemes_of_new_formats(tmp_path):
         assert name.startswith("x".join(map(str, sorted(n))) + f"_m{d['m']}_")
         blob = S.to_blob(n, d["u"], d["v"], d["w"])
         assert O.validate(blob) == 1, name
+
+
+@pytest.mark.gpu
+def test_flip_graph_with_basis_change_edges(tmp_path):
+    out = tmp_path / "schemes"
+    print("[cli] flip_graph 2x2x2 with sandwiching and basis edges", flush=True)
+    r = run("-n1", "2", "-n2", "2", "-n3", "2", "--schemes", "256", "--max-iterations", "2000",
+            "--resize-probability", "0", "--sandwiching-probability", "40", "--basis-probability", "40",
+            "--max-launches", "3", "--target-rank", "7", "--seed", "4", "--path", str(out), timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "  - sandwiching: 40" in r.stdout and "  - basis: 40" in r.stdout
+    assert "Best rank of 2x2x2 was improved from 8 to 7" in r.stdout
+    for f in glob.glob(str(out / "*.json")):
+        d = json.load(open(f))
+        assert O.validate(S.to_blob((2, 2, 2), d["u"], d["v"], d["w"])) == 1

@@ -155,3 +155,45 @@ def test_gpu_resize_bit_identical_to_oracle():
         assert np.array_equal(e.read_walkers()["best_rank"], ranks)
     finally:
         e.close()
+
+
+def _meta_params(p_sandwich, p_basis):
+    # per-iteration probabilities of a few percent: the edges fire often
+    return O.make_params(iterations=300, max_iterations=300, plus_iterations=300, p_sandwich=p_sandwich,
+                         p_basis=p_basis)
+
+
+@pytest.mark.parametrize("ps,pb", [(20.0, 0.0), (0.0, 20.0), (10.0, 10.0)])
+def test_sandwiching_and_basis_walk_emulation_matches_oracle(ps, pb):
+    # sandwiching and swapBasis (src/schemes/scheme_z2.cu:1277-1313) keep every
+    # scheme valid: a consistent change of basis of the three matrices
+    seed = 8
+    recs = _walked_records(64, seed, launches=1)
+    p = _meta_params(ps, pb)
+    ora = O.walk_records(recs, p, seed, ring=RING, nthreads=8)
+    emu, err = E.walk(recs, p, seed, wide=True)
+    assert err == 0
+    same_records(emu, ora, f"meta walk {ps} {pb}")
+    for w in range(len(ora)):
+        assert O.record_valid(ora, w, ring=RING) == 1 and O.record_valid(ora, w, best=True, ring=RING) == 1
+    # the edges did fire: the same walk without them diverges
+    plain = O.walk_records(recs, O.make_params(iterations=300, plus_iterations=300), seed, ring=RING, nthreads=8)
+    assert not np.array_equal(plain["aux"], ora["aux"])
+
+
+@pytest.mark.gpu
+def test_gpu_sandwiching_and_basis_walk_bit_identical():
+    import flipgraphgpu_amd as fg
+    seed, W = 8, 256
+    print("[resize] meta walk 3x3x3", flush=True)
+    e = fg.Engine(W, seed, max_iterations=300, plus_iterations=300, sandwiching_probability=10.0,
+                  basis_probability=10.0)
+    try:
+        e.init_naive(3, 3, 3)
+        recs = e.read_walkers()
+        for q in range(2):
+            assert e.walk(300) == 0
+            recs = O.walk_records(recs, _meta_params(10.0, 10.0), seed, ring=RING, nthreads=8)
+            same_records(e.read_walkers(), recs, f"meta launch {q}")
+    finally:
+        e.close()
