@@ -3,12 +3,14 @@
 // __host__ __device__; lanes never communicate, so running lanes one after
 // another on the host reproduces a kernel exactly.
 //
-// An adjacent lane pair owns one walker (the reference's randomWalkKernel runs
-// one thread per scheme, src/entities/flip_graph.cu:520-573).  Both lanes of
+// A lane pair (s, s + 32) owns walker slot s (the reference's randomWalkKernel
+// runs one thread per scheme, src/entities/flip_graph.cu:520-573).  The two
+// lanes sit in different half-waves, so their reads of the walker's LDS
+// column never compete for a bank within a half-wave.  Both lanes of
 // the pair run the same instruction stream on the same walker state; the
 // only work they split is the two halves of a flip (the index1 side with
 // component j, the index2 side with component k), which exchange their
-// results with one lane shuffle.  The walker's scheme lives in LDS for the
+// results with one permlane32 swap.  The walker's scheme lives in LDS for the
 // whole launch as three factor planes F[c][t][slot] (one 32-bit word per
 // factor: a Z2 mask, or for the integer
 // ring the coefficient magnitudes in bits 0-15 and their signs in bits
@@ -381,12 +383,14 @@ FGK_HD void orientation(int c, uint32_t coin, int &j, int &k) {
     }
 }
 
-// The value held by the other lane of this lane's pair (GPU only): a DPP
-// quad permutation [1, 0, 3, 2] swaps adjacent lanes in one VALU op.
-FGK_HD int pair_exchange(int v) {
+// The value held by the other lane of this lane's pair (GPU only): one
+// v_permlane32_swap exchanges the two half-waves.
+FGK_HD int pair_exchange(int v, int side) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false);
+    const auto r = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+    return (int)(side ? r[0] : r[1]);
 #else
+    (void)side;
     return v;
 #endif
 }
@@ -417,7 +421,7 @@ FGK_HD void copy_best(Lane &s, const WalkArgs &A, bool want, int lane, int w) {
         const int ms = __builtin_amdgcn_readlane(s.m, src);
         const int f0 = __builtin_amdgcn_readlane(s.n0, src), f1 = __builtin_amdgcn_readlane(s.n1, src),
                   f2 = __builtin_amdgcn_readlane(s.n2, src);
-        const int slot = lane / kLanesPerWalker, src_slot = src / kLanesPerWalker;
+        const int slot = lane % kWalkersPerGroup, src_slot = src % kWalkersPerGroup;
         if (A.bfmt && lane == 0) {
             int32_t *bf = A.bfmt + 4 * (size_t)(w - slot + src_slot);
             bf[0] = f0;
@@ -498,8 +502,8 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     const int hw = h_words(maxm);
     const int fs = lds_term_stride(maxm, (int)sizeof(Word));
     const size_t fbytes = walk_lds_f_bytes(maxm, (int)sizeof(Word));
-    const int slot = lane / kLanesPerWalker;
-    s.side = lane % kLanesPerWalker;
+    const int slot = lane % kWalkersPerGroup;
+    s.side = lane / kWalkersPerGroup;
     s.F = reinterpret_cast<Word *>(lds) + slot;
     s.H = lds + fbytes + (size_t)slot * 4;
     s.N = lds + fbytes + (size_t)3 * hw * WS * 4 + (size_t)slot * 4;

@@ -25,7 +25,7 @@ __device__ __attribute__((always_inline)) void walk_persistent(const WalkArgs &A
         if (lane == 0) item = atomicAdd(A.queue, 1);
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= A.nitems) break;
-        lane::walk_lane<Lane, kMode>(A, lds, lane, A.item_begin[item] + lane / kLanesPerWalker, A.item_end[item]);
+        lane::walk_lane<Lane, kMode>(A, lds, lane, A.item_begin[item] + lane % kWalkersPerGroup, A.item_end[item]);
     }
 }
 

@@ -7,8 +7,8 @@
 
 namespace fgk {
 
-// One wave64 per workgroup; each walker is served by an adjacent lane pair
-// (lane 2s handles the index1 side of a flip, lane 2s+1 the index2 side), so
+// One wave64 per workgroup; walker slot s is served by the lane pair (s, s + 32)
+// (lane s handles the index1 side of a flip, lane s + 32 the index2 side), so
 // a workgroup holds 32 walkers.
 constexpr int kThreadsPerGroup = 64;
 constexpr int kLanesPerWalker = 2;

@@ -188,7 +188,7 @@ struct ZTLane : LaneBaseT<uint32_t> {
         const int mc = side ? k : j, x = side ? i2 : i1, oc = side ? j : k;
         const uint32_t nv = side ? B : A, ov = side ? fj2 : fk1;
         const Half mine = flip_half(mc, x, nv, i, oc, fi, ov, check_reduce);
-        const int ot = pair_exchange(mine.t), orule = pair_exchange(mine.rule);
+        const int ot = pair_exchange(mine.t, side), orule = pair_exchange(mine.rule, side);
         const int tA = side ? ot : mine.t, rA = side ? orule : mine.rule;
         const int tB = side ? mine.t : ot, rB = side ? mine.rule : orule;
 #else
@@ -209,7 +209,7 @@ struct ZTLane : LaneBaseT<uint32_t> {
         leave(mc, x);
         if (mine.any >= 0) link_join(mc, x, mine.ji);
         f(mc, x) = nv;
-        addS(oc, pair_exchange((int)(Sc(mc) - before)));
+        addS(oc, pair_exchange((int)(Sc(mc) - before), side));
 #else
         leave(j, i1);
         if (ha.any >= 0) link_join(j, i1, ha.ji);

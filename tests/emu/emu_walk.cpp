@@ -86,7 +86,7 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
         for (int g = 0; g < G; ++g)
             for (int slot = 0; slot < fgk::kWalkersPerGroup; ++slot) {
                 // one call per walker: the host side of the lane code does both flip halves
-                const int lane = slot * fgk::kLanesPerWalker;
+                const int lane = slot;  // the side-0 lane of the slot's pair
                 const int w = g * fgk::kWalkersPerGroup + slot;
                 using namespace fgk::lane;
                 uint8_t *L = lds.data();
