@@ -353,6 +353,27 @@ struct LaneBaseT {
         return true;
     }
 
+    // The last term t < m of component c whose factor word equals v, or -1.
+    // Rows are loaded kScanBatch at a time with every load of a batch issued
+    // before the first compare, so a scan of m <= 32 terms waits for one LDS
+    // round trip (a loop over a run-time m would issue its remainder one
+    // load at a time).  Rows from m to the end of a batch are loaded but not
+    // compared; they lie inside the workgroup's LDS block, since the H and N
+    // planes (at least 6 * 8 rows of 32 words) follow the factor planes.
+    static constexpr int kScanBatch = 32;
+    FGK_HD int find_last(int c, uint32_t v) const {
+        const Word *Fc = F + c * fs * WS;
+        int any = -1;
+        for (int base = 0; base < m; base += kScanBatch) {
+            Word w[kScanBatch];
+#pragma unroll
+            for (int e = 0; e < kScanBatch; ++e) w[e] = Fc[(base + e) * WS];
+#pragma unroll
+            for (int e = 0; e < kScanBatch; ++e) any = (base + e < m && (uint32_t)w[e] == v) ? base + e : any;
+        }
+        return any;
+    }
+
     // ---- term edits (callers rebuild the classes afterwards) ----
     FGK_HD void copy_term(int dst, int src) {
         f(0, dst) = f(0, src);

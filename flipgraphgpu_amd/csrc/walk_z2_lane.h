@@ -215,9 +215,7 @@ struct Z2LaneT : LaneBaseT<WordT> {
         hf.any = -1;
         hf.t = 1 << 30;
         hf.rule = 0;
-        const Word *Fc = F + mc * fs * WS;
-#pragma unroll 16
-        for (int t = 0; t < m; ++t) hf.any = Fc[t * WS] == nv ? t : hf.any;
+        hf.any = this->find_last(mc, nv);
         if (hf.any >= 0)
             class_walk(mc, x, hf.any, hf.ji,
                        [&](int y) { return !check_reduce ? 0 : (f(i, y) == fi ? 1 : (f(oc, y) == ov ? 2 : 0)); },

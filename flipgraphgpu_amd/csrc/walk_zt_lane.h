@@ -161,9 +161,7 @@ struct ZTLane : LaneBaseT<uint32_t> {
         hf.any = -1;
         hf.t = 1 << 30;
         hf.rule = 0;
-        const uint32_t *Fc = F + mc * fs * WS;
-#pragma unroll 16
-        for (int t = 0; t < m; ++t) hf.any = Fc[t * WS] == nv ? t : hf.any;
+        hf.any = this->find_last(mc, nv);
         if (hf.any >= 0)
             class_walk(mc, x, hf.any, hf.ji,
                        [&](int y) { return check_reduce ? reduce_rule(i, mc, oc, y, fi, ov) : 0; }, hf.t, hf.rule);
