@@ -210,10 +210,6 @@ int main(int argc, char *argv[]) {
         std::cout << "Error sizes: schemes of more than 255 terms are not supported (now: 255)" << std::endl;
         return 0;
     }
-    if ((p_sandwich > 0 || p_basis > 0) && ring != FG_RING_Z2) {
-        std::cout << "Sandwiching and basis-change edges are built for Z2; use probability 0 with ZT" << std::endl;
-        return 0;
-    }
 
     std::cout << "Config:" << std::endl;
     std::cout << "- working ring: " << fgcli::ring_name(ring) << std::endl;
@@ -236,15 +232,11 @@ int main(int argc, char *argv[]) {
     std::cout << "  - resize: " << p_resize << std::endl;
     std::cout << "- log period: " << log_period << std::endl;
     std::cout << "- seed: " << seed << std::endl;
-    const bool resize = p_resize > 0 && ring == FG_RING_Z2;
+    const bool resize = p_resize > 0;
     if (resize) {
         if (max_rank == 0) max_rank = std::min(255, std::max(64, 2 * n1 * n2 * n3));
         if (max_rank > 255) max_rank = 255;
         std::cout << "- resized schemes grow to at most " << max_rank << " terms" << std::endl;
-    } else if (p_resize > 0) {
-        std::cout << "- note: resize edges are built for Z2; with ZT a positive resize probability only randomises "
-                     "the per-launch iteration count"
-                  << std::endl;
     }
 
     fg_config cfg{};
@@ -259,11 +251,11 @@ int main(int argc, char *argv[]) {
     cfg.seed = (uint64_t)seed;
     if (resize) {
         cfg.max_rank = max_rank;
-        cfg.word_bits = 32;  // resized factors may exceed 16 coefficients
+        if (ring == FG_RING_Z2) cfg.word_bits = 32;  // resized Z2 factors may exceed 16 coefficients
     }
     cfg.sandwiching_probability = p_sandwich;
     cfg.basis_probability = p_basis;
-    if (p_sandwich > 0 || p_basis > 0) cfg.word_bits = 32;
+    if ((p_sandwich > 0 || p_basis > 0) && ring == FG_RING_Z2) cfg.word_bits = 32;
     Run r;
     r.ring = ring;
     r.walkers = walkers;

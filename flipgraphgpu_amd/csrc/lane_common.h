@@ -596,25 +596,25 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     } else if constexpr (kMode == kModeResize) {
         if (active && (uint64_t)s.aux32() < A.thr_resize) {  // :579
             const int R = A.resize_max_rank, C = A.resize_max_coef;
-            if (s.aux32() < kThrHalf) s.swap_size_random();  // :584-585
+            if (s.aux32() < kThrHalf) rz_swap_size_random(s);  // :584-585
             bool merged = false;
             for (int i = 0; i < 3 && !merged; ++i) {  // :587-592
                 const int x = (int)(s.aux32() % (uint32_t)W);
                 const int32_t *bf = A.bfmt + 4 * (size_t)x;
-                merged = s.try_merge(A.best + (size_t)x * 3 * maxm, maxm, A.best_m[x], bf[0], bf[1], bf[2], R, C);
+                merged = rz_try_merge(s, A.best + (size_t)x * 3 * maxm, maxm, A.best_m[x], bf[0], bf[1], bf[2], R, C);
             }
             if (!merged) {  // :594-609
                 const uint32_t u = s.aux32();
                 if (u < kThr005) {
-                    s.try_project();
+                    rz_try_project(s);
                 } else if (u < kThr055) {
                     const int x = (int)(s.aux32() % (uint32_t)W);
                     const int32_t *bf = A.bfmt + 4 * (size_t)x;
-                    s.try_product_scheme(A.best + (size_t)x * 3 * maxm, maxm, A.best_m[x], bf[0], bf[1], bf[2], R, C);
+                    rz_try_product_scheme(s, A.best + (size_t)x * 3 * maxm, maxm, A.best_m[x], bf[0], bf[1], bf[2], R, C);
                 } else if (u < kThr085) {
-                    s.try_product_dim(R, C);
+                    rz_try_product_dim(s, R, C);
                 } else {
-                    s.try_extend(R, C);
+                    rz_try_extend(s, R, C);
                 }
             }
         }
@@ -701,7 +701,7 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
             }
             if constexpr (kMode == kModeMeta) {
                 if (A.use_sandwich && s.aux32() < A.thr_sandwich) s.sandwiching();  // :558-559
-                if (A.use_basis && s.aux32() < A.thr_basis) s.swap_basis();         // :561-562
+                if (A.use_basis && s.aux32() < A.thr_basis) rz_swap_basis(s);       // :561-562
             }
             if (flips_count >= A.plus_iterations && s.try_plus()) flips_count = 0;  // :564-565
         }

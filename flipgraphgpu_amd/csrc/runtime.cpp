@@ -230,8 +230,7 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
     for (const auto &s : starts) nn_max = std::max(nn_max, std::max(s.nn[0], std::max(s.nn[1], s.nn[2])));
     int variant = ring ? fgk::kVarZT : fgk::kVarZ2;
     const bool meta = ctx->cfg.sandwiching_probability > 0 || ctx->cfg.basis_probability > 0;
-    if (meta && ring) return fail("init: sandwiching and basis-change edges are built for Z2");
-    if (meta && ctx->cfg.word_bits == 16) return fail("init: sandwiching and basis-change edges need 32-bit factor words");
+    if (meta && !ring && ctx->cfg.word_bits == 16) return fail("init: sandwiching and basis-change edges need 32-bit factor words");
     if (!ring && ctx->cfg.word_bits == 16) {
         if (nn_max > 16) return fail("init: 16-bit factor words hold factors of up to 16 coefficients");
         variant = fgk::kVarZ2h;
@@ -338,6 +337,11 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
         HIPCHK(fgk::prepare_walk(variant, maxm, 2, &g2));
         if (variant == fgk::kVarZ2) HIPCHK(fgk::prepare_walk(variant, maxm, 4, &g2));
         if (variant == fgk::kVarZ2) HIPCHK(fgk::prepare_walk(variant, maxm, 5, &g2));
+    }
+    if (mode == 0 && ring == FG_RING_ZT) {
+        int g2 = 0;
+        HIPCHK(fgk::prepare_walk(variant, maxm, 4, &g2));
+        HIPCHK(fgk::prepare_walk(variant, maxm, 5, &g2));
     }
     if (ctx->groups_per_cu < 1) return fail("init: the walk kernel does not fit on a CU with this scheme size");
     ctx->grid = ctx->cfg.grid > 0 ? ctx->cfg.grid : ctx->groups_per_cu * ctx->cus;
@@ -539,8 +543,8 @@ extern "C" int fg_walk(fg_ctx *ctx, int iterations) {
 
 extern "C" int fg_resize(fg_ctx *ctx, double probability) {
     if (!ctx || !ctx->ready) return fail("fg_resize: context not initialised");
-    if (ctx->cfg.ring != FG_RING_Z2 || ctx->variant != fgk::kVarZ2 || ctx->minimize || ctx->flips)
-        return fail("fg_resize: resize edges are built for Z2 walk pools with 32-bit factor words (word_bits 32)");
+    if (ctx->variant == fgk::kVarZ2h || ctx->minimize || ctx->flips)
+        return fail("fg_resize: resize edges run on walk pools with 32-bit factor words (word_bits 32 for Z2)");
     if (!(probability > 0)) return 0;
     return walk_launch(ctx, 1, false, probability);
 }
@@ -645,10 +649,10 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced, double resize) {
     a.bfmt = ctx->bfmt;
     if (resize > 0) {
         // resizeKernel: MAX_RANK and MAX_MATRIX_ELEMENTS become the pool's
-        // term capacity and the 32-bit factor word
+        // term capacity and the coefficients a factor word holds
         a.thr_resize = resize >= 1.0 ? (1ull << 32) : (uint64_t)std::floor(resize * 4294967296.0);
         a.resize_max_rank = std::min(ctx->maxm, 350);
-        a.resize_max_coef = 32;
+        a.resize_max_coef = ctx->cfg.ring == FG_RING_ZT ? 16 : 32;
     }
     const int slot = ctx->log.on ? (int)(ctx->log.launches % ctx->log.horizon) : 0;
     if (learn) {

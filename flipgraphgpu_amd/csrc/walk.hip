@@ -8,6 +8,7 @@
 #include "walk_args.h"
 #include "walk_z2_lane.h"
 #include "walk_zt_lane.h"
+#include "resize_ops.h"
 
 namespace fgk {
 
@@ -68,16 +69,27 @@ __global__ void __launch_bounds__(kThreadsPerGroup) flips_zt_kernel(WalkArgs A) 
     walk_persistent<lane::ZTLane, lane::kModeFlips>(A, lds);
 }
 
-// resizeKernel (src/entities/flip_graph.cu:575-613), Z2 with 32-bit words.
+// resizeKernel (src/entities/flip_graph.cu:575-613): Z2 with 32-bit words, and the integer ring.
 __global__ void __launch_bounds__(kThreadsPerGroup) resize_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::Z2Lane, lane::kModeResize>(A, lds);
 }
 
-// Walk with the sandwiching and basis-change edges (Z2 with 32-bit words).
+__global__ void __launch_bounds__(kThreadsPerGroup) resize_zt_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::ZTLane, lane::kModeResize>(A, lds);
+}
+
+// Walk with the sandwiching and basis-change edges: Z2 with 32-bit words, and the
+// integer ring (whose sandwiching is empty in the reference).
 __global__ void __launch_bounds__(kThreadsPerGroup) walk_meta_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::Z2Lane, lane::kModeMeta>(A, lds);
+}
+
+__global__ void __launch_bounds__(kThreadsPerGroup) walk_meta_zt_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::ZTLane, lane::kModeMeta>(A, lds);
 }
 
 // Walk with a proposed first flip and trajectory logging (Z2).
@@ -245,7 +257,8 @@ static const void *walk_kernel(int variant, int mode) {
          reinterpret_cast<const void *>(learn_z2_kernel), nullptr, reinterpret_cast<const void *>(resize_z2_kernel),
          reinterpret_cast<const void *>(walk_meta_z2_kernel)},
         {reinterpret_cast<const void *>(walk_zt_kernel), reinterpret_cast<const void *>(minimize_zt_kernel), nullptr,
-         reinterpret_cast<const void *>(flips_zt_kernel), nullptr, nullptr},
+         reinterpret_cast<const void *>(flips_zt_kernel), reinterpret_cast<const void *>(resize_zt_kernel),
+         reinterpret_cast<const void *>(walk_meta_zt_kernel)},
         {reinterpret_cast<const void *>(walk_z2h_kernel), reinterpret_cast<const void *>(minimize_z2h_kernel),
          reinterpret_cast<const void *>(learn_z2h_kernel), nullptr, nullptr, nullptr},
     };

@@ -8,63 +8,6 @@
 namespace fgk {
 namespace lane {
 
-// ---- bit maps of the resize operators (factor (i, j) of an a x b matrix at
-// bit i * b + j) ----
-FGK_HD uint32_t rz_bit(uint32_t x, int pos) { return (x >> pos) & 1u; }
-// excludeRow (scheme_z2.cu:205-224)
-FGK_HD uint32_t rz_drop_row(uint32_t x, int a, int b, int row) {
-    uint32_t y = 0;
-    int ii = 0;
-    for (int i = 0; i < a; ++i) {
-        if (i == row) continue;
-        for (int j = 0; j < b; ++j) y |= rz_bit(x, i * b + j) << (ii * b + j);
-        ++ii;
-    }
-    return y;
-}
-// excludeColumn (scheme_z2.cu:184-203)
-FGK_HD uint32_t rz_drop_col(uint32_t x, int a, int b, int col) {
-    uint32_t y = 0;
-    for (int i = 0; i < a; ++i) {
-        int jj = 0;
-        for (int j = 0; j < b; ++j) {
-            if (j == col) continue;
-            y |= rz_bit(x, i * b + j) << (i * (b - 1) + jj);
-            ++jj;
-        }
-    }
-    return y;
-}
-// An a x b block placed at (di, dj) of a matrix with nb columns (addColumn,
-// the diagonal blocks of product and merge).
-FGK_HD uint32_t rz_embed(uint32_t x, int a, int b, int nb, int di, int dj) {
-    uint32_t y = 0;
-    for (int i = 0; i < a; ++i)
-        for (int j = 0; j < b; ++j) y |= rz_bit(x, i * b + j) << ((i + di) * nb + j + dj);
-    return y;
-}
-// The a x b factor whose bit (i, j) is bit (j, i) of a b x a factor (swapSize).
-FGK_HD uint32_t rz_transpose(uint32_t x, int a, int b) {
-    uint32_t y = 0;
-    for (int i = 0; i < a; ++i)
-        for (int j = 0; j < b; ++j) y |= rz_bit(x, j * a + i) << (i * b + j);
-    return y;
-}
-// Kronecker product of an xa x xb and a ya x yb factor (product of schemes,
-// scheme_z2.cu:819-862).
-FGK_HD uint32_t rz_kron(uint32_t x, int xa, int xb, uint32_t y, int ya, int yb) {
-    uint32_t z = 0;
-    const int nb = xb * yb;
-    for (int i = 0; i < xa * xb; ++i) {
-        if (!rz_bit(x, i)) continue;
-        const int r1 = i / xb, c1 = i % xb;
-        for (int j = 0; j < ya * yb; ++j)
-            if (rz_bit(y, j)) z |= 1u << ((r1 * ya + j / yb) * nb + c1 * yb + j % yb);
-    }
-    return z;
-}
-
-
 // GF(2) square matrices of up to 16 x 16 as row bitmasks (bit j of r[i] is
 // entry (i, j)), for sandwiching.
 struct Gf2Mat {
@@ -156,6 +99,7 @@ struct Z2LaneT : LaneBaseT<WordT> {
     using B::nn1;
     using B::nn2;
     static constexpr uint32_t kValueMask = 0xFFFFFFFFu;
+    static constexpr bool kSigned = false;  // factor words are plain masks
 
     // removeZeroes (scheme_z2.cu:152-164), including its trailing trim and
     // the missing re-check of a swapped-in term.
@@ -416,217 +360,9 @@ struct Z2LaneT : LaneBaseT<WordT> {
         return true;
     }
 
-    // ---- resize operators (scheme_z2.cu:643-979, 1165-1324) ----
-    FGK_HD int dim(int c) const { return (c == 0 ? n0 : 0) + (c == 1 ? n1 : 0) + (c == 2 ? n2 : 0); }
-    FGK_HD void set_dims(int a, int b, int c) {
-        n0 = a;
-        n1 = b;
-        n2 = c;
-        nn0 = a * b;
-        nn1 = b * c;
-        nn2 = c * a;
-    }
-    FGK_HD void set_dim(int p, int v) { set_dims(p == 0 ? v : n0, p == 1 ? v : n1, p == 2 ? v : n2); }
     FGK_HD void put(int c, int t, uint32_t v) { f(c, t) = (Word)v; }
 
-    // project (scheme_z2.cu:643-653): row q of dimension p goes
-    FGK_HD void project(int p, int q) {
-        const int pc = (p + 2) % 3, a = dim(p), b = dim((p + 1) % 3), c = dim(pc);
-        for (int t = 0; t < m; ++t) {
-            put(p, t, rz_drop_row(f(p, t), a, b, q));
-            put(pc, t, rz_drop_col(f(pc, t), c, a, q));
-        }
-        set_dim(p, a - 1);
-        remove_zeroes();
-        rebuild();
-    }
-    // extend (scheme_z2.cu:655-687)
-    FGK_HD void extend(int p) {
-        const int a = n0, b = n1, c = n2;
-        const int mc = (p + 2) % 3;  // the component that gains a column
-        for (int t = 0; t < m; ++t) put(mc, t, rz_embed(f(mc, t), dim(mc), dim(p), dim(p) + 1, 0, 0));
-        if (p == 0) {
-            for (int i = 0; i < c; ++i)
-                for (int j = 0; j < b; ++j) add_term(1u << (a * b + j), 1u << (j * c + i), 1u << (i * (a + 1) + a));
-        } else if (p == 1) {
-            for (int i = 0; i < a; ++i)
-                for (int j = 0; j < c; ++j) add_term(1u << (i * (b + 1) + b), 1u << (b * c + j), 1u << (j * a + i));
-        } else {
-            for (int i = 0; i < a; ++i)
-                for (int j = 0; j < b; ++j) add_term(1u << (i * b + j), 1u << (j * (c + 1) + c), 1u << (c * a + i));
-        }
-        set_dim(p, dim(p) + 1);
-        rebuild();
-    }
-    FGK_HD void add_term(uint32_t u, uint32_t v, uint32_t w) {
-        put(0, m, u);
-        put(1, m, v);
-        put(2, m, w);
-        ++m;
-    }
-    // product (scheme_z2.cu:689-747): dimension p doubled, the terms repeated
-    // in the second diagonal block
-    FGK_HD void product_dim(int p) {
-        const int d0 = p == 0 ? n0 : 0, d1 = p == 1 ? n1 : 0, d2 = p == 2 ? n2 : 0;
-        const int a = n0, b = n1, c = n2;
-        const int A2 = a + d0, B2 = b + d1, C2 = c + d2;
-        const int m0 = m;
-        for (int t = 0; t < m0; ++t) {
-            const uint32_t u = f(0, t), v = f(1, t), w = f(2, t);
-            put(0, t, rz_embed(u, a, b, B2, 0, 0));
-            put(1, t, rz_embed(v, b, c, C2, 0, 0));
-            put(2, t, rz_embed(w, c, a, A2, 0, 0));
-            put(0, t + m0, rz_embed(u, a, b, B2, d0, d1));
-            put(1, t + m0, rz_embed(v, b, c, C2, d1, d2));
-            put(2, t + m0, rz_embed(w, c, a, A2, d2, d0));
-        }
-        m = 2 * m0;
-        set_dims(A2, B2, C2);
-        rebuild();
-    }
-    // merge (scheme_z2.cu:749-817): scheme o (words o[c * stride + t], format
-    // o0 x o1 x o2, rank om) joins as the second block along p
-    FGK_HD void merge(const uint32_t *o, int stride, int om, int o0, int o1, int o2, int p) {
-        const int d0 = p == 0 ? n0 : 0, d1 = p == 1 ? n1 : 0, d2 = p == 2 ? n2 : 0;
-        const int a = n0, b = n1, c = n2;
-        const int A2 = a + (p == 0 ? o0 : 0), B2 = b + (p == 1 ? o1 : 0), C2 = c + (p == 2 ? o2 : 0);
-        for (int t = 0; t < m; ++t) {
-            put(0, t, rz_embed(f(0, t), a, b, B2, 0, 0));
-            put(1, t, rz_embed(f(1, t), b, c, C2, 0, 0));
-            put(2, t, rz_embed(f(2, t), c, a, A2, 0, 0));
-        }
-        for (int t = 0; t < om; ++t) {
-            put(0, m + t, rz_embed(o[t], o0, o1, B2, d0, d1));
-            put(1, m + t, rz_embed(o[stride + t], o1, o2, C2, d1, d2));
-            put(2, m + t, rz_embed(o[2 * stride + t], o2, o0, A2, d2, d0));
-        }
-        m += om;
-        set_dims(A2, B2, C2);
-        rebuild();
-    }
-    // product of schemes (scheme_z2.cu:819-862): term t1 * om + t2.  Terms are
-    // rewritten from the last one down, so a term is read before any of its
-    // replacements (positions >= t1 * om) can land on an unread term.
-    FGK_HD void product_scheme(const uint32_t *o, int stride, int om, int o0, int o1, int o2) {
-        const int a = n0, b = n1, c = n2;
-        for (int t1 = m - 1; t1 >= 0; --t1) {
-            const uint32_t u = f(0, t1), v = f(1, t1), w = f(2, t1);
-            for (int t2 = 0; t2 < om; ++t2) {
-                const int t = t1 * om + t2;
-                put(0, t, rz_kron(u, a, b, o[t2], o0, o1));
-                put(1, t, rz_kron(v, b, c, o[stride + t2], o1, o2));
-                put(2, t, rz_kron(w, c, a, o[2 * stride + t2], o2, o0));
-            }
-        }
-        m *= om;
-        set_dims(a * o0, b * o1, c * o2);
-        rebuild();
-    }
-    // swapSize (scheme_z2.cu:916-979)
-    FGK_HD void swap_size(int p1, int p2) {
-        if (p1 == p2) return;
-        if (p1 > p2) {
-            const int t = p1;
-            p1 = p2;
-            p2 = t;
-        }
-        // source component of each new component
-        const int i0 = (p1 == 0 && p2 == 1) ? 0 : ((p1 == 0 && p2 == 2) ? 1 : 2);
-        const int i1 = (p1 == 0 && p2 == 1) ? 2 : ((p1 == 0 && p2 == 2) ? 0 : 1);
-        const int i2 = (p1 == 0 && p2 == 1) ? 1 : ((p1 == 0 && p2 == 2) ? 2 : 0);
-        const int a = dim((i0 + 1) % 3), b = dim((i1 + 1) % 3), c = dim((i2 + 1) % 3);
-        for (int t = 0; t < m; ++t) {
-            const uint32_t x0 = f(i0, t), x1 = f(i1, t), x2 = f(i2, t);
-            put(0, t, rz_transpose(x0, a, b));
-            put(1, t, rz_transpose(x1, b, c));
-            put(2, t, rz_transpose(x2, c, a));
-        }
-        set_dims(a, b, c);
-        rebuild();
-    }
-
-    // isValidProject / isValidExtension / isValidProduct / tryMerge's and
-    // tryProduct's size rules (scheme_z2.cu:256-310, 1225-1275) with the
-    // pool's rank limit and factor width for MAX_RANK / MAX_MATRIX_ELEMENTS
-    FGK_HD bool valid_project(int i) const {
-        return dim(i) - 1 >= kMinProjectN && dim((i + 1) % 3) >= kMinProjectN && dim((i + 2) % 3) >= kMinProjectN;
-    }
-    FGK_HD static bool valid_sizes(int a, int b, int c, int max_coef) {
-        return a * b <= max_coef && b * c <= max_coef && c * a <= max_coef && a <= kMaxExtensionN &&
-               b <= kMaxExtensionN && c <= kMaxExtensionN;
-    }
-    FGK_HD bool valid_extension(int i, int max_rank, int max_coef) const {
-        const int a = n0 + (i == 0), b = n1 + (i == 1), c = n2 + (i == 2);
-        const int add = i == 0 ? b * c : (i == 1 ? a * c : a * b);
-        return m + add <= max_rank && valid_sizes(a, b, c, max_coef);
-    }
-    FGK_HD bool valid_product(int i, int max_rank, int max_coef) const {
-        const int a = n0 * (i == 0 ? 2 : 1), b = n1 * (i == 1 ? 2 : 1), c = n2 * (i == 2 ? 2 : 1);
-        return 2 * m <= max_rank && valid_sizes(a, b, c, max_coef);
-    }
-
-    FGK_HD bool try_project() {  // scheme_z2.cu:1165-1185
-        int idx[3], size = 0;
-        for (int i = 0; i < 3; ++i)
-            if (valid_project(i)) idx[size++] = i;
-        if (!size) return false;
-        const int p = pick3(idx, aux32() % (uint32_t)size);
-        const int q = (int)(aux32() % (uint32_t)dim(p));
-        project(p, q);
-        while (try_reduce()) {
-        }
-        return true;
-    }
-    FGK_HD bool try_extend(int max_rank, int max_coef) {  // :1187-1206
-        int idx[3], size = 0;
-        for (int i = 0; i < 3; ++i)
-            if (valid_extension(i, max_rank, max_coef)) idx[size++] = i;
-        if (!size) return false;
-        extend(pick3(idx, aux32() % (uint32_t)size));
-        while (try_reduce()) {
-        }
-        return true;
-    }
-    FGK_HD bool try_product_dim(int max_rank, int max_coef) {  // :1208-1223
-        int idx[3], size = 0;
-        for (int i = 0; i < 3; ++i)
-            if (valid_product(i, max_rank, max_coef)) idx[size++] = i;
-        if (!size) return false;
-        product_dim(pick3(idx, aux32() % (uint32_t)size));
-        return true;
-    }
-    FGK_HD bool try_merge(const uint32_t *o, int stride, int om, int o0, int o1, int o2, int max_rank, int max_coef) {
-        if (m + om > max_rank) return false;  // :1225-1254
-        const bool e0 = n0 == o0, e1 = n1 == o1, e2 = n2 == o2;
-        const int a0 = n0 + o0, a1 = n1 + o1, a2 = n2 + o2;
-        int p[3], size = 0;
-        if (a0 <= kMaxExtensionN && a0 * n1 <= max_coef && a0 * n2 <= max_coef && e1 && e2) p[size++] = 0;
-        if (e0 && a1 <= kMaxExtensionN && n0 * a1 <= max_coef && n2 * a1 <= max_coef && e2) p[size++] = 1;
-        if (e0 && e1 && a2 <= kMaxExtensionN && n0 * a2 <= max_coef && n1 * a2 <= max_coef) p[size++] = 2;
-        if (!size) return false;
-        merge(o, stride, om, o0, o1, o2, pick3(p, aux32() % (uint32_t)size));
-        return true;
-    }
-    FGK_HD bool try_product_scheme(const uint32_t *o, int stride, int om, int o0, int o1, int o2, int max_rank,
-                                   int max_coef) {  // :1256-1275
-        if (m * om > max_rank) return false;
-        const int a = n0 * o0, b = n1 * o1, c = n2 * o2;
-        if (!valid_sizes(a, b, c, max_coef)) return false;
-        product_scheme(o, stride, om, o0, o1, o2);
-        return true;
-    }
-    FGK_HD void swap_size_random() {  // :1315-1324, redraws bounded
-        for (int tries = 0; tries < kPlusMaxTries; ++tries) {
-            const int p1 = (int)(aux32() % 3u), p2 = (int)(aux32() % 3u);
-            if (p1 != p2) {
-                swap_size(p1, p2);
-                return;
-            }
-        }
-    }
-    FGK_HD static int pick3(const int *v, uint32_t q) { return q == 0 ? v[0] : (q == 1 ? v[1] : v[2]); }
-
-    // ---- basis changes (scheme_z2.cu:864-914, 1277-1313) ----
+    // ---- sandwiching (scheme_z2.cu:444-528, 1277-1300) ----
     // invertibleMatrixZ2 (:488-492): a random n x n matrix (row i = the low n
     // bits of one aux draw, random.cuh:24-32) until it is invertible; the
     // redraws are bounded.
@@ -665,40 +401,6 @@ struct Z2LaneT : LaneBaseT<WordT> {
         }
         rebuild();
     }
-    // swapBasis (:1302-1313): swaps two rows of A (rows of U, columns of W)
-    // or two columns of B (columns of V, rows of W).  A permutation keeps
-    // equal factors equal and coefficient counts unchanged, so the partner
-    // classes stay as they are, as the reference keeps its FlipSets.
-    FGK_HD void swap_basis() {
-        if (aux32() % 2u) {
-            const int i1 = (int)(aux32() % (uint32_t)n0), i2 = (int)(aux32() % (uint32_t)n0);
-            for (int t = 0; t < m; ++t) {
-                put(0, t, swap_rows(f(0, t), n0, n1, i1, i2));
-                put(2, t, swap_cols(f(2, t), n2, n0, i1, i2));
-            }
-        } else {
-            const int j1 = (int)(aux32() % (uint32_t)n2), j2 = (int)(aux32() % (uint32_t)n2);
-            for (int t = 0; t < m; ++t) {
-                put(1, t, swap_cols(f(1, t), n1, n2, j1, j2));
-                put(2, t, swap_rows(f(2, t), n2, n0, j1, j2));
-            }
-        }
-    }
-    FGK_HD static uint32_t swap_rows(uint32_t x, int a, int b, int i1, int i2) {
-        const uint32_t rm = (1u << b) - 1u;
-        const uint32_t r1 = (x >> (i1 * b)) & rm, r2 = (x >> (i2 * b)) & rm;
-        x &= ~((rm << (i1 * b)) | (rm << (i2 * b)));
-        return x | (r2 << (i1 * b)) | (r1 << (i2 * b));
-    }
-    FGK_HD static uint32_t swap_cols(uint32_t x, int a, int b, int j1, int j2) {
-        uint32_t y = x;
-        for (int i = 0; i < a; ++i) {
-            const uint32_t c1 = (x >> (i * b + j1)) & 1u, c2 = (x >> (i * b + j2)) & 1u;
-            y = (y & ~((1u << (i * b + j1)) | (1u << (i * b + j2)))) | (c2 << (i * b + j1)) | (c1 << (i * b + j2));
-        }
-        return y;
-    }
-
     // A given pair (component c, index1 t1 < index2 t2) flipped with the
     // orientation coins of tryFlip; false when the factors differ.
     FGK_HD bool try_flip_pair(int c, int t1, int t2, uint32_t coin, bool check_reduce) {

@@ -60,6 +60,11 @@ FGK_HD uint32_t inverse(uint32_t a) { return pack(val(a), ~sgn(a) & val(a)); }
 
 struct ZTLane : LaneBaseT<uint32_t> {
     static constexpr uint32_t kValueMask = 0xFFFFu;  // magnitudes; signs sit in bits 16-31
+    static constexpr bool kSigned = true;            // a bit map acts on both halves of a word
+
+    FGK_HD void put(int c, int t, uint32_t v) { f(c, t) = v; }
+    // sandwiching (scheme_integer.cu:1215-1217) is empty in the reference
+    FGK_HD void sandwiching() {}
 
     uint16_t *V;  // lane-offset base of the visited-candidate list
 

@@ -45,8 +45,8 @@ typedef struct {
     int32_t use_graph;          /* 1: replay each launch (queue reset, walk, Brent check) as a captured HIP graph */
     int32_t grid;               /* persistent workgroups per launch; 0 = all that fit on the GPU at once */
     int32_t word_bits;          /* Z2 LDS factor words: 16 or 32; 0 = 16 only when it lets more walker blocks run at once */
-    double sandwiching_probability; /* --sandwiching-probability (divided by max iterations; Z2, 32-bit words) */
-    double basis_probability;       /* --basis-probability (divided by max iterations; Z2, 32-bit words) */
+    double sandwiching_probability; /* --sandwiching-probability (divided by max iterations; Z2 needs 32-bit words; empty for ZT as in the reference) */
+    double basis_probability;       /* --basis-probability (divided by max iterations; Z2 needs 32-bit words) */
 } fg_config;
 
 /* Per-walker state, for tests and checkpointing. */
@@ -155,16 +155,18 @@ FG_API int fg_autotune(fg_ctx *ctx, int iterations, int32_t *grid);
  * number of invalid schemes found (0 expected) or < 0 on error. */
 FG_API int fg_walk(fg_ctx *ctx, int iterations);
 
-/* resizeKernel (src/entities/flip_graph.cu:575-613), Z2 pools with 32-bit
- * factor words: each walker resizes with the given probability -- half the
+/* resizeKernel (src/entities/flip_graph.cu:575-613) on a walk pool (Z2 with
+ * 32-bit factor words, or the integer ring): each walker resizes with the
+ * given probability -- half the
  * time it first swaps two dimensions (swapSize), then it tries to merge with
  * the best scheme of up to three random walkers (merge), else it projects
  * out a row (project, 5%), multiplies by a random walker's best scheme
  * (product of schemes, 50%), doubles a dimension (product, 30%) or extends
- * one (extend, 15%) (src/schemes/scheme_z2.cu:643-979,1165-1324).  The
- * reference's MAX_RANK and MAX_MATRIX_ELEMENTS are the pool's term capacity
- * and 32.  Runs the GPU Brent check of every scheme; returns the number of
- * invalid ones or < 0. */
+ * one (extend, 15%) (src/schemes/scheme_z2.cu:643-979,1165-1324,
+ * scheme_integer.cu:573-905).  The reference's MAX_RANK and
+ * MAX_MATRIX_ELEMENTS are the pool's term capacity and the width of a factor
+ * word (32 coefficients for Z2, 16 for the integer ring).  Runs the GPU Brent
+ * check of every scheme; returns the number of invalid ones or < 0. */
 FG_API int fg_resize(fg_ctx *ctx, double probability);
 
 /* schemes[i].copyTo(schemesBest[i]) of FlipGraph::updateRanks

@@ -891,10 +891,9 @@ void fgo_walk(fgo_walker *w, const fgo_walk_params *p) {
         if (p->use_expand && b[3] < p->thr_expand && s->m <= w->best.m + 2) { /* :553-556 */
             if (try_expand(s, aux_randint(&w->rng, 1, 2), &w->rng)) w->flips_count = 0;
         }
-        if (s->ring == FGO_RING_Z2) {
-            if (p->use_sandwich && fgo_rng_aux32(&w->rng) < p->thr_sandwich) sandwiching(s, &w->rng); /* :558-559 */
-            if (p->use_basis && fgo_rng_aux32(&w->rng) < p->thr_basis) swap_basis(s, &w->rng);       /* :561-562 */
-        }
+        /* :558-562; the integer ring's sandwiching is empty (scheme_integer.cu:1215-1217) */
+        if (p->use_sandwich && fgo_rng_aux32(&w->rng) < p->thr_sandwich && s->ring == FGO_RING_Z2) sandwiching(s, &w->rng);
+        if (p->use_basis && fgo_rng_aux32(&w->rng) < p->thr_basis) swap_basis(s, &w->rng);
         if (w->flips_count >= p->plus_iterations && try_plus(s, &w->rng)) w->flips_count = 0; /* :564-565 */
         if (s->m < w->min_m) w->min_m = s->m;
     }
@@ -1231,49 +1230,61 @@ int fgo_check_flipset_walk(int ring, int n1, int n2, int n3, uint64_t seed, int 
 
 /* Bit (i, j) of an a x b factor sits at i * b + j. */
 static uint64_t bit_at(uint64_t x, int pos) { return (x >> pos) & 1u; }
+static uint64_t embed(uint64_t x, int a, int b, int nb, int di, int dj);
 
-/* excludeRow (scheme_z2.cu:205-224): drop row `row` of every factor of
- * component `matrix` (n[matrix] x n[matrix + 1]). */
+/* Integer-ring factors carry signs in a second bitmask with the magnitude
+ * layout, so every bit map below is applied to v and to s alike (s is 0 for
+ * Z2). */
+static uint64_t drop_row_bits(uint64_t x, int a, int b, int row) {
+    uint64_t y = 0;
+    int ii = 0;
+    for (int i = 0; i < a; i++) {
+        if (i == row) continue;
+        for (int j = 0; j < b; j++) y |= bit_at(x, i * b + j) << (ii * b + j);
+        ii++;
+    }
+    return y;
+}
+
+static uint64_t drop_col_bits(uint64_t x, int a, int b, int column) {
+    uint64_t y = 0;
+    for (int i = 0; i < a; i++) {
+        int jj = 0;
+        for (int j = 0; j < b; j++) {
+            if (j == column) continue;
+            y |= bit_at(x, i * b + j) << (i * (b - 1) + jj);
+            jj++;
+        }
+    }
+    return y;
+}
+
+/* excludeRow (scheme_z2.cu:205-224, scheme_integer.cu:294-313): drop row
+ * `row` of every factor of component `matrix` (n[matrix] x n[matrix + 1]). */
 static void exclude_row(fgo_scheme *s, int matrix, int row) {
     const int a = s->n[matrix], b = s->n[(matrix + 1) % 3];
     for (int t = 0; t < s->m; t++) {
-        uint64_t x = s->v[matrix][t], y = 0;
-        int ii = 0;
-        for (int i = 0; i < a; i++) {
-            if (i == row) continue;
-            for (int j = 0; j < b; j++) y |= bit_at(x, i * b + j) << (ii * b + j);
-            ii++;
-        }
-        s->v[matrix][t] = y;
+        s->v[matrix][t] = drop_row_bits(s->v[matrix][t], a, b, row);
+        s->s[matrix][t] = drop_row_bits(s->s[matrix][t], a, b, row);
     }
 }
 
-/* excludeColumn (scheme_z2.cu:184-203) */
+/* excludeColumn (scheme_z2.cu:184-203, scheme_integer.cu:273-292) */
 static void exclude_column(fgo_scheme *s, int matrix, int column) {
     const int a = s->n[matrix], b = s->n[(matrix + 1) % 3];
     for (int t = 0; t < s->m; t++) {
-        uint64_t x = s->v[matrix][t], y = 0;
-        for (int i = 0; i < a; i++) {
-            int jj = 0;
-            for (int j = 0; j < b; j++) {
-                if (j == column) continue;
-                y |= bit_at(x, i * b + j) << (i * (b - 1) + jj);
-                jj++;
-            }
-        }
-        s->v[matrix][t] = y;
+        s->v[matrix][t] = drop_col_bits(s->v[matrix][t], a, b, column);
+        s->s[matrix][t] = drop_col_bits(s->s[matrix][t], a, b, column);
     }
 }
 
-/* addColumn (scheme_z2.cu:226-239): an empty last column; addRow
- * (:241-254) leaves the bit positions unchanged. */
+/* addColumn (scheme_z2.cu:226-239, scheme_integer.cu:315-328): an empty last
+ * column; addRow (:241-254) leaves the bit positions unchanged. */
 static void add_column(fgo_scheme *s, int matrix) {
     const int a = s->n[matrix], b = s->n[(matrix + 1) % 3];
     for (int t = 0; t < s->m; t++) {
-        uint64_t x = s->v[matrix][t], y = 0;
-        for (int i = 0; i < a; i++)
-            for (int j = 0; j < b; j++) y |= bit_at(x, i * b + j) << (i * (b + 1) + j);
-        s->v[matrix][t] = y;
+        s->v[matrix][t] = embed(s->v[matrix][t], a, b, b + 1, 0, 0);
+        s->s[matrix][t] = embed(s->s[matrix][t], a, b, b + 1, 0, 0);
     }
 }
 
@@ -1286,17 +1297,20 @@ static void z2_triplet(fgo_scheme *s, uint64_t u, uint64_t v, uint64_t w) {
     add_triplet(s, 0, 1, 2, a, b, c);
 }
 
-/* project (scheme_z2.cu:643-653) */
+/* project (scheme_z2.cu:643-653, scheme_integer.cu:573-584; the integer
+ * ring also fixes signs) */
 void fgo_project(fgo_scheme *s, int p, int q) {
     exclude_row(s, p, q);
     exclude_column(s, (p + 2) % 3, q);
     s->n[p]--;
     set_nn(s);
     remove_zeroes(s);
+    if (s->ring == FGO_RING_ZT) fgo_fix_signs(s);
     fgo_init_flips(s);
 }
 
-/* extend (scheme_z2.cu:655-687) */
+/* extend (scheme_z2.cu:655-687, scheme_integer.cu:586-618): the new terms
+ * have single +1 coefficients */
 void fgo_extend(fgo_scheme *s, int p) {
     const int n0 = s->n[0], n1 = s->n[1], n2 = s->n[2];
     if (p == 0) {
@@ -1329,8 +1343,8 @@ static uint64_t embed(uint64_t x, int a, int b, int nb, int di, int dj) {
     return y;
 }
 
-/* product (scheme_z2.cu:689-747): dimension p doubled, every term copied
- * into the second diagonal block. */
+/* product (scheme_z2.cu:689-747, scheme_integer.cu:620-678): dimension p
+ * doubled, every term copied into the second diagonal block. */
 void fgo_product_dim(fgo_scheme *s, int p) {
     int nn_[3] = {s->n[0], s->n[1], s->n[2]}, d[3];
     nn_[p] *= 2;
@@ -1339,10 +1353,11 @@ void fgo_product_dim(fgo_scheme *s, int p) {
     for (int t = 0; t < m; t++) {
         for (int c = 0; c < 3; c++) {
             const int c1 = (c + 1) % 3;
-            const uint64_t x = s->v[c][t];
+            const uint64_t x = s->v[c][t], g = s->s[c][t];
             s->v[c][t] = embed(x, s->n[c], s->n[c1], nn_[c1], 0, 0);
+            s->s[c][t] = embed(g, s->n[c], s->n[c1], nn_[c1], 0, 0);
             s->v[c][t + m] = embed(x, s->n[c], s->n[c1], nn_[c1], d[c], d[c1]);
-            s->s[c][t + m] = 0;
+            s->s[c][t + m] = embed(g, s->n[c], s->n[c1], nn_[c1], d[c], d[c1]);
             s->ok[c][t + m] = 1;
         }
     }
@@ -1352,7 +1367,8 @@ void fgo_product_dim(fgo_scheme *s, int p) {
     fgo_init_flips(s);
 }
 
-/* merge (scheme_z2.cu:749-817): `o` joins as the second block along p. */
+/* merge (scheme_z2.cu:749-817, scheme_integer.cu:680-745): `o` joins as the
+ * second block along p. */
 void fgo_merge(fgo_scheme *s, const fgo_scheme *o, int p) {
     int nn_[3], d[3];
     for (int i = 0; i < 3; i++) {
@@ -1363,12 +1379,13 @@ void fgo_merge(fgo_scheme *s, const fgo_scheme *o, int p) {
         for (int c = 0; c < 3; c++) {
             const int c1 = (c + 1) % 3;
             s->v[c][t] = embed(s->v[c][t], s->n[c], s->n[c1], nn_[c1], 0, 0);
+            s->s[c][t] = embed(s->s[c][t], s->n[c], s->n[c1], nn_[c1], 0, 0);
         }
     for (int t = 0; t < o->m; t++)
         for (int c = 0; c < 3; c++) {
             const int c1 = (c + 1) % 3;
             s->v[c][s->m + t] = embed(o->v[c][t], o->n[c], o->n[c1], nn_[c1], d[c], d[c1]);
-            s->s[c][s->m + t] = 0;
+            s->s[c][s->m + t] = embed(o->s[c][t], o->n[c], o->n[c1], nn_[c1], d[c], d[c1]);
             s->ok[c][s->m + t] = 1;
         }
     for (int c = 0; c < 3; c++) s->n[c] = nn_[c];
@@ -1377,8 +1394,9 @@ void fgo_merge(fgo_scheme *s, const fgo_scheme *o, int p) {
     fgo_init_flips(s);
 }
 
-/* product (scheme_z2.cu:819-862): Kronecker product of two schemes, term
- * index1 * m2 + index2. */
+/* product (scheme_z2.cu:819-862, scheme_integer.cu:747-789): Kronecker
+ * product of two schemes, term index1 * m2 + index2; an integer coefficient
+ * is the product of the two (signs XOR). */
 void fgo_product_scheme(fgo_scheme *s, const fgo_scheme *o) {
     fgo_scheme *a = (fgo_scheme *)malloc(sizeof(fgo_scheme));
     fgo_copy(a, s, 0);
@@ -1390,18 +1408,20 @@ void fgo_product_scheme(fgo_scheme *s, const fgo_scheme *o) {
             const int t = t1 * o->m + t2;
             for (int c = 0; c < 3; c++) {
                 const int c1 = (c + 1) % 3;
-                uint64_t y = 0;
+                uint64_t y = 0, g = 0;
                 for (int i = 0; i < a->nn[c]; i++) {
                     if (!bit_at(a->v[c][t1], i)) continue;
                     const int r1 = i / a->n[c1], q1 = i % a->n[c1];
                     for (int j = 0; j < o->nn[c]; j++) {
                         if (!bit_at(o->v[c][t2], j)) continue;
                         const int r2 = j / o->n[c1], q2 = j % o->n[c1];
-                        y |= 1ull << ((r1 * o->n[c] + r2) * s->n[c1] + q1 * o->n[c1] + q2);
+                        const int pos = (r1 * o->n[c] + r2) * s->n[c1] + q1 * o->n[c1] + q2;
+                        y |= 1ull << pos;
+                        g |= (bit_at(a->s[c][t1], i) ^ bit_at(o->s[c][t2], j)) << pos;
                     }
                 }
                 s->v[c][t] = y;
-                s->s[c][t] = 0;
+                s->s[c][t] = g;
                 s->ok[c][t] = 1;
             }
         }
@@ -1409,8 +1429,9 @@ void fgo_product_scheme(fgo_scheme *s, const fgo_scheme *o) {
     fgo_init_flips(s);
 }
 
-/* swapSize (scheme_z2.cu:916-979): the scheme of the transposed problem
- * obtained by exchanging dimensions p1 and p2. */
+/* swapSize (scheme_z2.cu:916-979, scheme_integer.cu:843-905): the scheme of
+ * the transposed problem obtained by exchanging dimensions p1 and p2; the
+ * integer ring then fixes signs. */
 void fgo_swap_size(fgo_scheme *s, int p1, int p2) {
     if (p1 == p2) return;
     if (p1 > p2) { int t = p1; p1 = p2; p2 = t; }
@@ -1420,22 +1441,29 @@ void fgo_swap_size(fgo_scheme *s, int p1, int p2) {
     else { idx[0] = 2; idx[1] = 1; idx[2] = 0; }
     for (int i = 0; i < 3; i++) nn_[i] = s->n[(idx[i] + 1) % 3];
     for (int t = 0; t < s->m; t++) {
-        uint64_t y[3];
+        uint64_t y[3], g[3];
         for (int c = 0; c < 3; c++) {
             const int a = nn_[c], b = nn_[(c + 1) % 3];
-            const uint64_t x = s->v[idx[c]][t];
-            y[c] = 0;
+            const uint64_t x = s->v[idx[c]][t], sx = s->s[idx[c]][t];
+            y[c] = g[c] = 0;
             for (int i = 0; i < a; i++)
-                for (int j = 0; j < b; j++) y[c] |= bit_at(x, j * a + i) << (i * b + j);
+                for (int j = 0; j < b; j++) {
+                    y[c] |= bit_at(x, j * a + i) << (i * b + j);
+                    g[c] |= bit_at(sx, j * a + i) << (i * b + j);
+                }
         }
-        for (int c = 0; c < 3; c++) s->v[c][t] = y[c];
+        for (int c = 0; c < 3; c++) {
+            s->v[c][t] = y[c];
+            s->s[c][t] = g[c];
+        }
     }
     for (int c = 0; c < 3; c++) s->n[c] = nn_[c];
     set_nn(s);
+    if (s->ring == FGO_RING_ZT) fgo_fix_signs(s);
     fgo_init_flips(s);
 }
 
-/* swapBasisRows / swapBasisColumns (scheme_z2.cu:864-914) */
+/* swapBasisRows / swapBasisColumns (scheme_z2.cu:864-914, scheme_integer.cu:791-841) */
 void fgo_swap_basis_rows(fgo_scheme *s, int i1, int i2) {
     const int n0 = s->n[0], n1 = s->n[1], n2 = s->n[2];
     int rows[64];
@@ -1443,13 +1471,21 @@ void fgo_swap_basis_rows(fgo_scheme *s, int i1, int i2) {
     rows[i1] = i2;
     rows[i2] = i1;
     for (int t = 0; t < s->m; t++) {
-        uint64_t u = 0, w = 0;
+        uint64_t u = 0, w = 0, su = 0, sw = 0;
         for (int i = 0; i < n0; i++)
-            for (int j = 0; j < n1; j++) u |= bit_at(s->v[0][t], rows[i] * n1 + j) << (i * n1 + j);
+            for (int j = 0; j < n1; j++) {
+                u |= bit_at(s->v[0][t], rows[i] * n1 + j) << (i * n1 + j);
+                su |= bit_at(s->s[0][t], rows[i] * n1 + j) << (i * n1 + j);
+            }
         for (int i = 0; i < n2; i++)
-            for (int j = 0; j < n0; j++) w |= bit_at(s->v[2][t], i * n0 + rows[j]) << (i * n0 + j);
+            for (int j = 0; j < n0; j++) {
+                w |= bit_at(s->v[2][t], i * n0 + rows[j]) << (i * n0 + j);
+                sw |= bit_at(s->s[2][t], i * n0 + rows[j]) << (i * n0 + j);
+            }
         s->v[0][t] = u;
         s->v[2][t] = w;
+        s->s[0][t] = su;
+        s->s[2][t] = sw;
     }
 }
 
@@ -1460,13 +1496,21 @@ void fgo_swap_basis_columns(fgo_scheme *s, int j1, int j2) {
     cols[j1] = j2;
     cols[j2] = j1;
     for (int t = 0; t < s->m; t++) {
-        uint64_t v = 0, w = 0;
+        uint64_t v = 0, w = 0, sv = 0, sw = 0;
         for (int i = 0; i < n1; i++)
-            for (int j = 0; j < n2; j++) v |= bit_at(s->v[1][t], i * n2 + cols[j]) << (i * n2 + j);
+            for (int j = 0; j < n2; j++) {
+                v |= bit_at(s->v[1][t], i * n2 + cols[j]) << (i * n2 + j);
+                sv |= bit_at(s->s[1][t], i * n2 + cols[j]) << (i * n2 + j);
+            }
         for (int i = 0; i < n2; i++)
-            for (int j = 0; j < n0; j++) w |= bit_at(s->v[2][t], cols[i] * n0 + j) << (i * n0 + j);
+            for (int j = 0; j < n0; j++) {
+                w |= bit_at(s->v[2][t], cols[i] * n0 + j) << (i * n0 + j);
+                sw |= bit_at(s->s[2][t], cols[i] * n0 + j) << (i * n0 + j);
+            }
         s->v[1][t] = v;
         s->v[2][t] = w;
+        s->s[1][t] = sv;
+        s->s[2][t] = sw;
     }
 }
 
@@ -1605,14 +1649,14 @@ void fgo_resize_walker(fgo_walker *w, const fgo_scheme *const *bests, int count,
 
 /* One resize pass over records (walkers walker_base + q): every walker
  * reads the best schemes the records hold before the pass. */
-int fgo_resize_records(fgo_walker_record *recs, int count, uint32_t walker_base, uint64_t seed, uint64_t thr_resize,
-                       int max_rank, int max_coef) {
+int fgo_resize_records(int ring, fgo_walker_record *recs, int count, uint32_t walker_base, uint64_t seed,
+                       uint64_t thr_resize, int max_rank, int max_coef) {
     fgo_resize_caps k = {max_rank, max_coef};
     fgo_walker *ws = (fgo_walker *)malloc(sizeof(fgo_walker) * (size_t)count);
     const fgo_scheme **bests = (const fgo_scheme **)malloc(sizeof(fgo_scheme *) * (size_t)count);
     if (!ws || !bests) { free(ws); free(bests); return -2; }
     for (int q = 0; q < count; q++) {
-        if (fgo_walker_from_record(&ws[q], FGO_RING_Z2, &recs[q], seed, walker_base + (uint32_t)q)) {
+        if (fgo_walker_from_record(&ws[q], ring, &recs[q], seed, walker_base + (uint32_t)q)) {
             free(ws); free(bests);
             return -1;
         }
@@ -1639,12 +1683,12 @@ int fgo_resize_records(fgo_walker_record *recs, int count, uint32_t walker_base,
  * 2 product(a), 3 merge(other, a), 4 product(other), 5 swapSize(a, b),
  * 6 swapBasisRows(a, b), 7 swapBasisColumns(a, b).  Returns the output
  * blob length. */
-int fgo_blob_resize_op(const int32_t *in, const int32_t *other, int op, int a, int b, int32_t *out) {
+int fgo_blob_resize_op(int ring, const int32_t *in, const int32_t *other, int op, int a, int b, int32_t *out) {
     fgo_scheme *s = (fgo_scheme *)malloc(sizeof(fgo_scheme)), *o = (fgo_scheme *)malloc(sizeof(fgo_scheme));
     int len = -1;
     if (!s || !o) goto done;
-    if (fgo_from_blob(s, FGO_RING_Z2, in, 0) != 1) goto done;
-    if (other && fgo_from_blob(o, FGO_RING_Z2, other, 0) != 1) goto done;
+    if (fgo_from_blob(s, ring, in, 0) != 1) goto done;
+    if (other && fgo_from_blob(o, ring, other, 0) != 1) goto done;
     switch (op) {
     case 0: fgo_project(s, a, b); break;
     case 1: fgo_extend(s, a); break;
@@ -1754,7 +1798,8 @@ static void sandwiching(fgo_scheme *s, fgo_rng *r) {
 }
 
 /* swapBasis (scheme_z2.cu:1302-1313); the partner lists stay valid, as a
- * permutation keeps equal factors equal. */
+ * permutation keeps equal factors equal.  The integer ring then fixes signs
+ * and rebuilds them (scheme_integer.cu:1219-1233). */
 static void swap_basis(fgo_scheme *s, fgo_rng *r) {
     if (fgo_rng_aux32(r) % 2u) {
         const int i1 = (int)(fgo_rng_aux32(r) % (uint32_t)s->n[0]), i2 = (int)(fgo_rng_aux32(r) % (uint32_t)s->n[0]);
@@ -1763,4 +1808,5 @@ static void swap_basis(fgo_scheme *s, fgo_rng *r) {
         const int j1 = (int)(fgo_rng_aux32(r) % (uint32_t)s->n[2]), j2 = (int)(fgo_rng_aux32(r) % (uint32_t)s->n[2]);
         fgo_swap_basis_columns(s, j1, j2);
     }
+    if (s->ring == FGO_RING_ZT && fgo_fix_signs(s)) fgo_init_flips(s);
 }

@@ -212,9 +212,9 @@ void fgo_swap_basis_rows(fgo_scheme *s, int i1, int i2);
 void fgo_swap_basis_columns(fgo_scheme *s, int j1, int j2);
 void fgo_resize_walker(fgo_walker *w, const fgo_scheme *const *bests, int count, uint64_t thr_resize,
                        const fgo_resize_caps *k);
-int fgo_resize_records(fgo_walker_record *recs, int count, uint32_t walker_base, uint64_t seed, uint64_t thr_resize,
-                       int max_rank, int max_coef);
-int fgo_blob_resize_op(const int32_t *in, const int32_t *other, int op, int a, int b, int32_t *out);
+int fgo_resize_records(int ring, fgo_walker_record *recs, int count, uint32_t walker_base, uint64_t seed,
+                       uint64_t thr_resize, int max_rank, int max_coef);
+int fgo_blob_resize_op(int ring, const int32_t *in, const int32_t *other, int op, int a, int b, int32_t *out);
 
 /* Additions reducer (fg_reducer.c). */
 typedef struct fgo_reducer fgo_reducer;

@@ -9,6 +9,7 @@
 #include "walk_args.h"
 #include "walk_z2_lane.h"
 #include "walk_zt_lane.h"
+#include "resize_ops.h"
 
 static uint32_t pack(int ring, uint64_t v, uint64_t s) {
     return ring ? (uint32_t)((v & 0xFFFFu) | ((s & v & 0xFFFFu) << 16)) : (uint32_t)v;
@@ -93,6 +94,8 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                 if (ring) {
                     if (mode == kModeMinimize) walk_lane<ZTLane, kModeMinimize>(a, L, lane, w, W);
                     else if (mode == kModeFlips) walk_lane<ZTLane, kModeFlips>(a, L, lane, w, W);
+                    else if (mode == kModeResize) walk_lane<ZTLane, kModeResize>(a, L, lane, w, W);
+                    else if (mode == kModeMeta) walk_lane<ZTLane, kModeMeta>(a, L, lane, w, W);
                     else walk_lane<ZTLane, kModeWalk>(a, L, lane, w, W);
                 } else if (mode == kModeResize) {
                     walk_lane<Z2Lane, kModeResize>(a, L, lane, w, W);

@@ -193,13 +193,15 @@ def walk_records_forced(recs, params, seed, forced=None, walker_base=0):
     return recs, log, mm
 
 
-def resize_records(recs, seed, thr_resize, max_rank, max_coef=32, walker_base=0):
-    """One resizeKernel pass (Z2) over walker records."""
+def resize_records(recs, seed, thr_resize, max_rank, max_coef=None, walker_base=0, ring=RING_Z2):
+    """One resizeKernel pass over walker records (max_coef: 32 for Z2, 16 for ZT)."""
+    if max_coef is None:
+        max_coef = 16 if ring == RING_ZT else 32
     recs = np.array(recs, dtype=RECORD_DTYPE, copy=True, order="C")
     L = lib()
-    L.fgo_resize_records.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64,
-                                     ctypes.c_int, ctypes.c_int]
-    rc = L.fgo_resize_records(recs.ctypes.data, len(recs), walker_base, seed, thr_resize, max_rank, max_coef)
+    L.fgo_resize_records.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64,
+                                     ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
+    rc = L.fgo_resize_records(ring, recs.ctypes.data, len(recs), walker_base, seed, thr_resize, max_rank, max_coef)
     assert rc == 0
     return recs
 
@@ -208,17 +210,17 @@ RESIZE_OPS = {"project": 0, "extend": 1, "product": 2, "merge": 3, "product_sche
               "swap_rows": 6, "swap_columns": 7}
 
 
-def resize_op(blob, op, a=0, b=0, other=None):
-    """One resize or basis operator of the oracle on a Z2 blob."""
+def resize_op(blob, op, a=0, b=0, other=None, ring=RING_Z2):
+    """One resize or basis operator of the oracle on a blob."""
     L = lib()
     i32p = ctypes.POINTER(ctypes.c_int32)
-    L.fgo_blob_resize_op.argtypes = [i32p, i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p]
+    L.fgo_blob_resize_op.argtypes = [ctypes.c_int, i32p, i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p]
     x, px = _i32(blob)
     po = None
     if other is not None:
         o, po = _i32(other)
     out = np.zeros(4 + 350 * 3 * 64, dtype=np.int32)
-    n = L.fgo_blob_resize_op(px, po, RESIZE_OPS[op], a, b, out.ctypes.data_as(i32p))
+    n = L.fgo_blob_resize_op(ring, px, po, RESIZE_OPS[op], a, b, out.ctypes.data_as(i32p))
     assert n > 0
     return out[:n].copy()
 

@@ -123,3 +123,21 @@ def test_flip_graph_with_basis_change_edges(tmp_path):
     for f in glob.glob(str(out / "*.json")):
         d = json.load(open(f))
         assert O.validate(S.to_blob((2, 2, 2), d["u"], d["v"], d["w"])) == 1
+
+
+@pytest.mark.gpu
+def test_flip_graph_integer_ring_with_resize_and_basis_edges(tmp_path):
+    # the reference's shipped build is the integer ring (src/config.cuh:6) with
+    # resize edges on by default
+    out = tmp_path / "schemes"
+    print("[cli] flip_graph ZT 3x3x3 with resize and basis edges", flush=True)
+    r = run("-n1", "3", "-n2", "3", "-n3", "3", "--ring", "ZT", "--schemes", "512", "--max-iterations", "400",
+            "--basis-probability", "20", "--max-launches", "3", "--seed", "6", "--path", str(out), timeout=180)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "invalid" not in r.stdout
+    files = glob.glob(str(out / "*_ZT.json"))
+    assert files
+    for f in files:
+        d = json.load(open(f))
+        n = tuple(d["n"])
+        assert O.validate(S.to_blob(n, d["u"], d["v"], d["w"]), O.RING_ZT) == 1, f
