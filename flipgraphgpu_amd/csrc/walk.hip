@@ -184,64 +184,84 @@ __global__ void fill_i32_kernel(int32_t *p, int32_t v, int n) {
     if (g < n) p[g] = v;
 }
 
-// Brent equations (src/schemes/scheme_z2.cu:3-29), bit-sliced: one thread
-// per (scheme, U index i); for each V index j the XOR over terms of the W
-// masks with u_t[i] v_t[j] = 1 must equal the single W bit of c[i/n2][j%n3].
+// Brent equations (src/schemes/scheme_z2.cu:3-29), bit-sliced: one thread per
+// (U index i, scheme s), schemes on the fast index so that a wave's loads of
+// term t are consecutive words.  One pass over the terms XORs the W mask of
+// each term with u_t[i] = 1 into acc[j] for every V index j with v_t[j] = 1;
+// acc[j] must then equal the single W bit of c[i/n2][j%n3] (or be zero).
 __global__ void verify_z2_kernel(VerifyArgs V) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = g / 64;
-    const int i = g % 64;
-    if (s >= V.count) return;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= 32L * V.count) return;
+    const int s = (int)(g % V.count);
+    const int i = (int)(g / V.count);
     const int n0 = V.fmt[4 * s], n1 = V.fmt[4 * s + 1], n2 = V.fmt[4 * s + 2];
     const int nn0 = n0 * n1, nn1 = n1 * n2;
     if (i >= nn0) return;
     const int m = V.m[s];
     const size_t stride = (size_t)V.count;
+    uint32_t acc[32];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) acc[j] = 0;
+    for (int t = 0; t < m; ++t) {
+        const uint32_t u = V.words[((size_t)0 * V.max_m + t) * stride + s];
+        if (!((u >> i) & 1u)) continue;
+        const uint32_t v = V.words[((size_t)1 * V.max_m + t) * stride + s];
+        const uint32_t w = V.words[((size_t)2 * V.max_m + t) * stride + s];
+#pragma unroll
+        for (int j = 0; j < 32; ++j) acc[j] ^= w & (0u - ((v >> j) & 1u));
+    }
+    const int i1 = i / n1, i2 = i % n1;
     bool ok = true;
-    for (int j = 0; j < nn1 && ok; ++j) {
-        uint32_t acc = 0;
-        for (int t = 0; t < m; ++t) {
-            const uint32_t u = V.words[((size_t)0 * V.max_m + t) * stride + s];
-            const uint32_t v = V.words[((size_t)1 * V.max_m + t) * stride + s];
-            if (((u >> i) & (v >> j) & 1u) != 0) acc ^= V.words[((size_t)2 * V.max_m + t) * stride + s];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+        if (j < nn1) {
+            const int j1 = j / n2, j2 = j % n2;
+            const uint32_t want = (i2 == j1) ? (1u << (j2 * n0 + i1)) : 0u;
+            ok = ok && acc[j] == want;
         }
-        const int i1 = i / n1, i2 = i % n1, j1 = j / n2, j2 = j % n2;
-        const uint32_t want = (i2 == j1) ? (1u << (j2 * n0 + i1)) : 0u;
-        ok = acc == want;
     }
     if (!ok) V.ok[s] = 0;
 }
 
 // Brent equations over the integers (src/schemes/scheme_integer.cu:3-38) for
 // packed ternary words (magnitudes bits 0-15, signs bits 16-31): one thread
-// per (scheme, U index i) accumulates sum_t u_t[i] v_t[j] w_t[k] for every
-// (j, k).
+// per (U index i, V index j, scheme s), schemes on the fast index.  One pass
+// over the terms adds the signed W row of every term with u_t[i] v_t[j] != 0
+// into acc[k]; acc[k] must be 1 exactly at the product's entry and 0 elsewhere.
 __global__ void verify_zt_kernel(VerifyArgs V) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = g / 64;
-    const int i = g % 64;
-    if (s >= V.count) return;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= 256L * V.count) return;
+    const int s = (int)(g % V.count);
+    const int ij = (int)(g / V.count);
+    const int i = ij / 16, j = ij % 16;
     const int n0 = V.fmt[4 * s], n1 = V.fmt[4 * s + 1], n2 = V.fmt[4 * s + 2];
     const int nn0 = n0 * n1, nn1 = n1 * n2, nn2 = n2 * n0;
-    if (i >= nn0) return;
+    if (i >= nn0 || j >= nn1) return;
     const int m = V.m[s];
     const size_t stride = (size_t)V.count;
+    int acc[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = 0;
+    for (int t = 0; t < m; ++t) {
+        const uint32_t u = V.words[((size_t)0 * V.max_m + t) * stride + s];
+        if (!((u >> i) & 1u)) continue;
+        const uint32_t v = V.words[((size_t)1 * V.max_m + t) * stride + s];
+        if (!((v >> j) & 1u)) continue;
+        const uint32_t w = V.words[((size_t)2 * V.max_m + t) * stride + s];
+        const uint32_t sg = ((u >> (16 + i)) ^ (v >> (16 + j))) & 1u;  // sign of u_t[i] v_t[j]
+        const uint32_t neg = (w >> 16) ^ (0u - sg);                     // bit k: the product's sign at k
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if ((w >> k) & 1u) acc[k] += ((neg >> k) & 1u) ? -1 : 1;
+    }
+    const int i1 = i / n1, i2 = i % n1, j1 = j / n2, j2 = j % n2;
     bool ok = true;
-    for (int j = 0; j < nn1 && ok; ++j) {
-        for (int k = 0; k < nn2 && ok; ++k) {
-            int e = 0;
-            for (int t = 0; t < m; ++t) {
-                const uint32_t u = V.words[((size_t)0 * V.max_m + t) * stride + s];
-                const uint32_t v = V.words[((size_t)1 * V.max_m + t) * stride + s];
-                const uint32_t w = V.words[((size_t)2 * V.max_m + t) * stride + s];
-                if ((u >> i) & (v >> j) & (w >> k) & 1u) {
-                    const uint32_t neg = ((u >> (16 + i)) ^ (v >> (16 + j)) ^ (w >> (16 + k))) & 1u;
-                    e += neg ? -1 : 1;
-                }
-            }
-            const int i1 = i / n1, i2 = i % n1, j1 = j / n2, j2 = j % n2, k1 = k / n0, k2 = k % n0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (k < nn2) {
+            const int k1 = k / n0, k2 = k % n0;
             const int target = (i2 == j1) && (i1 == k2) && (j2 == k1);
-            ok = e == target;
+            ok = ok && acc[k] == target;
         }
     }
     if (!ok) V.ok[s] = 0;
@@ -321,7 +341,8 @@ hipError_t launch_fill(int32_t *p, int32_t v, int n, hipStream_t stream) {
 }
 
 hipError_t launch_verify(int ring, const VerifyArgs &v, hipStream_t stream) {
-    const long threads = (long)v.count * 64;
+    // Z2: (i < 32, scheme); integer ring: (i < 16, j < 16, scheme)
+    const long threads = (long)v.count * (ring ? 256 : 32);
     const int block = 256;
     const int grid = (int)((threads + block - 1) / block);
     if (grid == 0) return hipSuccess;
