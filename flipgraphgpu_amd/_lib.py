@@ -497,3 +497,14 @@ class Scorer:
 
     def propose(self, engine, candidates=8, every=2, round_=0):
         _check(self.L.fg_scorer_propose(self.s, engine.ctx, candidates, every, round_), "fg_scorer_propose")
+
+    def last_proposals(self, max_tuples, candidates):
+        """(walkers [B], pairs [B][K][3], scores [B][K]) of the last fused proposal round."""
+        cap = max_tuples * candidates
+        wk = np.zeros(max_tuples, dtype=np.int32)
+        pr = np.zeros((cap, 3), dtype=np.int32)
+        sc = np.zeros(cap, dtype=np.float32)
+        self.L.fg_scorer_last_proposals.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int64]
+        B = _check(self.L.fg_scorer_last_proposals(self.s, wk.ctypes.data, pr.ctypes.data, sc.ctypes.data, cap),
+                   "fg_scorer_last_proposals")
+        return wk[:B], pr[:B * candidates].reshape(B, candidates, 3), sc[:B * candidates].reshape(B, candidates)

@@ -59,6 +59,12 @@ struct fg_scorer {
          obo = 0;
     float *w = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
     bf16 *wb = nullptr;
+    // fused proposal rounds: transposed weights and the candidates of the last round
+    bf16 *wt = nullptr;
+    int32_t *fmap = nullptr, *fpairs = nullptr, *ftup = nullptr;
+    float *fscores = nullptr;
+    long capF = 0;
+    int lastB = 0, lastK = 0;
     int step = 0;
 
     // workspace (capacity in tuples and pairs; backward buffers on demand)
@@ -85,7 +91,7 @@ struct fg_scorer {
     }
     ~fg_scorer() {
         free_work();
-        void *ps[] = {w, g, m1, m2, wb};
+        void *ps[] = {w, g, m1, m2, wb, wt, fmap, fpairs, ftup, fscores};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
@@ -425,11 +431,101 @@ extern "C" int fg_scorer_train(fg_scorer *s, fg_ctx *ctx, int steps, int batch, 
     return 0;
 }
 
+namespace {
+// A proposal round with the fused kernel: candidates for every proposing
+// walker in one launch, then one workgroup per walker runs the whole network
+// and writes its best candidate.
+int propose_fused(fg_scorer *s, const fgrt::PoolView &v, int candidates, int every, uint32_t round) {
+    const int L = (v.W + every - 1) / every;
+    const long need = (long)L * candidates;
+    if (need > s->capF || !s->fmap) {
+        void *ps[] = {s->fmap, s->fpairs, s->ftup, s->fscores};
+        for (void *p : ps)
+            if (p) (void)hipFree(p);
+        s->fmap = s->fpairs = s->ftup = nullptr;
+        s->fscores = nullptr;
+        s->capF = 0;
+        SCHK(salloc(&s->fmap, (size_t)std::max(L, 1)));
+        SCHK(salloc(&s->fpairs, (size_t)need * 3));
+        SCHK(salloc(&s->ftup, (size_t)need));
+        SCHK(salloc(&s->fscores, (size_t)need));
+        s->capF = need;
+    }
+    if (!s->wt) SCHK(salloc(&s->wt, (size_t)fgk::sc::kPD_ * (fgk::sc::kProposeMaxF + 2 * fgk::sc::kPD_)));
+    bf16 *W0t = s->wt, *A1t = W0t + (size_t)fgk::sc::kPD_ * fgk::sc::kProposeMaxF, *A2t = A1t + (size_t)fgk::sc::kPD_ * fgk::sc::kPD_;
+    SCHK(fgk::sc::launch_transpose_bf16(s->wb + s->oW0, s->F, s->D, fgk::sc::kProposeMaxF, W0t, s->stream));
+    SCHK(fgk::sc::launch_transpose_bf16(s->wb + s->oA1, s->D, s->D, s->D, A1t, s->stream));
+    SCHK(fgk::sc::launch_transpose_bf16(s->wb + s->oA2, s->D, s->D, s->D, A2t, s->stream));
+    std::vector<int32_t> map(L);
+    for (int q = 0; q < L; ++q) map[q] = q * every;
+    SCHK(hipMemcpyAsync(s->fmap, map.data(), (size_t)L * 4, hipMemcpyHostToDevice, s->stream));
+    fgk::sc::CandArgs c{};
+    c.words = v.cur;
+    c.m = v.m;
+    c.map = s->fmap;
+    c.W = v.W;
+    c.B = L;
+    c.M = s->M;
+    c.K = candidates;
+    c.key0 = v.key0;
+    c.key1 = v.key1;
+    c.round = round;
+    c.pairs = s->fpairs;
+    c.tup = s->ftup;
+    SCHK(fgk::sc::launch_candidates(c, s->stream));
+    fgk::sc::ProposeArgs a{};
+    a.words = v.cur;
+    a.m = v.m;
+    a.map = s->fmap;
+    a.W = v.W;
+    a.M = s->M;
+    a.nn0 = s->nn[0];
+    a.nn1 = s->nn[1];
+    a.nn2 = s->nn[2];
+    a.B = L;
+    a.K = candidates;
+    a.pairs = s->fpairs;
+    a.W0t = W0t;
+    a.A1t = A1t;
+    a.A2t = A2t;
+    a.B1 = s->wb + s->oB1;
+    a.B2 = s->wb + s->oB2;
+    a.Wz = s->wb + s->oWz;
+    a.b0 = s->w + s->ob0;
+    a.c1 = s->w + s->oc1;
+    a.c2 = s->w + s->oc2;
+    a.bz = s->w + s->obz;
+    a.wo = s->w + s->owo;
+    a.bo = s->w + s->obo;
+    a.scores = s->fscores;
+    a.forced = v.forced;
+    SCHK(fgk::sc::launch_propose(a, s->stream));
+    SCHK(hipStreamSynchronize(s->stream));
+    s->lastB = L;
+    s->lastK = candidates;
+    return 0;
+}
+}  // namespace
+
+extern "C" int fg_scorer_last_proposals(fg_scorer *s, int32_t *walkers, int32_t *pairs, float *scores, int64_t cap) {
+    if (!s) return sfail("fg_scorer_last_proposals: null scorer");
+    const long n = (long)s->lastB * s->lastK;
+    if (n == 0) return 0;
+    if (cap < n) return sfail("fg_scorer_last_proposals: buffer too small");
+    if (walkers) SCHK(hipMemcpy(walkers, s->fmap, (size_t)s->lastB * 4, hipMemcpyDeviceToHost));
+    if (pairs) SCHK(hipMemcpy(pairs, s->fpairs, (size_t)n * 12, hipMemcpyDeviceToHost));
+    if (scores) SCHK(hipMemcpy(scores, s->fscores, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return (int)s->lastB;
+}
+
 extern "C" int fg_scorer_propose(fg_scorer *s, fg_ctx *ctx, int candidates, int every, uint32_t round) {
     if (!s || candidates < 1 || every < 1) return sfail("fg_scorer_propose: bad arguments");
     fgrt::PoolView v;
     if (!fgrt::pool_view(ctx, &v)) return sfail("fg_scorer_propose: context not initialised");
     if (v.ring != FG_RING_Z2 || v.maxm != s->M) return sfail("fg_scorer_propose: pool does not match the scorer");
+    if (s->M <= fgk::sc::kProposeMaxTerms && s->F <= fgk::sc::kProposeMaxF && candidates <= fgk::sc::kProposeMaxK)
+        return propose_fused(s, v, candidates, every, round);
+    s->lastB = s->lastK = 0;
     const int chunk = 4096;
     const int L = (v.W + every - 1) / every;
     if (ensure(s, std::min(chunk, L), std::min(chunk, L) * candidates, false)) return -1;

@@ -105,5 +105,39 @@ hipError_t launch_candidates(const CandArgs &a, hipStream_t st);
 hipError_t launch_choose(const float *scores, const int32_t *pairs, const int32_t *map, int B, int K, int32_t *forced,
                          hipStream_t st);
 
+// Fused proposal round (inference only): one workgroup of 4 waves scores the
+// K candidate pairs of one pool walker with the whole network kept on chip --
+// term features, the three set layers on the MFMA units (each wave owns 32
+// of the 128 columns for all terms), the per-scheme means, the pair head --
+// and writes the best candidate into the proposal array.  The rounding
+// points are those of the unfused forward pass: every MFMA operand is bf16,
+// accumulation and the elementwise steps are FP32.  Schemes of up to
+// kProposeMaxTerms terms and kProposeMaxF feature bits; K <= kProposeMaxK.
+constexpr int kProposeMaxTerms = 64;
+constexpr int kProposeMaxF = 96;
+constexpr int kProposeMaxK = 16;
+constexpr int kPD_ = 128;  // hidden width of the network
+struct ProposeArgs {
+    const uint32_t *words;  // pool planes words[(c * M + t) * W + w]
+    const int32_t *m;
+    const int32_t *map;     // tuple -> walker
+    long W;
+    int M;                  // pool term capacity (<= kProposeMaxTerms)
+    int nn0, nn1, nn2;      // F = nn0 + nn1 + nn2 <= kProposeMaxF
+    int B, K;
+    const int32_t *pairs;   // [B * K][3] candidates (component -1: none)
+    const bf16 *W0t;        // [128][kProposeMaxF]: W0 transposed, zero beyond F
+    const bf16 *A1t, *A2t;  // [128][128]: A1, A2 transposed
+    const bf16 *B1, *B2;    // [128][128] as stored (k, d)
+    const bf16 *Wz;         // [256][64] as stored
+    const float *b0, *c1, *c2, *bz, *wo, *bo;
+    float *scores;          // [B * K] (may be null)
+    int32_t *forced;        // [W][4] proposals
+};
+hipError_t launch_propose(const ProposeArgs &a, hipStream_t st);
+// Column-major bf16 copy of a [rows][cols] row-major bf16 matrix, rows padded
+// with zeros to ld: out[c * ld + r].
+hipError_t launch_transpose_bf16(const bf16 *in, int rows, int cols, int ld, bf16 *out, hipStream_t st);
+
 }  // namespace sc
 }  // namespace fgk

@@ -392,6 +392,236 @@ __global__ void __launch_bounds__(64) candidates_kernel(CandArgs a) {
     }
 }
 
+__global__ void transpose_bf16_kernel(const bf16 *in, int rows, int cols, int ld, bf16 *out) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (long)cols * ld) return;
+    const int c = (int)(g / ld), r = (int)(g % ld);
+    out[g] = r < rows ? in[(long)r * cols + c] : (bf16)0.0f;
+}
+
+hipError_t launch_transpose_bf16(const bf16 *in, int rows, int cols, int ld, bf16 *out, hipStream_t st) {
+    hipLaunchKernelGGL(transpose_bf16_kernel, dim3(blocks((long)cols * ld)), dim3(256), 0, st, in, rows, cols, ld, out);
+    return hipGetLastError();
+}
+
+namespace {
+constexpr int kPD = 128;                         // hidden width
+constexpr int kPZ = 64;                          // pair-head width
+constexpr int kPXs = kProposeMaxF + 8;           // LDS row stride of the features (bf16)
+constexpr int kPEs = kPD + 8;                    // LDS row stride of the activations (bf16)
+constexpr int kPSmem = kProposeMaxTerms * kPD * 4;  // G in FP32; aliases the feature and activation tiles
+static_assert(kProposeMaxTerms * (kPXs + kPEs) * 2 <= kPSmem, "feature + activation tiles must fit the G tile");
+
+__device__ __forceinline__ float bf16r(float x) { return (float)(bf16)x; }
+
+// acc[rt][ct] += A[64 x K] (LDS, row stride lda) * Bt^T, for this wave's 32
+// columns; Bt is the transposed weight [128][ldb] in global memory.
+__device__ __forceinline__ void propose_mma(floatx4 (&acc)[4][2], const bf16 *A, int lda, const bf16 *Bt, int ldb,
+                                            int K, int wave, int lane) {
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        bf16x8 bfv[2];
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+            bfv[ct] = *reinterpret_cast<const bf16x8 *>(Bt + (long)(32 * wave + 16 * ct + (lane & 15)) * ldb + k0 +
+                                                        8 * (lane >> 4));
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+            const bf16x8 af = *reinterpret_cast<const bf16x8 *>(A + (rt * 16 + (lane & 15)) * lda + k0 + 8 * (lane >> 4));
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfv[ct], acc[rt][ct], 0, 0, 0);
+        }
+    }
+}
+
+// q[col] = sum_k mean[k] B[k][col] for this wave's 32 columns (two lanes per
+// column, each half of k), written to Qv.
+__device__ __forceinline__ void propose_mean_product(const bf16 *Mv, const bf16 *Bm, float *Qv, int wave, int lane) {
+    const int col = 32 * wave + (lane & 31), half = lane >> 5;
+    float q = 0.f;
+    for (int k = 64 * half; k < 64 * half + 64; ++k) q += (float)Mv[k] * (float)Bm[k * kPD + col];
+    q += __shfl_xor(q, 32);
+    if (lane < 32) Qv[col] = q;
+}
+}  // namespace
+
+__global__ void __launch_bounds__(256) propose_kernel(ProposeArgs a) {
+    __shared__ __attribute__((aligned(16))) unsigned char smem[kPSmem];
+    __shared__ __attribute__((aligned(16))) bf16 Mv[kPD];
+    __shared__ float Qv[kPD];
+    __shared__ float sc[kProposeMaxK];
+    bf16 *Es = reinterpret_cast<bf16 *>(smem);               // [64][kPEs]
+    bf16 *Xs = Es + kProposeMaxTerms * kPEs;                  // [64][kPXs]
+    float *Gs = reinterpret_cast<float *>(smem);             // [64][128], after the last layer
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int b = blockIdx.x;
+    const long w = a.map[b];
+    const int m = a.m[w];
+    const float inv_n = m > 0 ? 1.f / (float)m : 0.f;
+    const bf16 one = (bf16)1.0f, zero = (bf16)0.0f;
+
+    // term features: the factor bits of U, V, W (zero rows beyond the rank)
+    for (int q = tid; q < kProposeMaxTerms * kPXs; q += 256) Xs[q] = zero;
+    __syncthreads();
+    if (tid < 3 * kProposeMaxTerms) {
+        const int c = tid / kProposeMaxTerms, t = tid % kProposeMaxTerms;
+        if (t < m) {
+            const uint32_t x = a.words[((long)c * a.M + t) * a.W + w];
+            const int off = c == 0 ? 0 : (c == 1 ? a.nn0 : a.nn0 + a.nn1);
+            const int nnc = c == 0 ? a.nn0 : (c == 1 ? a.nn1 : a.nn2);
+            for (int q = 0; q < nnc; ++q) Xs[t * kPXs + off + q] = ((x >> q) & 1u) ? one : zero;
+        }
+    }
+    __syncthreads();
+
+    floatx4 acc[4][2];
+    auto clear = [&]() {
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
+    };
+    // column sums of the valid rows of this lane's tiles, reduced over the 4
+    // row groups of the wave; then the bf16 mean goes to Mv
+    auto means = [&](float (&v)[4][2][4]) {
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            float sum = 0.f;
+#pragma unroll
+            for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (rt * 16 + 4 * (lane >> 4) + q < m) sum += v[rt][ct][q];
+            sum += __shfl_xor(sum, 16);
+            sum += __shfl_xor(sum, 32);
+            if (lane < 16) Mv[32 * wave + 16 * ct + lane] = (bf16)(sum * inv_n);
+        }
+    };
+    float v[4][2][4];
+
+    // e = relu(x W0 + b0)
+    clear();
+    propose_mma(acc, Xs, kPXs, a.W0t, kProposeMaxF, kProposeMaxF, wave, lane);
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int col = 32 * wave + 16 * ct + (lane & 15);
+                const float x = acc[rt][ct][q] + a.b0[col];
+                v[rt][ct][q] = x > 0.f ? x : 0.f;
+            }
+    means(v);
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                Es[(rt * 16 + 4 * (lane >> 4) + q) * kPEs + 32 * wave + 16 * ct + (lane & 15)] = (bf16)v[rt][ct][q];
+    __syncthreads();
+
+    // h = relu(e A1 + mean(e) B1 + c1), then g = relu(h A2 + mean(h) B2 + c2)
+    for (int layer = 0; layer < 2; ++layer) {
+        const bf16 *At = layer ? a.A2t : a.A1t;
+        const bf16 *Bm = layer ? a.B2 : a.B1;
+        const float *cb = layer ? a.c2 : a.c1;
+        propose_mean_product(Mv, Bm, Qv, wave, lane);
+        clear();
+        propose_mma(acc, Es, kPEs, At, kPD, kPD, wave, lane);
+        __syncthreads();  // every wave is done with Es, Qv is complete
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int col = 32 * wave + 16 * ct + (lane & 15), row = rt * 16 + 4 * (lane >> 4) + q;
+                    const float x = acc[rt][ct][q] + Qv[col] + cb[col];
+                    v[rt][ct][q] = (x > 0.f && row < m) ? x : 0.f;
+                }
+        if (layer == 0) {
+            means(v);
+#pragma unroll
+            for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        Es[(rt * 16 + 4 * (lane >> 4) + q) * kPEs + 32 * wave + 16 * ct + (lane & 15)] = (bf16)v[rt][ct][q];
+        } else {
+#pragma unroll
+            for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        Gs[(rt * 16 + 4 * (lane >> 4) + q) * kPD + 32 * wave + 16 * ct + (lane & 15)] = v[rt][ct][q];
+        }
+        __syncthreads();
+    }
+
+    // pair head: z = relu([g_a + g_b, g_a * g_b] Wz + bz), s = z . wo[c] + bo[c];
+    // 8 candidates per pass, 32 lanes per candidate, two outputs per lane
+    for (int k0 = 0; k0 < a.K; k0 += 8) {
+        const int k = k0 + tid / 32, jj = tid % 32;
+        const long p = (long)b * a.K + k;
+        const int c = k < a.K ? a.pairs[3 * p] : -1;
+        float part = 0.f;
+        if (c >= 0) {
+            const int t1 = a.pairs[3 * p + 1], t2 = a.pairs[3 * p + 2];
+            float z0 = 0.f, z1 = 0.f;
+            for (int i = 0; i < kPD; ++i) {
+                const float x = Gs[t1 * kPD + i], y = Gs[t2 * kPD + i];
+                const float qs = bf16r(x + y), qm = bf16r(x * y);
+                z0 += qs * (float)a.Wz[i * kPZ + 2 * jj] + qm * (float)a.Wz[(kPD + i) * kPZ + 2 * jj];
+                z1 += qs * (float)a.Wz[i * kPZ + 2 * jj + 1] + qm * (float)a.Wz[(kPD + i) * kPZ + 2 * jj + 1];
+            }
+            z0 += a.bz[2 * jj];
+            z1 += a.bz[2 * jj + 1];
+            z0 = z0 > 0.f ? z0 : 0.f;
+            z1 = z1 > 0.f ? z1 : 0.f;
+            part = z0 * a.wo[c * kPZ + 2 * jj] + z1 * a.wo[c * kPZ + 2 * jj + 1];
+        }
+#pragma unroll
+        for (int d = 16; d > 0; d >>= 1) part += __shfl_xor(part, d);
+        if (jj == 0 && k < a.K) {
+            const float s = c >= 0 ? a.bo[c] + part : -1e30f;
+            sc[k] = s;
+            if (a.scores) a.scores[p] = s;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {  // the first best valid candidate, as the choose kernel picks it
+        int best = -1;
+        float bs = 0.f;
+        for (int k = 0; k < a.K; ++k) {
+            if (a.pairs[3 * ((long)b * a.K + k)] < 0) continue;
+            if (best < 0 || sc[k] > bs) {
+                best = k;
+                bs = sc[k];
+            }
+        }
+        int32_t *f = a.forced + w * 4;
+        if (best < 0) {
+            f[0] = -1;
+            f[1] = f[2] = f[3] = 0;
+        } else {
+            const long p = (long)b * a.K + best;
+            f[0] = a.pairs[3 * p];
+            f[1] = a.pairs[3 * p + 1];
+            f[2] = a.pairs[3 * p + 2];
+            f[3] = 1;
+        }
+    }
+}
+
+hipError_t launch_propose(const ProposeArgs &a, hipStream_t st) {
+    if (a.B <= 0) return hipSuccess;
+    hipLaunchKernelGGL(propose_kernel, dim3((unsigned)a.B), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_candidates(const CandArgs &a, hipStream_t st) {
     const size_t lds = (size_t)6 * a.M * 4;
     hipLaunchKernelGGL(candidates_kernel, dim3(a.B), dim3(64), lds, st, a);

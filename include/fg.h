@@ -253,6 +253,12 @@ FG_API int fg_scorer_train(fg_scorer *s, fg_ctx *ctx, int steps, int batch, floa
 /* For walkers w with w % every == 0: draw `candidates` equal-factor pairs
  * uniformly, score them and propose the best (fg_set_forced's layout). */
 FG_API int fg_scorer_propose(fg_scorer *s, fg_ctx *ctx, int candidates, int every, uint32_t round);
+/* Candidates and scores of the last proposal round (fused rounds only:
+ * schemes of up to 64 terms and 96 factor bits, at most 16 candidates):
+ * walkers [rounds' tuples], pairs [tuples][candidates][3], scores
+ * [tuples][candidates] (-1e30 for a missing candidate).  Returns the number of
+ * tuples (0 when the last round was not fused); any output may be null. */
+FG_API int fg_scorer_last_proposals(fg_scorer *s, int32_t *walkers, int32_t *pairs, float *scores, int64_t cap);
 /* fg_scorer_propose + fg_walk_forced. */
 FG_API int fg_walk_learned(fg_ctx *ctx, fg_scorer *s, int iterations, int candidates, int every, uint32_t round);
 

@@ -120,3 +120,54 @@ def test_training_and_proposals():
         assert proposed > 0
         bad = e.walk_learned(sc, 200, candidates=8, every=2, round_=2)
         assert bad == 0
+
+
+@pytest.mark.parametrize("n,M", [((4, 4, 4), 64), ((3, 3, 3), 27)])
+def test_fused_proposals_match_oracle_scores(n, M):
+    # the fused proposal kernel (whole network in one workgroup per walker)
+    # scores every candidate as the float64 oracle does at the same bf16
+    # rounding points, and proposes the first best-scoring candidate
+    print(f"[learned] fused proposals n={n}", flush=True)
+    W, K, every = 512, 8, 2
+    nn = (n[0] * n[1], n[1] * n[2], n[2] * n[0])
+    with fg.Engine(W, 9, max_iterations=300, plus_iterations=300, max_rank=M, word_bits=32) as e, \
+            fg.Scorer(*n, M, seed=5) as sc:
+        e.init_naive(*n)
+        for _ in range(2):
+            assert e.walk(300) == 0
+        p = SR.init_params(sum(nn), 13).astype(np.float32)
+        sc.set_params(p)
+        sc.propose(e, candidates=K, every=every, round_=3)
+        walkers, pairs, scores = sc.last_proposals(W, K)
+        forced = e.get_forced()
+        recs = e.read_walkers()
+        layout = sc.layout()
+    assert len(walkers) == (W + every - 1) // every
+    assert np.array_equal(walkers, np.arange(0, W, every))
+    rows, ref_idx = [], []
+    words = np.zeros((len(walkers), 3, M), dtype=np.uint32)
+    m = recs["m"][walkers].astype(np.int32)
+    for b, w in enumerate(walkers):
+        words[b] = recs["cur_v"][w][:, :M].astype(np.uint32)
+        for k in range(K):
+            if pairs[b, k, 0] >= 0:
+                rows.append(b)
+                ref_idx.append((b, k))
+    assert rows, "no candidate pairs"
+    tw = words[rows]
+    tm = m[rows]
+    tp = np.array([pairs[b, k] for b, k in ref_idx], dtype=np.int32)
+    s_ref, _ = SR.forward(p.astype(np.float64), layout, tw, tm, tp, nn)
+    s_gpu = np.array([scores[b, k] for b, k in ref_idx])
+    assert _rel(s_gpu, s_ref) < 1e-2, _rel(s_gpu, s_ref)
+    for b, w in enumerate(walkers):
+        valid = pairs[b, :, 0] >= 0
+        if not valid.any():
+            assert forced[w, 0] == -1
+            continue
+        k = int(np.argmax(np.where(valid, scores[b], -np.inf)))
+        assert tuple(forced[w, :3]) == tuple(pairs[b, k]) and forced[w, 3] == 1
+        # every candidate is an equal-factor pair of the walker's scheme
+        for kk in np.nonzero(valid)[0]:
+            c, t1, t2 = pairs[b, kk]
+            assert t1 < t2 < m[b] and recs["cur_v"][w][c][t1] == recs["cur_v"][w][c][t2]
