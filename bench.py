@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "ttt"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "ttt", "propose"],
                     help="c2: the throughput bench (default); c1: 2x2x2 time to rank 7; c3: batched uneven formats; "
                          "c4: integer-ring walks + additions reducer + complexity minimizer; ttt: learned vs uniform")
     ap.add_argument("--launches", type=int, default=0, help="launch budget of the c1/c3/c4 workloads (0: default)")
@@ -205,6 +205,78 @@ def time_to_target(a):
     print(json.dumps(out))
 
 
+def propose_flop(n, M, K, D=128, Z=64):
+    """Algorithmic FLOPs (2 per multiply-add) of one walker's proposal: three set
+    layers over M terms (the first on the F = n1n2 + n2n3 + n3n1 factor bits),
+    the two mean products and the pair head of K candidates (2*D inputs, Z
+    outputs).  Padding the kernel executes (feature bits rounded up to its
+    K-step) is not counted."""
+    F = n[0] * n[1] + n[1] * n[2] + n[2] * n[0]
+    return 2 * M * (F + 2 * D) * D + 2 * 2 * D * D + 2 * K * 2 * D * Z
+
+
+def workload_propose(a):
+    """Learned-proposal path at 4x4x4 Z2 (rank 64, 131072 walkers): log uniform
+    launches, train the scorer for a few steps, then time proposal rounds (every
+    2nd walker, 8 candidates) against plain launches.  The fused proposal
+    kernel's duration comes from HIP events on the scorer's stream."""
+    import numpy as np
+
+    import flipgraphgpu_amd as fg
+
+    W, L, K, every = a.ttt_walkers, a.ttt_launch, a.candidates, a.learned_every
+    e = fg.Engine(W, 5, max_iterations=L, plus_iterations=a.ttt_plus, expand_probability=0.0)
+    e.init_naive(4, 4, 4)
+    e.log_enable(max_batches=12, every=1, horizon=2)
+    for q in range(13):
+        e.walk(L)
+    sc = fg.Scorer(4, 4, 4, 64, seed=7)
+    t0 = time.perf_counter()
+    steps = 20
+    sc.train(e, steps=steps, batch=a.train_batch, lr=a.lr, seed=1)
+    t_train = (time.perf_counter() - t0) / steps
+    e.close()
+    e = fg.Engine(W, 6, max_iterations=L, plus_iterations=a.ttt_plus, expand_probability=0.0)
+    e.init_naive(4, 4, 4)
+    for q in range(max(a.warmup, 1) + 2):
+        e.walk(L)
+    sc.propose(e, candidates=K, every=every, round_=0)
+    kern, rounds, learned, plain = [], [], [], []
+    for q in range(a.steps):
+        t0 = time.perf_counter()
+        sc.propose(e, candidates=K, every=every, round_=q + 1)
+        rounds.append(time.perf_counter() - t0)
+        kern.append(sc.last_propose_ms())
+        t0 = time.perf_counter()
+        e.walk_forced(L)
+        learned.append(time.perf_counter() - t0 + rounds[-1])
+    for q in range(a.steps):
+        t0 = time.perf_counter()
+        e.walk(L)
+        plain.append(time.perf_counter() - t0)
+    sc.close()
+    e.close()
+    B = (W + every - 1) // every
+    flop = propose_flop((4, 4, 4), 64, K) * B
+    k_ms = float(np.mean(kern))
+    achieved = flop / (k_ms * 1e-3) / 1e12
+    out = {
+        "metric": "fused proposal round time, 4x4x4 Z2", "value": round(1e3 * float(np.mean(rounds)), 3),
+        "unit": "ms", "higher_is_better": False, "n_gpus": 1, "steps": a.steps, "dtype": "bf16",
+        "data": "synthetic", "vs_baseline": None,
+        "propose_kernel_ms": round(k_ms, 3),
+        "learned_launch_ms": round(1e3 * float(np.mean(learned)), 3),
+        "plain_launch_ms": round(1e3 * float(np.mean(plain)), 3),
+        "train_step_ms": round(1e3 * t_train, 3),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(achieved / 2500.0, 4), "traffic": None},
+        "config": {"workload": f"4x4x4 Z2 rank 64, {W} walkers, every {every}th walker proposes ({B} proposals), "
+                               f"{K} candidates each, launches of {L} iterations", "walkers": W, "proposing": B,
+                   "candidates": K, "flop_per_round": flop},
+    }
+    print(json.dumps(out))
+
+
 def workload_c1(a):
     """2x2x2 Z2 from the naive rank-8 scheme, 1024 walkers, plus after a
     10000-flip plateau: wall time to a verified rank-7 scheme (flip budget 1e6
@@ -360,6 +432,9 @@ def main():
     a = parse()
     if a.workload in ("c1", "c3", "c4") and a.gpus == 1:
         {"c1": workload_c1, "c3": workload_c3, "c4": workload_c4}[a.workload](a)
+        return
+    if a.workload == "propose" and a.gpus == 1:
+        workload_propose(a)
         return
     if a.workload == "ttt":
         a.ttt = True

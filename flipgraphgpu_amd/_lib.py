@@ -508,3 +508,10 @@ class Scorer:
         B = _check(self.L.fg_scorer_last_proposals(self.s, wk.ctypes.data, pr.ctypes.data, sc.ctypes.data, cap),
                    "fg_scorer_last_proposals")
         return wk[:B], pr[:B * candidates].reshape(B, candidates, 3), sc[:B * candidates].reshape(B, candidates)
+
+    def last_propose_ms(self):
+        """Duration (ms, HIP events) of the fused proposal kernel of the last round; 0 if it was not fused."""
+        ms = ctypes.c_float()
+        self.L.fg_scorer_last_propose_ms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+        _check(self.L.fg_scorer_last_propose_ms(self.s, ctypes.byref(ms)), "fg_scorer_last_propose_ms")
+        return float(ms.value)

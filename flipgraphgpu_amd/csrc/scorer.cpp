@@ -65,6 +65,9 @@ struct fg_scorer {
     float *fscores = nullptr;
     long capF = 0;
     int lastB = 0, lastK = 0;
+    // HIP events around the fused kernel of the last round, and its duration
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.0f;
     int step = 0;
 
     // workspace (capacity in tuples and pairs; backward buffers on demand)
@@ -91,6 +94,8 @@ struct fg_scorer {
     }
     ~fg_scorer() {
         free_work();
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
         void *ps[] = {w, g, m1, m2, wb, wt, fmap, fpairs, ftup, fscores};
         for (void *p : ps)
             if (p) (void)hipFree(p);
@@ -499,13 +504,24 @@ int propose_fused(fg_scorer *s, const fgrt::PoolView &v, int candidates, int eve
     a.bo = s->w + s->obo;
     a.scores = s->fscores;
     a.forced = v.forced;
+    if (!s->ev0) SCHK(hipEventCreate(&s->ev0));
+    if (!s->ev1) SCHK(hipEventCreate(&s->ev1));
+    SCHK(hipEventRecord(s->ev0, s->stream));
     SCHK(fgk::sc::launch_propose(a, s->stream));
+    SCHK(hipEventRecord(s->ev1, s->stream));
     SCHK(hipStreamSynchronize(s->stream));
+    SCHK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
     s->lastB = L;
     s->lastK = candidates;
     return 0;
 }
 }  // namespace
+
+extern "C" int fg_scorer_last_propose_ms(fg_scorer *s, float *ms) {
+    if (!s || !ms) return sfail("fg_scorer_last_propose_ms: bad arguments");
+    *ms = s->lastB ? s->last_ms : 0.0f;
+    return 0;
+}
 
 extern "C" int fg_scorer_last_proposals(fg_scorer *s, int32_t *walkers, int32_t *pairs, float *scores, int64_t cap) {
     if (!s) return sfail("fg_scorer_last_proposals: null scorer");

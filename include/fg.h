@@ -259,6 +259,10 @@ FG_API int fg_scorer_propose(fg_scorer *s, fg_ctx *ctx, int candidates, int ever
  * [tuples][candidates] (-1e30 for a missing candidate).  Returns the number of
  * tuples (0 when the last round was not fused); any output may be null. */
 FG_API int fg_scorer_last_proposals(fg_scorer *s, int32_t *walkers, int32_t *pairs, float *scores, int64_t cap);
+/* Duration in ms of the fused proposal kernel of the last round (HIP events on
+ * the scorer's stream; 0 when the last round was not fused).  No counterpart in
+ * the reference, which has no learned proposals. */
+FG_API int fg_scorer_last_propose_ms(fg_scorer *s, float *ms);
 /* fg_scorer_propose + fg_walk_forced. */
 FG_API int fg_walk_learned(fg_ctx *ctx, fg_scorer *s, int iterations, int candidates, int every, uint32_t round);
 
