@@ -197,8 +197,10 @@ struct LaneBaseT {
     FGK_HD void rebuild() {
         S0 = S1 = S2 = 0;
         pop = 0;
-        for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < 3; ++c) {
             for (int t = 0; t < m; ++t) pop += popc(f(c, t) & vmask);
+            for (int t = m; t < fs; ++t) f(c, t) = 0;  // the scan's rows past the rank
+        }
         for (int c = 0; c < 3; ++c) {
             for (int q = 0; q < hw; ++q) hword(c, q) = 0;
             for (int t = 0; t < m; ++t) nx(c, t) = 0xFF;
@@ -353,14 +355,15 @@ struct LaneBaseT {
         return true;
     }
 
-    // The last term t < m of component c whose factor word equals v, or -1.
-    // Rows are loaded kScanBatch at a time with every load of a batch issued
-    // before the first compare, so a scan of m <= 32 terms waits for one LDS
-    // round trip (a loop over a run-time m would issue its remainder one
-    // load at a time).  Rows from m to the end of a batch are loaded but not
-    // compared; they lie inside the workgroup's LDS block, since the H and N
-    // planes (at least 6 * 8 rows of 32 words) follow the factor planes.
-    static constexpr int kScanBatch = 32;
+    // The last term t < m of component c whose factor word equals v (v != 0),
+    // or -1.  Rows are loaded kScanBatch at a time with every load of a batch
+    // issued before the first compare, so a scan of m <= 32 terms waits for
+    // one LDS round trip (a loop over a run-time m would issue its remainder
+    // one load at a time).  The rows from m to the end of the plane are zero
+    // (rebuild clears them) and the plane holds whole batches
+    // (lds_term_stride), so rows past m never match and need no rank test:
+    // each row costs a compare and a select of a constant index.
+    static constexpr int kScanBatch = kScanRows;
     FGK_HD int find_last(int c, uint32_t v) const {
         const Word *Fc = F + c * fs * WS;
         int any = -1;
@@ -368,8 +371,19 @@ struct LaneBaseT {
             Word w[kScanBatch];
 #pragma unroll
             for (int e = 0; e < kScanBatch; ++e) w[e] = Fc[(base + e) * WS];
+            // four independent select chains (rows e mod 4), so the compiler
+            // can interleave them; the last match is the largest of the four
+            int l0 = -1, l1 = -1, l2 = -1, l3 = -1;
 #pragma unroll
-            for (int e = 0; e < kScanBatch; ++e) any = (base + e < m && (uint32_t)w[e] == v) ? base + e : any;
+            for (int e = 0; e < kScanBatch; e += 4) {
+                l0 = (uint32_t)w[e] == v ? e : l0;
+                l1 = (uint32_t)w[e + 1] == v ? e + 1 : l1;
+                l2 = (uint32_t)w[e + 2] == v ? e + 2 : l2;
+                l3 = (uint32_t)w[e + 3] == v ? e + 3 : l3;
+            }
+            const int a01 = l0 > l1 ? l0 : l1, a23 = l2 > l3 ? l2 : l3;
+            const int local = a01 > a23 ? a01 : a23;
+            any = local >= 0 ? base + local : any;
         }
         return any;
     }

@@ -83,14 +83,19 @@ constexpr int32_t kErrCapacity = 4; // a term would exceed the LDS capacity
 constexpr int kHTermsPad = 32;
 __host__ __device__ inline int h_words(int max_m) { return ((max_m + kHTermsPad - 1) / kHTermsPad) * (kHTermsPad / 4); }
 
-// LDS term stride of the factor planes F.  The two lanes of a walker scan
-// two different components j != k in step; padding the stride makes their
-// rows fall in different LDS bank groups: with 16-bit words a row of 32
-// walkers covers a quarter of the 64 banks, so the stride is made 1 mod 4
-// (component offsets 1, 2 or 3 quarters apart); with 32-bit words a row
-// covers half the banks and the stride is made odd.
+// Rows of a factor plane compared per batch by the class-member scan.
+constexpr int kScanRows = 32;
+
+// LDS term stride of the factor planes F.  It holds the term capacity
+// rounded up to whole scan batches, so a scan never reads past its own
+// component's rows (the rows from the rank up are kept zero).  The two lanes
+// of a walker scan two different components j != k in step; padding the
+// stride makes their rows fall in different LDS bank groups: with 16-bit
+// words a row of 32 walkers covers a quarter of the 64 banks, so the stride
+// is made 1 mod 4 (component offsets 1, 2 or 3 quarters apart); with 32-bit
+// words a row covers half the banks and the stride is made odd.
 __host__ __device__ inline int lds_term_stride(int max_m, int word_bytes) {
-    int fs = max_m;
+    int fs = (max_m + kScanRows - 1) / kScanRows * kScanRows;
     if (word_bytes == 2) {
         while (fs % 4 != 1) ++fs;
     } else if (fs % 2 == 0) {
