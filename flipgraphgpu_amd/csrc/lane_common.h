@@ -312,38 +312,34 @@ struct LaneBaseT {
         }
         t1 = -1;
         for (int q = 0; q < hw; q += 8) {
-            uint32_t wd[8], sm[8];
+            uint32_t wd[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) wd[e] = hword(c, q + e);
-            uint32_t tot = 0;
+            // running sums P[e] are non-decreasing, so the word holding
+            // position r is the count of P[e] <= r, and the last such P[e]
+            // is what the words before it hold (branch-free, no indexing)
+            uint32_t acc = 0, below = 0;
+            int cnt = 0;
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                sm[e] = byte_sum(wd[e]);
-                tot += sm[e];
+                acc += byte_sum(wd[e]);
+                const bool le = acc <= r;
+                below = le ? acc : below;
+                cnt += le ? 1 : 0;
             }
-            if (r >= tot) {
-                r -= tot;
+            if (cnt == 8) {
+                r -= acc;
                 continue;
             }
-            // first word whose running sum passes r (values selected, no indexing)
-            uint32_t word = 0, acc = 0;
-            int base = -1;
+            r -= below;
+            uint32_t word = wd[0];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const bool here = base < 0 && r < acc + sm[e];
-                word = here ? wd[e] : word;
-                base = here ? (q + e) * 4 : base;
-                r = here ? r - acc : r;
-                acc += sm[e];
-            }
-            for (int b = 0; b < 4; ++b) {
-                const uint32_t hv = (word >> (8 * b)) & 0xFFu;
-                if (r < hv) {
-                    t1 = base + b;
-                    break;
-                }
-                r -= hv;
-            }
+            for (int e = 1; e < 8; ++e) word = cnt == e ? wd[e] : word;
+            // the same within the word's four term bytes
+            const uint32_t c0 = word & 0xFFu, c1 = c0 + ((word >> 8) & 0xFFu), c2 = c1 + ((word >> 16) & 0xFFu);
+            const int b = (r >= c0 ? 1 : 0) + (r >= c1 ? 1 : 0) + (r >= c2 ? 1 : 0);
+            r -= b == 0 ? 0u : (b == 1 ? c0 : (b == 2 ? c1 : c2));
+            t1 = (q + cnt) * 4 + b;
             break;
         }
         if (t1 < 0 || t1 >= m) {
