@@ -37,7 +37,8 @@
 // Randomness: Philox4x32-10 keyed by the seed, counter (iteration, walker,
 // stream).  One block per walk iteration supplies the pair draw, the two
 // orientation coins, the 1% equal-rank best-copy draw and the expand
-// draw; rare operators use the aux stream.  The CPU oracle consumes the
+// draw (the two lanes of a pair compute the blocks of two consecutive
+// iterations and swap them); rare operators use the aux stream.  The CPU oracle consumes the
 // same stream in the same order, so a walker's trajectory is bit-identical.
 #pragma once
 
@@ -660,11 +661,45 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     int min_m = s.m;
     const int log_slot = (kMode == kModeLearn && A.log_every > 0 && w % A.log_every == 0) ? w / A.log_every : -1;
 
+#if defined(__HIP_DEVICE_COMPILE__)
+    // The two lanes of a pair draw the main blocks of two iterations at once
+    // (side 0 the block of `it`, side 1 that of `it + 1`) and swap them, so
+    // each lane runs one Philox per two iterations.
+    uint32_t nb0 = 0, nb1 = 0, nb2 = 0, nb3 = 0;
+    bool have_next = false;
+#endif
     for (int iter = 0; iter < trips; ++iter) {
         bool flipped = false, copy = false;
         uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
         if (iter < iters) {
+#if defined(__HIP_DEVICE_COMPILE__)
+            if (have_next) {
+                b0 = nb0;
+                b1 = nb1;
+                b2 = nb2;
+                b3 = nb3;
+                have_next = false;
+            } else {
+                const uint64_t own = it + (uint64_t)s.side;
+                uint32_t x0, x1, x2, x3;
+                philox10((uint32_t)own, (uint32_t)(own >> 32), (uint32_t)w, 0u, s.k0, s.k1, x0, x1, x2, x3);
+                const uint32_t y0 = (uint32_t)pair_exchange((int)x0, s.side),
+                               y1 = (uint32_t)pair_exchange((int)x1, s.side),
+                               y2 = (uint32_t)pair_exchange((int)x2, s.side),
+                               y3 = (uint32_t)pair_exchange((int)x3, s.side);
+                b0 = s.side ? y0 : x0;
+                b1 = s.side ? y1 : x1;
+                b2 = s.side ? y2 : x2;
+                b3 = s.side ? y3 : x3;
+                nb0 = s.side ? x0 : y0;
+                nb1 = s.side ? x1 : y1;
+                nb2 = s.side ? x2 : y2;
+                nb3 = s.side ? x3 : y3;
+                have_next = true;
+            }
+#else
             philox10((uint32_t)it, (uint32_t)(it >> 32), (uint32_t)w, 0u, s.k0, s.k1, b0, b1, b2, b3);
+#endif
             ++it;
             const int rank = s.m;
             bool ok = false;
