@@ -231,15 +231,19 @@ struct LaneBaseT {
     }
 
     // Term x leaves its component-c class (its factor is about to change).
+    // Each step issues both loads of a member before using them and writes
+    // H back unconditionally, so a step waits for one LDS round trip.
     FGK_HD void leave(int c, int x) {
         int y = nx(c, x);
         if (y == x) return;
         int p = x, cnt = 0;
         while (y != x) {
-            if (y < x) h(c, y) = (uint8_t)(h(c, y) - 1);
+            const int ny = nx(c, y);
+            const uint8_t hy = h(c, y);
+            h(c, y) = (uint8_t)(hy - (y < x ? 1 : 0));
             ++cnt;
             p = y;
-            y = nx(c, y);
+            y = ny;
             if (cnt > m) {
                 fail(kErrWalk);
                 return;
@@ -253,35 +257,40 @@ struct LaneBaseT {
 
     // Walk the component-c class containing y0 that singleton term x is about
     // to join: raise H of the members below x, collect what the join needs,
-    // and let `rule(y)` test each member for a reduction (0 = none); the
-    // lowest member with a rule is returned in (best_t, best_rule).  H is
-    // touched before the caller knows whether a reduction follows, which is
-    // harmless: every reduction rebuilds the classes.
+    // and let `rule(a, b)` test each member for a reduction from its factors
+    // a = f(ca, y) and b = f(cb, y) (0 = none); the lowest member with a rule
+    // is returned in (best_t, best_rule).  Every load of a member is issued
+    // before the first use and H is written back unconditionally, so a step
+    // waits for one LDS round trip.  H is touched before the caller knows
+    // whether a reduction follows, which is harmless: every reduction
+    // rebuilds the classes.
     struct JoinInfo {
         int cnt, gt, p, mx;
     };
     template <class Rule>
-    FGK_HD void class_walk(int c, int x, int y0, JoinInfo &ji, Rule rule, int &best_t, int &best_rule) {
+    FGK_HD void class_walk(int c, int x, int y0, int ca, int cb, JoinInfo &ji, Rule rule, int &best_t,
+                           int &best_rule) {
         int y = y0;
         ji.cnt = 0;
         ji.gt = 0;
         ji.p = -1;
         ji.mx = -1;
         do {
+            const int ny = nx(c, y);
+            const uint8_t hy = h(c, y);
+            const uint32_t a = f(ca, y), b = f(cb, y);
+            const bool below = y < x;
+            h(c, y) = (uint8_t)(hy + (below ? 1 : 0));
             ++ji.cnt;
-            if (y < x) {
-                h(c, y) = (uint8_t)(h(c, y) + 1);
-                if (y > ji.p) ji.p = y;
-            } else {
-                ++ji.gt;
-            }
-            if (y > ji.mx) ji.mx = y;
-            const int r = rule(y);
+            ji.p = below && y > ji.p ? y : ji.p;
+            ji.gt += below ? 0 : 1;
+            ji.mx = y > ji.mx ? y : ji.mx;
+            const int r = rule(a, b);
             if (r && y < best_t) {
                 best_t = y;
                 best_rule = r;
             }
-            y = nx(c, y);
+            y = ny;
             if (ji.cnt > m) {
                 fail(kErrWalk);
                 return;

@@ -161,8 +161,8 @@ struct Z2LaneT : LaneBaseT<WordT> {
         hf.rule = 0;
         hf.any = this->find_last(mc, nv);
         if (hf.any >= 0)
-            class_walk(mc, x, hf.any, hf.ji,
-                       [&](int y) { return !check_reduce ? 0 : (f(i, y) == fi ? 1 : (f(oc, y) == ov ? 2 : 0)); },
+            class_walk(mc, x, hf.any, i, oc, hf.ji,
+                       [&](uint32_t a, uint32_t b) { return !check_reduce ? 0 : (a == fi ? 1 : (b == ov ? 2 : 0)); },
                        hf.t, hf.rule);
         return hf;
     }

@@ -112,12 +112,16 @@ struct ZTLane : LaneBaseT<uint32_t> {
     // x = index2, where `k` is the other changed component): returns
     // 1..4 for the first applicable rule, 0 for none.
     FGK_HD int reduce_rule(int i, int j, int k, int y, uint32_t fix, uint32_t fkx) {
-        const int cmpI = tern::cmp(f(i, y), fix);
-        if (cmpI == 1 && tern::limit_sum(f(k, y), fkx, k != 2)) return 1;
-        if (i == 2 && cmpI == -1 && tern::limit_sub(f(k, y), fkx, true)) return 2;
-        const int cmpK = tern::cmp(f(k, y), fkx);
-        if (cmpK == 1 && tern::limit_sum(f(i, y), fix, i != 2)) return 3;
-        if (k == 2 && cmpK == -1 && tern::limit_sub(f(i, y), fix, true)) return 4;
+        return reduce_rule_v(i, k, f(i, y), f(k, y), fix, fkx);
+    }
+    // The same from term y's factors fiy = f(i, y) and fky = f(k, y).
+    FGK_HD static int reduce_rule_v(int i, int k, uint32_t fiy, uint32_t fky, uint32_t fix, uint32_t fkx) {
+        const int cmpI = tern::cmp(fiy, fix);
+        if (cmpI == 1 && tern::limit_sum(fky, fkx, k != 2)) return 1;
+        if (i == 2 && cmpI == -1 && tern::limit_sub(fky, fkx, true)) return 2;
+        const int cmpK = tern::cmp(fky, fkx);
+        if (cmpK == 1 && tern::limit_sum(fiy, fix, i != 2)) return 3;
+        if (k == 2 && cmpK == -1 && tern::limit_sub(fiy, fix, true)) return 4;
         return 0;
     }
     FGK_HD void apply_rule(int rule, int i, int k, int y, int x) {
@@ -168,8 +172,9 @@ struct ZTLane : LaneBaseT<uint32_t> {
         hf.rule = 0;
         hf.any = this->find_last(mc, nv);
         if (hf.any >= 0)
-            class_walk(mc, x, hf.any, hf.ji,
-                       [&](int y) { return check_reduce ? reduce_rule(i, mc, oc, y, fi, ov) : 0; }, hf.t, hf.rule);
+            class_walk(mc, x, hf.any, i, oc, hf.ji,
+                       [&](uint32_t a, uint32_t b) { return check_reduce ? reduce_rule_v(i, oc, a, b, fi, ov) : 0; },
+                       hf.t, hf.rule);
         return hf;
     }
 
