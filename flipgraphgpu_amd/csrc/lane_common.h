@@ -120,6 +120,7 @@ struct LaneBaseT {
     int visit_cap;  // integer ring: candidates one tryFlip may visit
     uint32_t vmask; // coefficient bits of a factor word (all for Z2, bits 0-15 for the integer ring)
     int pop;        // coefficient count of the scheme, kept for getComplexity
+    bool track_pop; // keep pop current through flips (the minimizer reads it every flip)
     int sel_c, sel_t1, sel_t2;  // pair of the last tryFlip (component, index1 < index2)
 
     // Extra per-lane LDS beyond the common planes (none by default).
@@ -149,7 +150,12 @@ struct LaneBaseT {
     FGK_HD static int popc(uint32_t x) { return __builtin_popcount(x); }
     // coefficient count change of a flip that replaces factors a0 -> a1 and b0 -> b1
     FGK_HD void pop_update(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1) {
-        pop += popc(a1 & vmask) + popc(b1 & vmask) - popc(a0 & vmask) - popc(b0 & vmask);
+        if (track_pop) pop += popc(a1 & vmask) + popc(b1 & vmask) - popc(a0 & vmask) - popc(b0 & vmask);
+    }
+    FGK_HD void recount_pop() {
+        pop = 0;
+        for (int c = 0; c < 3; ++c)
+            for (int t = 0; t < m; ++t) pop += popc(f(c, t) & vmask);
     }
 
     // ---- aux RNG stream ----
@@ -356,8 +362,8 @@ struct LaneBaseT {
             fail(kErrSelect);
             return false;
         }
-        t2 = t1;
-        for (uint32_t q = 0; q <= r; ++q) t2 = nx(c, t2);
+        t2 = nx(c, t1);
+        for (uint32_t q = 1; q <= r; ++q) t2 = nx(c, t2);
         return true;
     }
 
@@ -561,6 +567,7 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     s.visit_cap = A.zt_visit_cap;
     s.vmask = Lane::kValueMask;
     s.pop = 0;
+    s.track_pop = kMode == kModeMinimize;
     int best_rank = 0, best_m = 0, flips_count = 0;
     uint64_t it = 0, total_flips = 0, sum_m = 0;
     s.aux = 0;
@@ -774,7 +781,10 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
         A.aux_idx[w] = s.aidx;
         A.total_flips[w] = total_flips;
         A.sum_m[w] = sum_m;
-        if (A.cx) A.cx[w] = s.complexity();
+        if (A.cx) {
+            if (!s.track_pop) s.recount_pop();
+            A.cx[w] = s.complexity();
+        }
         if constexpr (kMode == kModeResize) {
             A.fmt[4 * w] = s.n0;
             A.fmt[4 * w + 1] = s.n1;
