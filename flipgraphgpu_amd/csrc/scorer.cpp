@@ -456,11 +456,18 @@ int propose_fused(fg_scorer *s, const fgrt::PoolView &v, int candidates, int eve
         SCHK(salloc(&s->fscores, (size_t)need));
         s->capF = need;
     }
-    if (!s->wt) SCHK(salloc(&s->wt, (size_t)fgk::sc::kPD_ * (fgk::sc::kProposeMaxF + 2 * fgk::sc::kPD_)));
+    if (!s->wt)
+        SCHK(salloc(&s->wt, (size_t)fgk::sc::kPD_ * (fgk::sc::kProposeMaxF + 2 * fgk::sc::kPD_) +
+                                (size_t)s->Z * 2 * s->D + (size_t)2 * fgk::sc::kPD_ * fgk::sc::kPD_));
     bf16 *W0t = s->wt, *A1t = W0t + (size_t)fgk::sc::kPD_ * fgk::sc::kProposeMaxF, *A2t = A1t + (size_t)fgk::sc::kPD_ * fgk::sc::kPD_;
+    bf16 *Wzt = A2t + (size_t)fgk::sc::kPD_ * fgk::sc::kPD_;
+    bf16 *B1t = Wzt + (size_t)s->Z * 2 * s->D, *B2t = B1t + (size_t)fgk::sc::kPD_ * fgk::sc::kPD_;
     SCHK(fgk::sc::launch_transpose_bf16(s->wb + s->oW0, s->F, s->D, fgk::sc::kProposeMaxF, W0t, s->stream));
     SCHK(fgk::sc::launch_transpose_bf16(s->wb + s->oA1, s->D, s->D, s->D, A1t, s->stream));
     SCHK(fgk::sc::launch_transpose_bf16(s->wb + s->oA2, s->D, s->D, s->D, A2t, s->stream));
+    SCHK(fgk::sc::launch_transpose_bf16(s->wb + s->oWz, 2 * s->D, s->Z, 2 * s->D, Wzt, s->stream));
+    SCHK(fgk::sc::launch_transpose_bf16(s->wb + s->oB1, s->D, s->D, s->D, B1t, s->stream));
+    SCHK(fgk::sc::launch_transpose_bf16(s->wb + s->oB2, s->D, s->D, s->D, B2t, s->stream));
     std::vector<int32_t> map(L);
     for (int q = 0; q < L; ++q) map[q] = q * every;
     SCHK(hipMemcpyAsync(s->fmap, map.data(), (size_t)L * 4, hipMemcpyHostToDevice, s->stream));
@@ -493,9 +500,10 @@ int propose_fused(fg_scorer *s, const fgrt::PoolView &v, int candidates, int eve
     a.W0t = W0t;
     a.A1t = A1t;
     a.A2t = A2t;
-    a.B1 = s->wb + s->oB1;
-    a.B2 = s->wb + s->oB2;
+    a.B1t = B1t;
+    a.B2t = B2t;
     a.Wz = s->wb + s->oWz;
+    a.Wzt = Wzt;
     a.b0 = s->w + s->ob0;
     a.c1 = s->w + s->oc1;
     a.c2 = s->w + s->oc2;

@@ -128,8 +128,9 @@ struct ProposeArgs {
     const int32_t *pairs;   // [B * K][3] candidates (component -1: none)
     const bf16 *W0t;        // [128][kProposeMaxF]: W0 transposed, zero beyond F
     const bf16 *A1t, *A2t;  // [128][128]: A1, A2 transposed
-    const bf16 *B1, *B2;    // [128][128] as stored (k, d)
+    const bf16 *B1t, *B2t;  // [128][128]: B1, B2 transposed
     const bf16 *Wz;         // [256][64] as stored
+    const bf16 *Wzt;        // [64][256]: Wz transposed
     const float *b0, *c1, *c2, *bz, *wo, *bo;
     float *scores;          // [B * K] (may be null)
     int32_t *forced;        // [W][4] proposals

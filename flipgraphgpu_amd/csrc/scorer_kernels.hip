@@ -410,7 +410,9 @@ constexpr int kPZ = 64;                          // pair-head width
 constexpr int kPXs = kProposeMaxF + 8;           // LDS row stride of the features (bf16)
 constexpr int kPEs = kPD + 8;                    // LDS row stride of the activations (bf16)
 constexpr int kPSmem = kProposeMaxTerms * kPD * 4;  // G in FP32; aliases the feature and activation tiles
+constexpr int kPQs = 2 * kPD + 8;                // LDS row stride of the pair-head rows (bf16)
 static_assert(kProposeMaxTerms * (kPXs + kPEs) * 2 <= kPSmem, "feature + activation tiles must fit the G tile");
+static_assert(16 * kPQs * 2 <= kPSmem && kProposeMaxK <= 16, "pair-head rows must fit the G tile");
 
 __device__ __forceinline__ float bf16r(float x) { return (float)(bf16)x; }
 
@@ -433,14 +435,32 @@ __device__ __forceinline__ void propose_mma(floatx4 (&acc)[4][2], const bf16 *A,
     }
 }
 
-// q[col] = sum_k mean[k] B[k][col] for this wave's 32 columns (two lanes per
-// column, each half of k), written to Qv.
-__device__ __forceinline__ void propose_mean_product(const bf16 *Mv, const bf16 *Bm, float *Qv, int wave, int lane) {
-    const int col = 32 * wave + (lane & 31), half = lane >> 5;
-    float q = 0.f;
-    for (int k = 64 * half; k < 64 * half + 64; ++k) q += (float)Mv[k] * (float)Bm[k * kPD + col];
-    q += __shfl_xor(q, 32);
-    if (lane < 32) Qv[col] = q;
+// q[col] = sum_k mean[k] B[k][col] for this wave's 32 columns, on the MFMA
+// units: the mean is row 0 of an otherwise zero 16-row A tile, Bt is B
+// transposed [col][k]; row 0 of the result sits in lanes 0-15 (element 0).
+__device__ __forceinline__ void propose_mean_product(const bf16 *Mv, const bf16 *Bt, float *Qv, int wave, int lane) {
+    floatx4 q[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+    const bf16 zero = (bf16)0.0f;
+#pragma unroll
+    for (int k0 = 0; k0 < kPD; k0 += 32) {
+        bf16x8 af;
+        if ((lane & 15) == 0) {
+            af = *reinterpret_cast<const bf16x8 *>(Mv + k0 + 8 * (lane >> 4));
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) af[e] = zero;
+        }
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            const bf16x8 bfv =
+                *reinterpret_cast<const bf16x8 *>(Bt + (long)(32 * wave + 16 * ct + (lane & 15)) * kPD + k0 + 8 * (lane >> 4));
+            q[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfv, q[ct], 0, 0, 0);
+        }
+    }
+    if (lane < 16) {
+        Qv[32 * wave + lane] = q[0][0];
+        Qv[32 * wave + 16 + lane] = q[1][0];
+    }
 }
 }  // namespace
 
@@ -498,9 +518,11 @@ __global__ void __launch_bounds__(256) propose_kernel(ProposeArgs a) {
     };
     float v[4][2][4];
 
-    // e = relu(x W0 + b0)
+    // e = relu(x W0 + b0); the K loop stops at the factor bits rounded up to
+    // the MFMA's K step (the feature and weight columns beyond them are zero)
     clear();
-    propose_mma(acc, Xs, kPXs, a.W0t, kProposeMaxF, kProposeMaxF, wave, lane);
+    const int kF = (a.nn0 + a.nn1 + a.nn2 + 31) & ~31;
+    propose_mma(acc, Xs, kPXs, a.W0t, kProposeMaxF, kF, wave, lane);
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
@@ -524,7 +546,7 @@ __global__ void __launch_bounds__(256) propose_kernel(ProposeArgs a) {
     // h = relu(e A1 + mean(e) B1 + c1), then g = relu(h A2 + mean(h) B2 + c2)
     for (int layer = 0; layer < 2; ++layer) {
         const bf16 *At = layer ? a.A2t : a.A1t;
-        const bf16 *Bm = layer ? a.B2 : a.B1;
+        const bf16 *Bm = layer ? a.B2t : a.B1t;
         const float *cb = layer ? a.c2 : a.c1;
         propose_mean_product(Mv, Bm, Qv, wave, lane);
         clear();
@@ -561,35 +583,67 @@ __global__ void __launch_bounds__(256) propose_kernel(ProposeArgs a) {
         __syncthreads();
     }
 
-    // pair head: z = relu([g_a + g_b, g_a * g_b] Wz + bz), s = z . wo[c] + bo[c];
-    // 8 candidates per pass, 32 lanes per candidate, two outputs per lane
-    for (int k0 = 0; k0 < a.K; k0 += 8) {
-        const int k = k0 + tid / 32, jj = tid % 32;
+    // pair head: z = relu([g_a + g_b, g_a * g_b] Wz + bz), s = z . wo[c] + bo[c],
+    // up to 16 candidates at once.  The candidate rows q = [bf16(g_a + g_b),
+    // bf16(g_a * g_b)] are built from G (thread: candidate tid / 16, hidden
+    // units 8 (tid % 16) .. + 7), then written as a [16][256] bf16 tile over
+    // G; each wave multiplies it by its 16 columns of Wz on the MFMA units.
+    bf16 *Qs = reinterpret_cast<bf16 *>(smem);  // [16][kPQs], after G is read
+    {
+        const int k = tid / 16, i0 = 8 * (tid % 16);
         const long p = (long)b * a.K + k;
         const int c = k < a.K ? a.pairs[3 * p] : -1;
-        float part = 0.f;
+        bf16x8 qs, qm;
         if (c >= 0) {
             const int t1 = a.pairs[3 * p + 1], t2 = a.pairs[3 * p + 2];
-            float z0 = 0.f, z1 = 0.f;
-            for (int i = 0; i < kPD; ++i) {
-                const float x = Gs[t1 * kPD + i], y = Gs[t2 * kPD + i];
-                const float qs = bf16r(x + y), qm = bf16r(x * y);
-                z0 += qs * (float)a.Wz[i * kPZ + 2 * jj] + qm * (float)a.Wz[(kPD + i) * kPZ + 2 * jj];
-                z1 += qs * (float)a.Wz[i * kPZ + 2 * jj + 1] + qm * (float)a.Wz[(kPD + i) * kPZ + 2 * jj + 1];
-            }
-            z0 += a.bz[2 * jj];
-            z1 += a.bz[2 * jj + 1];
-            z0 = z0 > 0.f ? z0 : 0.f;
-            z1 = z1 > 0.f ? z1 : 0.f;
-            part = z0 * a.wo[c * kPZ + 2 * jj] + z1 * a.wo[c * kPZ + 2 * jj + 1];
-        }
 #pragma unroll
-        for (int d = 16; d > 0; d >>= 1) part += __shfl_xor(part, d);
-        if (jj == 0 && k < a.K) {
-            const float s = c >= 0 ? a.bo[c] + part : -1e30f;
-            sc[k] = s;
-            if (a.scores) a.scores[p] = s;
+            for (int e = 0; e < 8; ++e) {
+                const float x = Gs[t1 * kPD + i0 + e], y = Gs[t2 * kPD + i0 + e];
+                qs[e] = (bf16)(x + y);
+                qm[e] = (bf16)(x * y);
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) qs[e] = qm[e] = zero;
         }
+        if (tid < kProposeMaxK) sc[tid] = 0.f;
+        __syncthreads();  // every G read is done
+        *reinterpret_cast<bf16x8 *>(Qs + k * kPQs + i0) = qs;
+        *reinterpret_cast<bf16x8 *>(Qs + k * kPQs + kPD + i0) = qm;
+        __syncthreads();
+    }
+    {
+        floatx4 z = {0.f, 0.f, 0.f, 0.f};
+        const int col = 16 * wave + (lane & 15);
+#pragma unroll
+        for (int k0 = 0; k0 < 2 * kPD; k0 += 32) {
+            const bf16x8 af = *reinterpret_cast<const bf16x8 *>(Qs + (lane & 15) * kPQs + k0 + 8 * (lane >> 4));
+            const bf16x8 bfv = *reinterpret_cast<const bf16x8 *>(a.Wzt + (long)col * (2 * kPD) + k0 + 8 * (lane >> 4));
+            z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfv, z, 0, 0, 0);
+        }
+        // lane holds candidates 4 (lane / 16) + q of column col
+        float part[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = 4 * (lane >> 4) + q;
+            const int c = k < a.K ? a.pairs[3 * ((long)b * a.K + k)] : -1;
+            float x = z[q] + a.bz[col];
+            x = x > 0.f ? x : 0.f;
+            part[q] = c >= 0 ? x * a.wo[c * kPZ + col] : 0.f;
+#pragma unroll
+            for (int d = 8; d > 0; d >>= 1) part[q] += __shfl_xor(part[q], d);
+        }
+        if ((lane & 15) == 0)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) atomicAdd(&sc[4 * (lane >> 4) + q], part[q]);
+    }
+    __syncthreads();
+    if (tid < a.K) {
+        const long p = (long)b * a.K + tid;
+        const int c = a.pairs[3 * p];
+        const float s = c >= 0 ? a.bo[c] + sc[tid] : -1e30f;
+        sc[tid] = s;
+        if (a.scores) a.scores[p] = s;
     }
     __syncthreads();
     if (tid == 0) {  // the first best valid candidate, as the choose kernel picks it
