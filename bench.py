@@ -115,6 +115,52 @@ def profiled_traffic(path):
     return None
 
 
+def auc(scores, labels):
+    """Area under the ROC curve (Mann-Whitney rank statistic; ties get mean ranks)."""
+    import numpy as np
+    from scipy.stats import rankdata
+
+    pos = labels > 0.5
+    P, N = int(pos.sum()), int((~pos).sum())
+    if P == 0 or N == 0:
+        return float("nan")
+    r = rankdata(scores)
+    return float((r[pos].sum() - P * (P + 1) / 2) / (P * N))
+
+
+def heldout_ranking(a, sc, W, L):
+    """How well the trained scorer ranks flips it has not seen: flips logged by a
+    fresh uniform walk (seed 2000, every 16th walker, the training horizon),
+    scored by the trained network and by its untrained initial state (same
+    seed); AUC against the logged labels."""
+    import numpy as np
+
+    import flipgraphgpu_amd as fg
+
+    e = fg.Engine(W, 2000, max_iterations=L, plus_iterations=a.ttt_plus, expand_probability=0.0)
+    e.init_naive(4, 4, 4)
+    batches = 4
+    e.log_enable(max_batches=batches, every=16, horizon=a.log_horizon)
+    for q in range(batches + a.log_horizon - 1):
+        if e.walk(L):
+            raise RuntimeError("invalid schemes while logging")
+    n = e.log_info()["rows"]
+    words, m, pairs, y = e.log_read(0, n, 64)
+    e.close()
+    sc0 = fg.Scorer(4, 4, 4, 64, seed=7)
+
+    def scores(s):
+        out = []
+        for b in range(0, n, 16384):
+            out.append(s.eval(words[b:b + 16384], m[b:b + 16384], pairs[b:b + 16384])[0])
+        return np.concatenate(out)
+
+    res = {"rows": int(n), "positives": int((y > 0.5).sum()), "auc_trained": auc(scores(sc), y),
+           "auc_untrained": auc(scores(sc0), y)}
+    sc0.close()
+    return res
+
+
 def time_to_target(a):
     """Median wall time over seeds to a verified n=4 Z2 scheme of rank <= target,
     uniform walks vs learned + uniform proposals.  The scorer is trained once,
@@ -173,6 +219,8 @@ def time_to_target(a):
     t_train = time.perf_counter() - t0
     e.close()
     progress(f"[ttt] trained {a.train_steps} steps in {t_train:.1f} s, loss {losses[:20].mean():.4f} -> {losses[-50:].mean():.4f}")
+    heldout = heldout_ranking(a, sc, W, L)
+    progress(f"[ttt] held-out AUC: trained {heldout['auc_trained']:.4f}, untrained {heldout['auc_untrained']:.4f}")
     res = {"uniform": [], "learned": []}
     for seed in range(1, a.seeds + 1):
         for arm, scorer in (("uniform", None), ("learned", sc)):
@@ -196,6 +244,7 @@ def time_to_target(a):
                                f"plus-iterations {a.ttt_plus}; learned arm: every {a.learned_every}th walker's first "
                                f"flip of each launch is the scorer's best of {a.candidates} uniform candidates",
                    "walkers": W, "launch_iterations": L},
+        "heldout": heldout,
         "training": {"logged_flips": info["rows"], "positives": info["positives"],
                      "log_seconds": round(t_log, 2), "train_seconds": round(t_train, 2),
                      "steps": a.train_steps, "batch": a.train_batch,
