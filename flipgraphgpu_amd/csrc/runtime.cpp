@@ -785,12 +785,17 @@ extern "C" int fg_log_read(fg_ctx *ctx, int64_t begin, int64_t count, uint32_t *
     const std::vector<int64_t> &v = ctx->log.valid;
     if (begin < 0 || count < 0 || begin + count > (int64_t)v.size()) return fail("fg_log_read: range out of bounds");
     const size_t wpr = (size_t)3 * ctx->maxm;
-    for (int64_t q = 0; q < count; ++q) {
+    // one copy per array for each run of consecutive log rows
+    for (int64_t q = 0; q < count;) {
         const size_t r = (size_t)v[begin + q];
-        if (words) HIPCHK(hipMemcpy(words + q * wpr, ctx->log.words + r * wpr, wpr * 4, hipMemcpyDeviceToHost));
-        if (m) HIPCHK(hipMemcpy(m + q, ctx->log.m + r, 4, hipMemcpyDeviceToHost));
-        if (pairs) HIPCHK(hipMemcpy(pairs + q * 3, ctx->log.pairs + r * 3, 12, hipMemcpyDeviceToHost));
-        if (labels) HIPCHK(hipMemcpy(labels + q, ctx->log.y + r, 4, hipMemcpyDeviceToHost));
+        int64_t n = 1;
+        while (q + n < count && (size_t)v[begin + q + n] == r + (size_t)n) ++n;
+        if (words)
+            HIPCHK(hipMemcpy(words + q * wpr, ctx->log.words + r * wpr, (size_t)n * wpr * 4, hipMemcpyDeviceToHost));
+        if (m) HIPCHK(hipMemcpy(m + q, ctx->log.m + r, (size_t)n * 4, hipMemcpyDeviceToHost));
+        if (pairs) HIPCHK(hipMemcpy(pairs + q * 3, ctx->log.pairs + r * 3, (size_t)n * 12, hipMemcpyDeviceToHost));
+        if (labels) HIPCHK(hipMemcpy(labels + q, ctx->log.y + r, (size_t)n * 4, hipMemcpyDeviceToHost));
+        q += n;
     }
     return 0;
 }
