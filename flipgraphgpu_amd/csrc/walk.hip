@@ -30,75 +30,82 @@ __device__ __attribute__((always_inline)) void walk_persistent(const WalkArgs &A
     }
 }
 
-__global__ void __launch_bounds__(kThreadsPerGroup) walk_z2_kernel(WalkArgs A) {
+// Walk-family kernels: one wave per workgroup.  With 32-bit factor words a
+// workgroup's LDS block is at least 18.8 KB (factor planes of 33 rows), so at
+// most 8 fit a CU: 2 waves per SIMD.  Telling the compiler so lets it use up
+// to 256 VGPRs (3x3x3 Z2 launch 35.9 -> 33.7 ms, integer ring 29.3 -> 25.3
+// ms); the grid follows the occupancy the runtime queries, registers included.
+#define FGK_WALK_KERNEL __global__ void __launch_bounds__(kThreadsPerGroup) __attribute__((amdgpu_waves_per_eu(1, 2)))
+
+FGK_WALK_KERNEL walk_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::Z2Lane, lane::kModeWalk>(A, lds);
 }
 
-__global__ void __launch_bounds__(kThreadsPerGroup) walk_zt_kernel(WalkArgs A) {
+FGK_WALK_KERNEL walk_zt_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::ZTLane, lane::kModeWalk>(A, lds);
 }
 
 // Z2 with 16-bit factor words in LDS (factors of up to 16 coefficients).
-__global__ void __launch_bounds__(kThreadsPerGroup) walk_z2h_kernel(WalkArgs A) {
+FGK_WALK_KERNEL walk_z2h_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::Z2Lane16, lane::kModeWalk>(A, lds);
 }
 
 // Complexity minimizer (minimizeKernel, src/entities/complexity_minimizer.cu:190-214).
-__global__ void __launch_bounds__(kThreadsPerGroup) minimize_z2_kernel(WalkArgs A) {
+FGK_WALK_KERNEL minimize_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::Z2Lane, lane::kModeMinimize>(A, lds);
 }
 
-__global__ void __launch_bounds__(kThreadsPerGroup) minimize_zt_kernel(WalkArgs A) {
+FGK_WALK_KERNEL minimize_zt_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::ZTLane, lane::kModeMinimize>(A, lds);
 }
 
-__global__ void __launch_bounds__(kThreadsPerGroup) minimize_z2h_kernel(WalkArgs A) {
+FGK_WALK_KERNEL minimize_z2h_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::Z2Lane16, lane::kModeMinimize>(A, lds);
 }
 
 // Scheme flips of the additions reducer (flipSchemesKernel,
 // src/entities/scheme_additions_reducer.cu:448-460), integer ring.
-__global__ void __launch_bounds__(kThreadsPerGroup) flips_zt_kernel(WalkArgs A) {
+FGK_WALK_KERNEL flips_zt_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::ZTLane, lane::kModeFlips>(A, lds);
 }
 
 // resizeKernel (src/entities/flip_graph.cu:575-613): Z2 with 32-bit words, and the integer ring.
-__global__ void __launch_bounds__(kThreadsPerGroup) resize_z2_kernel(WalkArgs A) {
+FGK_WALK_KERNEL resize_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::Z2Lane, lane::kModeResize>(A, lds);
 }
 
-__global__ void __launch_bounds__(kThreadsPerGroup) resize_zt_kernel(WalkArgs A) {
+FGK_WALK_KERNEL resize_zt_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::ZTLane, lane::kModeResize>(A, lds);
 }
 
 // Walk with the sandwiching and basis-change edges: Z2 with 32-bit words, and the
 // integer ring (whose sandwiching is empty in the reference).
-__global__ void __launch_bounds__(kThreadsPerGroup) walk_meta_z2_kernel(WalkArgs A) {
+FGK_WALK_KERNEL walk_meta_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::Z2Lane, lane::kModeMeta>(A, lds);
 }
 
-__global__ void __launch_bounds__(kThreadsPerGroup) walk_meta_zt_kernel(WalkArgs A) {
+FGK_WALK_KERNEL walk_meta_zt_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::ZTLane, lane::kModeMeta>(A, lds);
 }
 
 // Walk with a proposed first flip and trajectory logging (Z2).
-__global__ void __launch_bounds__(kThreadsPerGroup) learn_z2_kernel(WalkArgs A) {
+FGK_WALK_KERNEL learn_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::Z2Lane, lane::kModeLearn>(A, lds);
 }
 
-__global__ void __launch_bounds__(kThreadsPerGroup) learn_z2h_kernel(WalkArgs A) {
+FGK_WALK_KERNEL learn_z2h_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::Z2Lane16, lane::kModeLearn>(A, lds);
 }
