@@ -312,10 +312,47 @@ struct LaneBaseT {
         addS(c, ji.cnt);
     }
 
+    // The term t1 holding position r of component c's H plane, in blocks of
+    // NB words (NB divides hw); r is left as the offset within t1's count.
+    template <int NB>
+    FGK_HD void select_scan(int c, uint32_t &r, int &t1) {
+        for (int q = 0; q < hw; q += NB) {
+            uint32_t wd[NB];
+#pragma unroll
+            for (int e = 0; e < NB; ++e) wd[e] = hword(c, q + e);
+            // running sums P[e] are non-decreasing, so the word holding
+            // position r is the count of P[e] <= r, and the last such P[e]
+            // is what the words before it hold (branch-free, no indexing)
+            uint32_t acc = 0, below = 0;
+            int cnt = 0;
+#pragma unroll
+            for (int e = 0; e < NB; ++e) {
+                acc += byte_sum(wd[e]);
+                const bool le = acc <= r;
+                below = le ? acc : below;
+                cnt += le ? 1 : 0;
+            }
+            if (cnt == NB) {
+                r -= acc;
+                continue;
+            }
+            r -= below;
+            uint32_t word = wd[0];
+#pragma unroll
+            for (int e = 1; e < NB; ++e) word = cnt == e ? wd[e] : word;
+            // the same within the word's four term bytes
+            const uint32_t c0 = word & 0xFFu, c1 = c0 + ((word >> 8) & 0xFFu), c2 = c1 + ((word >> 16) & 0xFFu);
+            const int b = (r >= c0 ? 1 : 0) + (r >= c1 ? 1 : 0) + (r >= c2 ? 1 : 0);
+            r -= b == 0 ? 0u : (b == 1 ? c0 : (b == 2 ? c1 : c2));
+            t1 = (q + cnt) * 4 + b;
+            break;
+        }
+    }
+
     // The unordered pair at position r (< S0 + S1 + S2) of the fresh-list
-    // order: component, then t1, then t2.  H is scanned in blocks of 8 words
-    // (32 terms; the 8 loads are issued together) with byte sums, then t2 is
-    // r' + 1 steps along N from t1.
+    // order: component, then t1, then t2.  H is scanned in blocks of 8 or 16
+    // words (32 or 64 terms; the loads of a block are issued together) with
+    // byte sums, then t2 is r' + 1 steps along N from t1.
     FGK_HD bool select_pair(uint32_t r, int &c, int &t1, int &t2) {
         c = 0;
         if (r >= S0) {
@@ -327,37 +364,10 @@ struct LaneBaseT {
             }
         }
         t1 = -1;
-        for (int q = 0; q < hw; q += 8) {
-            uint32_t wd[8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) wd[e] = hword(c, q + e);
-            // running sums P[e] are non-decreasing, so the word holding
-            // position r is the count of P[e] <= r, and the last such P[e]
-            // is what the words before it hold (branch-free, no indexing)
-            uint32_t acc = 0, below = 0;
-            int cnt = 0;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                acc += byte_sum(wd[e]);
-                const bool le = acc <= r;
-                below = le ? acc : below;
-                cnt += le ? 1 : 0;
-            }
-            if (cnt == 8) {
-                r -= acc;
-                continue;
-            }
-            r -= below;
-            uint32_t word = wd[0];
-#pragma unroll
-            for (int e = 1; e < 8; ++e) word = cnt == e ? wd[e] : word;
-            // the same within the word's four term bytes
-            const uint32_t c0 = word & 0xFFu, c1 = c0 + ((word >> 8) & 0xFFu), c2 = c1 + ((word >> 16) & 0xFFu);
-            const int b = (r >= c0 ? 1 : 0) + (r >= c1 ? 1 : 0) + (r >= c2 ? 1 : 0);
-            r -= b == 0 ? 0u : (b == 1 ? c0 : (b == 2 ? c1 : c2));
-            t1 = (q + cnt) * 4 + b;
-            break;
-        }
+        // planes of 16 H words (64 terms) are scanned in one block, so every
+        // H read of the selection goes out in one round trip
+        if (hw % 16 == 0) select_scan<16>(c, r, t1);
+        else select_scan<8>(c, r, t1);
         if (t1 < 0 || t1 >= m) {
             fail(kErrSelect);
             return false;
