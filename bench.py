@@ -34,6 +34,7 @@ def parse():
     ap.add_argument("--walkers", type=int, default=65536)
     ap.add_argument("--iterations", type=int, default=10000)
     ap.add_argument("--n", type=int, nargs=3, default=[3, 3, 3])
+    ap.add_argument("--ring", default="z2", choices=["z2", "zt"], help="z2: GF(2); zt: integers, coefficients in {-1,0,1}")
     ap.add_argument("--plus-iterations", type=int, default=20000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--word-bits", type=int, default=0, choices=[0, 16, 32],
@@ -503,8 +504,9 @@ def main():
 
     info = fg.device_info()
     n = tuple(a.n)
-    e = fg.Engine(a.walkers, a.seed, max_iterations=a.iterations, plus_iterations=a.plus_iterations,
-                  expand_probability=0.0, word_bits=a.word_bits)
+    zt = a.ring == "zt"
+    e = fg.Engine(a.walkers, a.seed, ring=fg.RING_ZT if zt else fg.RING_Z2, max_iterations=a.iterations,
+                  plus_iterations=a.plus_iterations, expand_probability=0.0, word_bits=0 if zt else a.word_bits)
     e.init_naive(*n)
     for _ in range(a.warmup):
         bad = e.walk(a.iterations)
@@ -522,7 +524,8 @@ def main():
     c1 = e.counters()
     flips = c1["flips"] - c0["flips"]
     sum_m = c1["sum_m"] - c0["sum_m"]
-    ops = 2 * flips + 2 * sum_m
+    # W = 2 + 2m word operations per Z2 flip, 4 + 4m for the integer ring (two words per factor)
+    ops = (4 if zt else 2) * (flips + sum_m)
     recs = e.read_walkers()
     hist = {int(k): int(v) for k, v in zip(*__import__("numpy").unique(recs["best_m"], return_counts=True))}
     _, best_rank, _ = e.best()
@@ -530,11 +533,11 @@ def main():
 
     peak = info["cu"] * info["clock_khz"] * 1e3 * 64
     traffic_src = "profiles/walk_z2_3x3x3_hbm_traffic.md"
-    default_shape = n == (3, 3, 3) and a.walkers == 65536 and a.iterations == 10000
+    default_shape = n == (3, 3, 3) and a.walkers == 65536 and a.iterations == 10000 and not zt
     traffic = profiled_traffic(os.path.join(ROOT, traffic_src)) if default_shape else None
     achieved = ops / (kernel_ms / 1e3) if kernel_ms > 0 else 0.0
     out = {
-        "metric": "flips/s (Z2 flip-graph random walk, 3x3x3, 65536 walkers)",
+        "metric": f"flips/s ({'integer-ring' if zt else 'Z2'} flip-graph random walk, {n[0]}x{n[1]}x{n[2]}, {a.walkers} walkers)",
         "value": flips / dt,
         "unit": "flips/s",
         "n_gpus": 1,
@@ -547,7 +550,7 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": f"{n[0]}x{n[1]}x{n[2]} Z2 walk from the naive rank-{n[0] * n[1] * n[2]} scheme, "
+            "workload": f"{n[0]}x{n[1]}x{n[2]} {'integer-ring' if zt else 'Z2'} walk from the naive rank-{n[0] * n[1] * n[2]} scheme, "
                         f"{a.walkers} walkers, {a.iterations} iterations per walker per step, "
                         f"plus-iterations {a.plus_iterations}, Brent check of every scheme after each step",
             "walkers": a.walkers,

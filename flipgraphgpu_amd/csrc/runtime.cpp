@@ -111,7 +111,8 @@ struct fg_ctx {
     int grid = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    int graph_iterations = -1;
+    int graph_iterations = -1;  // launch the captured graph replays: iterations and grid
+    int graph_grid = -1;
 
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -119,6 +120,7 @@ struct fg_ctx {
         graph_exec = nullptr;
         graph = nullptr;
         graph_iterations = -1;
+        graph_grid = -1;
     }
     void release() {
         drop_graph();
@@ -688,7 +690,7 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced, double resize) {
         return verify_planes(ctx, ctx->cfg.ring, ctx->cur, ctx->m, ctx->fmt, ctx->W, ctx->maxm, ctx->ok);
     };
     if (ctx->cfg.use_graph && mode != 2 && mode != 4) {
-        if (ctx->graph_iterations != a.iterations) {
+        if (ctx->graph_iterations != a.iterations || ctx->graph_grid != ctx->grid) {
             ctx->drop_graph();
             HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             const int rc = enqueue(false);
@@ -699,6 +701,7 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced, double resize) {
             ctx->graph = g;
             HIPCHK(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
             ctx->graph_iterations = a.iterations;
+            ctx->graph_grid = ctx->grid;
         }
         HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
         HIPCHK(hipGraphLaunch(ctx->graph_exec, ctx->stream));

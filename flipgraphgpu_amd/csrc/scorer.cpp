@@ -63,7 +63,8 @@ struct fg_scorer {
     bf16 *wt = nullptr;
     int32_t *fmap = nullptr, *fpairs = nullptr, *ftup = nullptr;
     float *fscores = nullptr;
-    long capF = 0;
+    long capF = 0;  // candidate tuples the fused buffers hold
+    long capL = 0;  // proposing walkers fmap holds
     int lastB = 0, lastK = 0;
     // HIP events around the fused kernel of the last round, and its duration
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -443,18 +444,19 @@ namespace {
 int propose_fused(fg_scorer *s, const fgrt::PoolView &v, int candidates, int every, uint32_t round) {
     const int L = (v.W + every - 1) / every;
     const long need = (long)L * candidates;
-    if (need > s->capF || !s->fmap) {
+    if (need > s->capF || L > s->capL || !s->fmap) {
         void *ps[] = {s->fmap, s->fpairs, s->ftup, s->fscores};
         for (void *p : ps)
             if (p) (void)hipFree(p);
         s->fmap = s->fpairs = s->ftup = nullptr;
         s->fscores = nullptr;
-        s->capF = 0;
+        s->capF = s->capL = 0;
         SCHK(salloc(&s->fmap, (size_t)std::max(L, 1)));
         SCHK(salloc(&s->fpairs, (size_t)need * 3));
         SCHK(salloc(&s->ftup, (size_t)need));
         SCHK(salloc(&s->fscores, (size_t)need));
         s->capF = need;
+        s->capL = L;
     }
     if (!s->wt)
         SCHK(salloc(&s->wt, (size_t)fgk::sc::kPD_ * (fgk::sc::kProposeMaxF + 2 * fgk::sc::kPD_) +

@@ -116,8 +116,9 @@ __host__ __device__ inline size_t walk_lds_base_bytes(int max_m, int word_bytes 
 }
 __host__ __device__ inline size_t z2_lds_bytes(int max_m, int word_bytes = 4) { return walk_lds_base_bytes(max_m, word_bytes); }
 
-// Integer ring: the candidate pairs one tryFlip visits are remembered in
-// LDS (16-bit indices, kZtMaxVisits per walker).
+// Integer ring: the candidate pairs one tryFlip visits one at a time
+// (before its exhaustive pass) are remembered in LDS (16-bit positions,
+// kZtMaxVisits per walker, kept ascending).
 constexpr int kZtMaxVisits = 16;
 __host__ __device__ inline size_t zt_lds_bytes(int max_m) {
     return walk_lds_base_bytes(max_m, 4) + (size_t)kWalkersPerGroup * kZtMaxVisits * 2;

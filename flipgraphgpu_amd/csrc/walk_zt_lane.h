@@ -357,50 +357,104 @@ struct ZTLane : LaneBaseT<uint32_t> {
         return false;
     }
 
-    // tryFlip (scheme_integer.cu:907-975): candidates are visited in a
-    // uniformly random order without repetition (each visit draws a
-    // candidate and redraws while it was already visited, at most
-    // kZtRedraws times); at most min(visit_cap, kZtMaxVisits) are visited and
-    // the first admissible one is applied.  Visit 0 takes r and coin from the
-    // main block, later visits draw both from the aux stream.
-    static constexpr int kZtRedraws = 64;
+    // The two admissibility tests of candidate (component c, t1 < t2) under
+    // the orientation coins cn (scheme_integer.cu:944-971); on success op
+    // holds the flip to apply: i, j, k, index1, index2.
+    struct ZtOp {
+        int i, j, k, a, b;
+    };
+    FGK_HD bool zt_candidate(int c, int t1, int t2, uint32_t cn, ZtOp &op) {
+        int j, k;
+        orientation(c, cn, j, k);
+        if (cn & 2u) {
+            const int t = t1;
+            t1 = t2;
+            t2 = t;
+        }
+        const uint32_t j1 = f(j, t1), j2 = f(j, t2), k1 = f(k, t1), k2 = f(k, t2);
+        if (tern::limit_sum(j1, j2, j != 2) && tern::limit_sub(k2, k1, false)) {
+            const bool fwd = k == 2 || tern::pos_sub(k2, k1);
+            op = {c, j, k, fwd ? t1 : t2, fwd ? t2 : t1};
+            return true;
+        }
+        if (tern::limit_sum(k1, k2, k != 2) && tern::limit_sub(j2, j1, false)) {
+            const bool fwd = j == 2 || tern::pos_sub(j2, j1);
+            op = {c, k, j, fwd ? t1 : t2, fwd ? t2 : t1};
+            return true;
+        }
+        return false;
+    }
+
+    // tryFlip (scheme_integer.cu:907-975).  The reference visits all
+    // candidates in a uniformly random order, each with two fresh coins, and
+    // applies the first admissible one; it fails only when none is.  Phase 1
+    // visits up to min(visit_cap, kZtMaxVisits) candidates one at a time:
+    // visit q draws k uniform in [0, total - q) (visit 0 from the main block
+    // r, later visits from the aux stream) and takes the k-th candidate not
+    // yet visited (the visited list is kept ascending), with coins from the
+    // main block (visit 0) or the aux stream.  If all fail and unvisited
+    // candidates remain, phase 2 draws one coin word per unvisited candidate
+    // (fresh-list order) and picks uniformly among those admissible under
+    // their coins by reservoir sampling (a draw per admissible candidate
+    // after the first).  The first admissible candidate of a uniform order
+    // with independent coins is a uniform pick among the candidates
+    // admissible under their coins, so both phases follow the reference's
+    // distribution.
     FGK_HD bool try_flip(uint32_t r, uint32_t coin, bool check_reduce) {
         const uint32_t total = S0 + S1 + S2;
         if (!total) return false;
-        int cap = visit_cap < (int)total ? visit_cap : (int)total;
-        if (cap > kZtMaxVisits) cap = kZtMaxVisits;
+        if (total > 0xFFFFu) {  // the visited list holds 16-bit positions
+            fail(kErrCapacity);
+            return false;
+        }
+        int cap = visit_cap > 0 && visit_cap < kZtMaxVisits ? visit_cap : kZtMaxVisits;
+        if ((uint32_t)cap > total) cap = (int)total;
+        ZtOp op;
+        int nv = 0;
         for (int q = 0; q < cap; ++q) {
-            uint32_t cand = draw_below(q == 0 ? r : aux32(), total);
-            for (int redraws = 0;;) {
-                bool seen = false;
-                for (int o = 0; o < q; ++o) seen |= visited(o) == cand;
-                if (!seen) break;
-                if (++redraws > kZtRedraws) return false;
-                cand = draw_below(aux32(), total);
-            }
-            visited(q) = (uint16_t)cand;
+            uint32_t cand = draw_below(q == 0 ? r : aux32(), total - (uint32_t)q);
+            for (int o = 0; o < nv; ++o) cand += visited(o) <= cand ? 1u : 0u;
+            int pos = nv++;
+            for (; pos > 0 && visited(pos - 1) > cand; --pos) visited(pos) = visited(pos - 1);
+            visited(pos) = (uint16_t)cand;
             const uint32_t cn = q == 0 ? coin : aux32();
-            int c, t1, t2, j, k;
+            int c, t1, t2;
             if (!select_pair(cand, c, t1, t2)) return false;
-            orientation(c, cn, j, k);
-            if (cn & 2u) {
-                const int t = t1;
-                t1 = t2;
-                t2 = t;
-            }
-            const uint32_t j1 = f(j, t1), j2 = f(j, t2), k1 = f(k, t1), k2 = f(k, t2);
-            if (tern::limit_sum(j1, j2, j != 2) && tern::limit_sub(k2, k1, false)) {
-                if (k == 2 || tern::pos_sub(k2, k1)) flip(c, j, k, t1, t2, check_reduce);
-                else flip(c, j, k, t2, t1, check_reduce);
-                return true;
-            }
-            if (tern::limit_sum(k1, k2, k != 2) && tern::limit_sub(j2, j1, false)) {
-                if (j == 2 || tern::pos_sub(j2, j1)) flip(c, k, j, t1, t2, check_reduce);
-                else flip(c, k, j, t2, t1, check_reduce);
+            if (zt_candidate(c, t1, t2, cn, op)) {
+                flip(op.i, op.j, op.k, op.a, op.b, check_reduce);
                 return true;
             }
         }
-        return false;
+        if ((uint32_t)nv >= total) return false;
+        int n = 0, pv = 0, bc = 0, bt1 = 0, bt2 = 0;
+        uint32_t bcoin = 0, idx = 0;
+        for (int c = 0; c < 3; ++c) {
+            for (int t1 = 0; t1 < m; ++t1) {
+                const int above = this->h(c, t1);
+                int y = t1;
+                for (int q = 0; q < above; ++q) {
+                    y = nx(c, y);
+                    const uint32_t cur = idx++;
+                    if (pv < nv && visited(pv) == cur) {
+                        ++pv;
+                        continue;
+                    }
+                    const uint32_t cn = aux32();
+                    if (!zt_candidate(c, t1, y, cn, op)) continue;
+                    ++n;
+                    if (n == 1 || draw_below(aux32(), (uint32_t)n) == 0) {
+                        bc = c;
+                        bt1 = t1;
+                        bt2 = y;
+                        bcoin = cn;
+                    }
+                }
+            }
+        }
+        if (!n) return false;
+        zt_candidate(bc, bt1, bt2, bcoin, op);
+        flip(op.i, op.j, op.k, op.a, op.b, check_reduce);
+        return true;
     }
 };
 

@@ -41,7 +41,7 @@ typedef struct {
     double reduce_probability;  /* --reduce-probability (divided by max iterations) */
     uint64_t seed;              /* --seed (0 is replaced by time(0) in the CLI, not here) */
     int32_t max_rank;           /* term capacity per walker; 0 = max(naive rank, input ranks) */
-    int32_t zt_visit_cap;       /* integer ring: candidates one tryFlip may visit (0 = 16) */
+    int32_t zt_visit_cap;       /* integer ring: candidates one tryFlip visits one at a time before its exhaustive pass (0 = 16, at most 16) */
     int32_t use_graph;          /* 1: replay each launch (queue reset, walk, Brent check) as a captured HIP graph */
     int32_t grid;               /* persistent workgroups per launch; 0 = all that fit on the GPU at once */
     int32_t word_bits;          /* Z2 LDS factor words: 16 or 32; 0 = 16 only when it lets more walker blocks run at once */

@@ -593,53 +593,126 @@ static int try_flip_pair_z2(fgo_scheme *s, int c, int t1, int t2, uint32_t coin,
     return 1;
 }
 
-/* Integer tryFlip (scheme_integer.cu:907-975).  The reference visits every
- * candidate pair in a uniformly random order (a Fisher-Yates permutation of
- * all of them, drawn up front) and applies the first admissible one.  Here
- * the same visiting order is drawn lazily: each visit draws a candidate
- * uniformly and redraws while it was already visited (at most
- * FGO_ZT_REDRAWS redraws), so only visited candidates cost draws; at most
- * min(zt_visit_cap, FGO_ZT_MAX_VISITS) candidates are visited.  Visit 0
- * takes the main block (b[0] candidate, b[1] coins); later visits draw the
- * candidate and then the coins from the aux stream. */
+/* Integer tryFlip (scheme_integer.cu:907-975).  The reference draws a
+ * Fisher-Yates permutation of all candidate pairs up front
+ * (random.cuh:9-22), draws two orientation coins per visited candidate and
+ * applies the first candidate whose coins make it admissible; it fails only
+ * when every candidate, with its coins, is inadmissible.  REFERENCE mode
+ * does exactly that (stored-list order, one aux word per permutation step,
+ * one per coin).
+ *
+ * CANONICAL mode (the GPU's rule) draws the same process lazily.
+ * Phase 1 visits up to min(zt_visit_cap, FGO_ZT_MAX_VISITS) candidates one
+ * at a time: visit q draws k uniform in [0, size - q) (visit 0 from the main
+ * block b[0], later visits from the aux stream) and takes the k-th candidate
+ * not yet visited, so it is uniform over the unvisited ones; its coins are
+ * b[1] for visit 0, an aux word later (bit 0 swaps j/k, bit 1 swaps the
+ * indices).  When every visited candidate is inadmissible and unvisited ones
+ * remain, phase 2 gives each unvisited candidate (fresh-list order) its own
+ * coin word from the aux stream and picks uniformly among the admissible
+ * ones by reservoir sampling (one aux word per admissible candidate after
+ * the first).  The first admissible candidate of a uniformly random order,
+ * each candidate with independent coins, is a uniform pick among the
+ * candidates admissible under their coins, so both phases follow the
+ * reference's distribution, and tryFlip fails exactly when no unvisited
+ * candidate is admissible under its coins. */
 #define FGO_ZT_MAX_VISITS 16
-#define FGO_ZT_REDRAWS 64
-static int try_flip_zt(fgo_scheme *s, const uint32_t b[4], fgo_rng *r, const fgo_walk_params *p) {
-    int size = total_pairs(s, p->select_mode);
+
+/* The two admissibility tests of one candidate (component c, indices t1 <
+ * t2) under coin word `coin` (scheme_integer.cu:944-971); on success op
+ * holds the flip (i, j, k, index1, index2) to apply. */
+static int zt_candidate(const fgo_scheme *s, int c, int t1, int t2, uint32_t coin, int op[5]) {
+    int i, j, k;
+    pick_orientation(c, (int)coin, &i, &j, &k);
+    if (coin & 2) { int t = t1; t1 = t2; t2 = t; }
+    tv j1 = tv_get(s, j, t1), j2 = tv_get(s, j, t2), k1 = tv_get(s, k, t1), k2 = tv_get(s, k, t2);
+    if (tv_limit_sum(j1, j2, j != 2) && tv_limit_sub(k2, k1, 0)) {
+        int fwd = k == 2 || tv_pos_sub(k2, k1);
+        op[0] = i; op[1] = j; op[2] = k; op[3] = fwd ? t1 : t2; op[4] = fwd ? t2 : t1;
+        return 1;
+    }
+    if (tv_limit_sum(k1, k2, k != 2) && tv_limit_sub(j2, j1, 0)) {
+        int fwd = j == 2 || tv_pos_sub(j2, j1);
+        op[0] = i; op[1] = k; op[2] = j; op[3] = fwd ? t1 : t2; op[4] = fwd ? t2 : t1;
+        return 1;
+    }
+    return 0;
+}
+
+/* flip of the last integer tryFlip of this thread (i, j, k, index1, index2) */
+static __thread int tl_zt_op[5];
+
+static void apply_zt(fgo_scheme *s, const int op[5], int check_reduce) {
+    for (int q = 0; q < 5; q++) tl_zt_op[q] = op[q];
+    flip_zt(s, op[0], op[1], op[2], op[3], op[4], check_reduce);
+}
+
+/* REFERENCE mode: the reference's permutation, in stored-list order. */
+static int try_flip_zt_reference(fgo_scheme *s, fgo_rng *r, int check_reduce) {
+    int size = total_pairs(s, FGO_SELECT_REFERENCE);
     if (!size) return 0;
-    int cap = p->zt_visit_cap < size ? p->zt_visit_cap : size;
-    if (cap > FGO_ZT_MAX_VISITS) cap = FGO_ZT_MAX_VISITS;
-    int visited[FGO_ZT_MAX_VISITS];
-    int nv = 0;
+    int *perm = (int *)malloc(sizeof(int) * (size_t)size);
+    for (int q = 0; q < size; q++) perm[q] = q;
+    for (int q = size - 1; q > 0; q--) { /* randomPermutation, random.cuh:14-21 */
+        int x = (int)(fgo_rng_aux32(r) % (uint32_t)(q + 1));
+        int t = perm[q]; perm[q] = perm[x]; perm[x] = t;
+    }
+    int ok = 0, op[5];
+    for (int q = 0; q < size && !ok; q++) {
+        int c, t1, t2;
+        decode_reference(s, perm[q], &c, &t1, &t2);
+        uint32_t coin = fgo_rng_aux32(r) % 2u;           /* :944-948 */
+        coin |= (fgo_rng_aux32(r) % 2u) << 1;            /* :950-954 */
+        ok = zt_candidate(s, c, t1, t2, coin, op);
+    }
+    free(perm);
+    if (ok) apply_zt(s, op, check_reduce);
+    return ok;
+}
+
+static int try_flip_zt(fgo_scheme *s, const uint32_t b[4], fgo_rng *r, const fgo_walk_params *p) {
+    if (p->select_mode == FGO_SELECT_REFERENCE) return try_flip_zt_reference(s, r, p->check_reduce);
+    int size = fgo_count_pairs(s);
+    if (!size) return 0;
+    int cap = p->zt_visit_cap > 0 && p->zt_visit_cap < FGO_ZT_MAX_VISITS ? p->zt_visit_cap : FGO_ZT_MAX_VISITS;
+    if (cap > size) cap = size;
+    int visited[FGO_ZT_MAX_VISITS]; /* ascending */
+    int nv = 0, op[5], c, t1, t2;
     for (int q = 0; q < cap; q++) {
-        int cand = (int)draw_below(q == 0 ? b[0] : fgo_rng_aux32(r), (uint32_t)size);
-        int redraws = 0;
-        for (;;) {
-            int seen = 0;
-            for (int o = 0; o < nv; o++) seen |= visited[o] == cand;
-            if (!seen) break;
-            if (++redraws > FGO_ZT_REDRAWS) return 0;
-            cand = (int)draw_below(fgo_rng_aux32(r), (uint32_t)size);
-        }
-        visited[nv++] = cand;
+        int cand = (int)draw_below(q == 0 ? b[0] : fgo_rng_aux32(r), (uint32_t)(size - q));
+        for (int o = 0; o < nv; o++) cand += visited[o] <= cand;  /* the cand-th unvisited */
+        int pos = nv++;
+        while (pos > 0 && visited[pos - 1] > cand) { visited[pos] = visited[pos - 1]; pos--; }
+        visited[pos] = cand;
         uint32_t coin = q == 0 ? b[1] : fgo_rng_aux32(r);
-        int c, t1, t2, i, j, k;
-        select_pair(s, p->select_mode, (uint32_t)cand, &c, &t1, &t2);
-        pick_orientation(c, (int)coin, &i, &j, &k);
-        if (coin & 2) { int t = t1; t1 = t2; t2 = t; }
-        tv j1 = tv_get(s, j, t1), j2 = tv_get(s, j, t2), k1 = tv_get(s, k, t1), k2 = tv_get(s, k, t2);
-        if (tv_limit_sum(j1, j2, j != 2) && tv_limit_sub(k2, k1, 0)) {
-            if (k == 2 || tv_pos_sub(k2, k1)) flip_zt(s, i, j, k, t1, t2, p->check_reduce);
-            else flip_zt(s, i, j, k, t2, t1, p->check_reduce);
-            return 1;
-        }
-        if (tv_limit_sum(k1, k2, k != 2) && tv_limit_sub(j2, j1, 0)) {
-            if (j == 2 || tv_pos_sub(j2, j1)) flip_zt(s, i, k, j, t1, t2, p->check_reduce);
-            else flip_zt(s, i, k, j, t2, t1, p->check_reduce);
+        fgo_pick_canonical(s, (uint32_t)cand, &c, &t1, &t2);
+        if (zt_candidate(s, c, t1, t2, coin, op)) {
+            apply_zt(s, op, p->check_reduce);
             return 1;
         }
     }
-    return 0;
+    if (nv >= size) return 0;
+    /* phase 2: reservoir pick among the unvisited candidates admissible
+     * under their own coins */
+    int n = 0, pv = 0, idx = 0, bc = 0, bt1 = 0, bt2 = 0;
+    uint32_t bcoin = 0;
+    for (int cc = 0; cc < 3; cc++)
+        for (int a = 0; a < s->m; a++)
+            for (int bb = a + 1; bb < s->m; bb++) {
+                if (!fac_eq(s, cc, a, bb)) continue;
+                int cur = idx++;
+                if (pv < nv && visited[pv] == cur) { pv++; continue; }
+                uint32_t coin = fgo_rng_aux32(r);
+                if (!zt_candidate(s, cc, a, bb, coin, op)) continue;
+                n++;
+                if (n == 1 || draw_below(fgo_rng_aux32(r), (uint32_t)n) == 0) {
+                    bc = cc; bt1 = a; bt2 = bb; bcoin = coin;
+                }
+            }
+    if (!n) return 0;
+    zt_candidate(s, bc, bt1, bt2, bcoin, op);
+    apply_zt(s, op, p->check_reduce);
+    return 1;
 }
 
 /* tryPlus (scheme_z2.cu:1029-1050, scheme_integer.cu:977-998).  The
@@ -1809,4 +1882,31 @@ static void swap_basis(fgo_scheme *s, fgo_rng *r) {
         fgo_swap_basis_columns(s, j1, j2);
     }
     if (s->ring == FGO_RING_ZT && fgo_fix_signs(s)) fgo_init_flips(s);
+}
+
+/* One integer tryFlip on a scheme (tests): the walker's main block `it` of
+ * (seed, walker) supplies the candidate and coin words, its aux stream the
+ * rest.  Returns 1 when a flip was applied (op gets i, j, k, index1, index2),
+ * 0 when none, < 0 on error; out gets the scheme afterwards. */
+int fgo_blob_try_flip_zt(const int32_t *in, uint64_t seed, uint32_t walker, uint64_t it, int select_mode,
+                         int visit_cap, int check_reduce, int32_t *out, int32_t *op) {
+    fgo_scheme *s = (fgo_scheme *)malloc(sizeof(fgo_scheme));
+    if (!s) return -1;
+    if (fgo_from_blob(s, FGO_RING_ZT, in, 0) < 0) { free(s); return -1; }
+    fgo_rng r;
+    fgo_rng_init(&r, seed, walker);
+    r.it = it;
+    uint32_t b[4];
+    fgo_rng_main_block(&r, b);
+    fgo_walk_params p;
+    memset(&p, 0, sizeof p);
+    p.select_mode = select_mode;
+    p.zt_visit_cap = visit_cap;
+    p.check_reduce = check_reduce;
+    for (int q = 0; q < 5; q++) tl_zt_op[q] = -1;
+    int ok = try_flip_zt(s, b, &r, &p);
+    for (int q = 0; q < 5; q++) op[q] = ok ? tl_zt_op[q] : -1;
+    fgo_to_blob(s, out);
+    free(s);
+    return ok;
 }

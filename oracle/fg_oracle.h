@@ -83,7 +83,8 @@ typedef struct {
     uint32_t thr_best_eq;  /* equal-rank best copy, 0.01 * 2^32 */
     int select_mode;       /* FGO_SELECT_* */
     int check_reduce;      /* tryFlip(checkReduce) */
-    int zt_visit_cap;      /* max candidates one integer tryFlip visits (at most 16) */
+    int zt_visit_cap;      /* candidates one integer tryFlip visits one at a time before its
+                              exhaustive pass (at most 16; <= 0: 16) */
     int mode;              /* 0: flip-graph walk; 1: complexity minimizer (best_rank holds the complexity);
                               3: the additions reducer's scheme flips (iterations = max flips) */
     int use_sandwich;      /* sandwiching edge (Z2): P = thr_sandwich / 2^32 per applied flip */
@@ -186,6 +187,8 @@ int fgo_blob_naive(int ring, int n1, int n2, int n3, int32_t *out);
  * 3 reduce(i,i1,i2,sub); 4 tryReduce; 5 fixSigns.  Returns new blob length or -1. */
 int fgo_blob_replay(int ring, const int32_t *in, const int32_t *ops, int nops, int32_t *out);
 int fgo_blob_pairs(int ring, const int32_t *blob, int mode, uint32_t *out, int cap);
+int fgo_blob_try_flip_zt(const int32_t *in, uint64_t seed, uint32_t walker, uint64_t it, int select_mode,
+                         int visit_cap, int check_reduce, int32_t *out, int32_t *op);
 int fgo_check_flipset_walk(int ring, int n1, int n2, int n3, uint64_t seed, int iterations, int walkers,
                            int plus_iterations);
 

@@ -197,21 +197,34 @@ def test_split_keeps_brent():
 
 
 def test_zt_flips_keep_brent_over_integers():
-    blob = S.to_blob(*S.strassen_squared())
+    """Flips admitted by an independent restatement of tryFlip's tests
+    (test_oracle_zt_flip.candidate_ops), applied one at a time from the naive
+    3x3x3 and 2x3x4 integer schemes, keep the scheme exact over Z at every
+    step, with coefficients in {-1, 0, 1} and U, V rows first-nonzero-positive.
+    (Strassen^2 has no two equal factors, so it admits no flip at all.)"""
+    from test_oracle_zt_flip import candidate_ops, first_nonzero_positive
+
+    for n in ((3, 3, 3), (2, 3, 4)):
+        _flip_steps(O.naive_blob(*n, ring=O.RING_ZT), candidate_ops, first_nonzero_positive)
+
+
+def _flip_steps(blob, candidate_ops, first_nonzero_positive):
     rng = random.Random(4)
-    for _ in range(200):
-        prs = O.pairs(blob, ring=O.RING_ZT)
-        if not prs:
+    steps = 0
+    for _ in range(120):
+        ops = [op for c in candidate_ops(blob) for op in c if op is not None]
+        if not ops:
             break
-        c, a, b = rng.choice(prs)
-        j, k = [x for x in range(3) if x != c]
-        out = O.replay(blob, [(0, c, j, k, a, b, 1)], ring=O.RING_ZT)
+        i, j, k, a, b = rng.choice(ops)
+        out = O.replay(blob, [(0, i, j, k, a, b, 1)], ring=O.RING_ZT)
         n, U, V, W = S.from_blob(out)
-        if np.abs(U).max() <= 1 and np.abs(V).max() <= 1 and np.abs(W).max() <= 1:
-            # admissible flips stay in {-1,0,1} and keep the tensor exact over Z
-            if O.validate(out, O.RING_ZT) == 1:
-                assert S.brent_ok(n, U, V, W, "ZT")
-                blob = out
+        assert np.abs(U).max() <= 1 and np.abs(V).max() <= 1 and np.abs(W).max() <= 1
+        assert all(first_nonzero_positive(row) for M in (U, V) for row in M)
+        assert S.brent_ok(n, U, V, W, "ZT")
+        assert O.validate(out, O.RING_ZT) == 1
+        blob = out
+        steps += 1
+    assert steps >= 20
 
 
 def test_fix_signs_makes_u_and_v_first_nonzero_positive():
