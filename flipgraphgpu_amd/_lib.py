@@ -92,6 +92,7 @@ def load_library():
     L.fg_best.argtypes = [vp, i32p, ctypes.c_int64, i32p, i32p]
     L.fg_walker_scheme.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int64]
     L.fg_verify.argtypes = [ctypes.c_int, i32p, ctypes.c_int64, i32p]
+    L.fg_verify_noalloc.argtypes = [ctypes.c_int, i32p, ctypes.c_int64, i32p]
     L.fg_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                               ctypes.POINTER(ctypes.c_uint64)]
     L.fg_last_walk_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
@@ -339,13 +340,16 @@ class Engine:
         return out[:n]
 
 
-def verify(blobs, ring=RING_Z2):
-    """GPU Brent check of a list of single-scheme blobs; returns a bool array."""
+def verify(blobs, ring=RING_Z2, noalloc=False):
+    """GPU Brent check of a list of single-scheme blobs; returns a bool array.
+    noalloc: through fg_verify_noalloc (staging in the library's fixed scratch)."""
     L = load_library()
     flat = np.concatenate([np.array([len(blobs)], dtype=np.int32)] + [np.asarray(b, dtype=np.int32) for b in blobs])
     f, p = _i32(flat)
     ok = np.zeros(len(blobs), dtype=np.int32)
-    _check(L.fg_verify(ring, p, len(f), ok.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "fg_verify")
+    fn = L.fg_verify_noalloc if noalloc else L.fg_verify
+    _check(fn(ring, p, len(f), ok.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+           "fg_verify_noalloc" if noalloc else "fg_verify")
     return ok.astype(bool)
 
 

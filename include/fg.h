@@ -189,6 +189,16 @@ FG_API int fg_walker_scheme(fg_ctx *ctx, int walker, int which, int32_t *out_blo
  * Returns the number of valid schemes or < 0. */
 FG_API int fg_verify(int ring, const int32_t *blob, int64_t len, int32_t *out_ok);
 
+/* fg_verify without any allocation (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The staging (a copy
+ * of the blob, the packed factor planes and the flags) lives in a per-GPU
+ * file-scope device array of 8 MiB, held until the process ends; a blob that
+ * needs more is rejected (< 0, see fg_last_error).  Runs on the null stream
+ * and returns when out_ok is written.  Calls that overlap on one GPU share its
+ * scratch array and race.  Same arguments, results and errors as fg_verify. */
+FG_API int fg_verify_noalloc(int ring, const int32_t *blob, int64_t len, int32_t *out_ok);
+
 /* Aggregate counters since fg_init_*: applied flips, sum of ranks at those
  * flips, walk iterations. */
 FG_API int fg_counters(fg_ctx *ctx, uint64_t *flips, uint64_t *sum_m, uint64_t *iterations);
