@@ -8,8 +8,9 @@
 namespace fgk {
 
 // Walk kernel variants: Z2 with 32-bit factor words, the integer ring
-// (packed ternary words), Z2 with 16-bit factor words in LDS.
-constexpr int kVarZ2 = 0, kVarZT = 1, kVarZ2h = 2;
+// (packed ternary words), Z2 with 16-bit factor words in LDS (all three with
+// partner lists), Z2 with 16-bit words and class masks (at most 32 terms).
+constexpr int kVarZ2 = 0, kVarZT = 1, kVarZ2h = 2, kVarZ2m = 3;
 
 size_t walk_lds_bytes(int variant, int max_m);
 // Sets the kernel's LDS limit and reports how many workgroups fit per CU.

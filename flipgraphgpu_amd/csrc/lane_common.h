@@ -45,6 +45,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "walk_args.h"
 
 #define FGK_HD __host__ __device__ __forceinline__
@@ -97,13 +99,21 @@ FGK_HD void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_
 }
 
 // Word: the LDS type of one factor (uint16_t for Z2 factors of up to 16
-// coefficients, uint32_t otherwise).
-template <typename WordT>
+// coefficients, uint32_t otherwise).  kMaskT selects how class membership is
+// kept beside H: false, the N plane (next member of each term's class);
+// true, a plane CM of 32-bit class masks (bit u of CM[c][t] set when terms t
+// and u have equal component-c factors, t included), for schemes of at most
+// 32 terms.  With masks a class is known without walking it: a term leaves
+// or joins a class with one store per member and no dependent reads.
+template <typename WordT, bool kMaskT = false>
 struct LaneBaseT {
     using Word = WordT;
+    static constexpr bool kMask = kMaskT;
     Word *F;      // walker-slot-offset plane bases
     uint8_t *H;   // slot-offset byte base of the packed H plane
     uint8_t *N;
+    uint32_t *CM; // kMask: slot-offset base of the class-mask plane
+    int cs;       // kMask: term stride of CM
     int side;     // 0: index1 / component-j half of a flip, 1: index2 / component-k half
     int32_t *err;
     int maxm;     // term capacity
@@ -131,6 +141,19 @@ struct LaneBaseT {
     FGK_HD uint8_t &h(int c, int t) { return H[((c * hw + (t >> 2)) * WS) * 4 + (t & 3)]; }
     FGK_HD uint32_t &hword(int c, int q) { return reinterpret_cast<uint32_t *>(H)[(c * hw + q) * WS]; }
     FGK_HD uint8_t &nx(int c, int t) { return N[((c * hw + (t >> 2)) * WS) * 4 + (t & 3)]; }
+    FGK_HD uint32_t &cm(int c, int t) { return CM[(c * cs + t) * WS]; }
+    // H[c][t] += d (d = +-1) as one LDS add on t's dword that returns nothing,
+    // so the wave does not wait for it (the byte never wraps: H counts are
+    // below 256 and a decremented count is at least 1).
+    FGK_HD void h_add(int c, int t, int d) {
+        uint32_t *p = &hword(c, t >> 2);
+        const uint32_t v = (uint32_t)d << (8 * (t & 3));
+#if defined(__HIP_DEVICE_COMPILE__)
+        atomicAdd(p, v);
+#else
+        *p += v;
+#endif
+    }
     FGK_HD void fail(int32_t code) { fgk_atomic_or(err, code); }
 
     // Component-indexed scalars are selected arithmetically: a select
@@ -200,40 +223,93 @@ struct LaneBaseT {
     }
 
     // ---- partner classes ----
-    // initFlips (src/schemes/scheme_z2.cu:141-150) for the N/H representation.
+    // The classes of component c from factors held in registers: rows [0, R)
+    // are loaded at once (rows from the rank up are zero, the plane holds at
+    // least R rows), then each class is found by comparing its first
+    // unassigned member with all R registers, so no comparison waits for an
+    // LDS read.  Writes H and, per the layout, the class masks or the next
+    // members; returns the component's pair count.
+    template <int R>
+    FGK_HD uint32_t rebuild_rows(int c) {
+        using M = typename std::conditional<(R > 32), uint64_t, uint32_t>::type;
+        uint32_t v[R];
+#pragma unroll
+        for (int e = 0; e < R; ++e) v[e] = f(c, e);
+        const M live = m >= R ? ~M(0) : ((M(1) << m) - M(1));
+        M done = ~live;
+        uint32_t S = 0;
+        while (~done) {
+            const int t = ctz_m(~done);
+            const uint32_t vt = f(c, t);
+            M mask = 0;
+#pragma unroll
+            for (int e = 0; e < R; ++e) mask |= M(v[e] == vt ? 1u : 0u) << e;
+            mask &= live;
+            done |= mask;
+            const uint32_t k = popc_m(mask);
+            S += k * (k - 1) / 2;
+            for (M q = mask; q; q &= q - M(1)) {
+                const int y = ctz_m(q);
+                const M above = mask & ~((M(2) << y) - M(1));
+                h(c, y) = (uint8_t)popc_m(above);
+                if constexpr (kMask) cm(c, y) = (uint32_t)mask;
+                else nx(c, y) = (uint8_t)(above ? ctz_m(above) : ctz_m(mask));
+            }
+        }
+        return S;
+    }
+    FGK_HD static int ctz_m(uint32_t x) { return __builtin_ctz(x); }
+    FGK_HD static int ctz_m(uint64_t x) { return __builtin_ctzll(x); }
+    FGK_HD static uint32_t popc_m(uint32_t x) { return (uint32_t)__builtin_popcount(x); }
+    FGK_HD static uint32_t popc_m(uint64_t x) { return (uint32_t)__builtin_popcountll(x); }
+
+    // initFlips (src/schemes/scheme_z2.cu:141-150) for the N/H (or class-mask)
+    // representation.  Schemes of up to 64 terms take the register path
+    // (rebuild_rows); larger ones walk the plane.
     FGK_HD void rebuild() {
         S0 = S1 = S2 = 0;
-        pop = 0;
-        for (int c = 0; c < 3; ++c) {
-            for (int t = 0; t < m; ++t) pop += popc(f(c, t) & vmask);
+        for (int c = 0; c < 3; ++c)
             for (int t = m; t < fs; ++t) f(c, t) = 0;  // the scan's rows past the rank
-        }
+        if (track_pop) recount_pop();
         for (int c = 0; c < 3; ++c) {
             for (int q = 0; q < hw; ++q) hword(c, q) = 0;
-            for (int t = 0; t < m; ++t) nx(c, t) = 0xFF;
             uint32_t S = 0;
-            for (int t = 0; t < m; ++t) {
-                if (nx(c, t) != 0xFF) continue;
-                const uint32_t v = f(c, t);
-                int prev = t, cnt = 0;
-                for (int u = t + 1; u < m; ++u) {
-                    if (f(c, u) == v) {
-                        nx(c, prev) = (uint8_t)u;
-                        prev = u;
-                        ++cnt;
-                    }
+            if (m <= 32) {
+                S = rebuild_rows<32>(c);
+            } else if constexpr (!kMask) {
+                if (m <= 64 && fs >= 64) {
+                    S = rebuild_rows<64>(c);
+                } else {
+                    S = rebuild_walk(c);
                 }
-                nx(c, prev) = (uint8_t)t;
-                int y = t, r = cnt;
-                do {
-                    h(c, y) = (uint8_t)r;
-                    --r;
-                    y = nx(c, y);
-                } while (y != t && r >= 0);
-                S += (uint32_t)(cnt * (cnt + 1) / 2);
             }
             addS(c, (int)S);
         }
+    }
+    FGK_HD uint32_t rebuild_walk(int c) {
+        for (int t = 0; t < m; ++t) nx(c, t) = 0xFF;
+        uint32_t S = 0;
+        for (int t = 0; t < m; ++t) {
+            if (nx(c, t) != 0xFF) continue;
+            const uint32_t v = f(c, t);
+            int prev = t, cnt = 0;
+            for (int u = t + 1; u < m; ++u) {
+                if (f(c, u) == v) {
+                    nx(c, prev) = (uint8_t)u;
+                    prev = u;
+                    ++cnt;
+                }
+            }
+            nx(c, prev) = (uint8_t)t;
+            int y = t, r = cnt;
+            do {
+                h(c, y) = (uint8_t)r;
+                --r;
+                y = nx(c, y);
+            } while (y != t && r >= 0);
+            S += (uint32_t)(cnt * (cnt + 1) / 2);
+        }
+        return S;
     }
 
     // Term x leaves its component-c class (its factor is about to change).
@@ -371,6 +447,17 @@ struct LaneBaseT {
         if (t1 < 0 || t1 >= m) {
             fail(kErrSelect);
             return false;
+        }
+        if constexpr (kMask) {
+            // the (r + 1)-th member above t1, from t1's class mask
+            uint32_t q = cm(c, t1) & ~((2u << t1) - 1u);
+            for (uint32_t e = 0; e < r; ++e) q &= q - 1u;
+            t2 = q ? __builtin_ctz(q) : -1;
+            if (t2 < 0) {
+                fail(kErrSelect);
+                return false;
+            }
+            return true;
         }
         t2 = nx(c, t1);
         for (uint32_t q = 1; q <= r; ++q) t2 = nx(c, t2);
@@ -563,8 +650,16 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     s.side = lane / kWalkersPerGroup;
     s.F = reinterpret_cast<Word *>(lds) + slot;
     s.H = lds + fbytes + (size_t)slot * 4;
-    s.N = lds + fbytes + (size_t)3 * hw * WS * 4 + (size_t)slot * 4;
-    s.lane_extra(lds + walk_lds_base_bytes(maxm, (int)sizeof(Word)), slot);
+    if constexpr (Lane::kMask) {
+        s.N = nullptr;
+        s.cs = lds_mask_stride(maxm);
+        s.CM = reinterpret_cast<uint32_t *>(lds + fbytes + (size_t)3 * hw * WS * 4) + slot;
+    } else {
+        s.CM = nullptr;
+        s.cs = 0;
+        s.N = lds + fbytes + (size_t)3 * hw * WS * 4 + (size_t)slot * 4;
+        s.lane_extra(lds + walk_lds_base_bytes(maxm, (int)sizeof(Word)), slot);
+    }
     s.fs = fs;
     s.err = A.err;
     s.maxm = maxm;

@@ -233,7 +233,16 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
     int variant = ring ? fgk::kVarZT : fgk::kVarZ2;
     const bool meta = ctx->cfg.sandwiching_probability > 0 || ctx->cfg.basis_probability > 0;
     if (meta && !ring && ctx->cfg.word_bits == 16) return fail("init: sandwiching and basis-change edges need 32-bit factor words");
-    if (!ring && ctx->cfg.word_bits == 16) {
+    // class masks: Z2 schemes of at most 32 terms with factors of up to 16
+    // coefficients, unless 32-bit words, partner lists or the meta edges are asked for
+    const bool masks_fit = !ring && nn_max <= 16 && maxm <= fgk::kMaskMaxTerms && !meta && ctx->cfg.word_bits != 32;
+    if (ctx->cfg.partner_layout < 0 || ctx->cfg.partner_layout > 2) return fail("init: partner_layout must be 0, 1 or 2");
+    if (ctx->cfg.partner_layout == 2 && !masks_fit)
+        return fail("init: class masks need Z2 schemes of at most 32 terms, factors of up to 16 coefficients, "
+                    "16-bit words and no sandwiching or basis edges");
+    if (ctx->cfg.partner_layout == 2 || (ctx->cfg.partner_layout == 0 && masks_fit && ctx->cfg.word_bits == 0)) {
+        variant = fgk::kVarZ2m;
+    } else if (!ring && ctx->cfg.word_bits == 16) {
         if (nn_max > 16) return fail("init: 16-bit factor words hold factors of up to 16 coefficients");
         variant = fgk::kVarZ2h;
     } else if (!ring && nn_max <= 16 && ctx->cfg.word_bits != 32 && !meta) {
@@ -545,7 +554,7 @@ extern "C" int fg_walk(fg_ctx *ctx, int iterations) {
 
 extern "C" int fg_resize(fg_ctx *ctx, double probability) {
     if (!ctx || !ctx->ready) return fail("fg_resize: context not initialised");
-    if (ctx->variant == fgk::kVarZ2h || ctx->minimize || ctx->flips)
+    if (ctx->variant == fgk::kVarZ2h || ctx->variant == fgk::kVarZ2m || ctx->minimize || ctx->flips)
         return fail("fg_resize: resize edges run on walk pools with 32-bit factor words (word_bits 32 for Z2)");
     if (!(probability > 0)) return 0;
     return walk_launch(ctx, 1, false, probability);

@@ -53,6 +53,12 @@ FGK_WALK_KERNEL walk_z2h_kernel(WalkArgs A) {
     walk_persistent<lane::Z2Lane16, lane::kModeWalk>(A, lds);
 }
 
+// Z2 with class masks (at most 32 terms, factors of up to 16 coefficients).
+FGK_WALK_KERNEL walk_z2m_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2LaneM, lane::kModeWalk>(A, lds);
+}
+
 // Complexity minimizer (minimizeKernel, src/entities/complexity_minimizer.cu:190-214).
 FGK_WALK_KERNEL minimize_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
@@ -67,6 +73,11 @@ FGK_WALK_KERNEL minimize_zt_kernel(WalkArgs A) {
 FGK_WALK_KERNEL minimize_z2h_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::Z2Lane16, lane::kModeMinimize>(A, lds);
+}
+
+FGK_WALK_KERNEL minimize_z2m_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2LaneM, lane::kModeMinimize>(A, lds);
 }
 
 // Scheme flips of the additions reducer (flipSchemesKernel,
@@ -108,6 +119,11 @@ FGK_WALK_KERNEL learn_z2_kernel(WalkArgs A) {
 FGK_WALK_KERNEL learn_z2h_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
     walk_persistent<lane::Z2Lane16, lane::kModeLearn>(A, lds);
+}
+
+FGK_WALK_KERNEL learn_z2m_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2LaneM, lane::kModeLearn>(A, lds);
 }
 
 // copyTo of the minimizer's re-broadcast (complexity_minimizer.cu:130-133):
@@ -275,11 +291,12 @@ __global__ void verify_zt_kernel(VerifyArgs V) {
 }
 
 size_t walk_lds_bytes(int variant, int max_m) {
+    if (variant == kVarZ2m) return z2m_lds_bytes(max_m);
     return variant == kVarZT ? zt_lds_bytes(max_m) : z2_lds_bytes(max_m, variant == kVarZ2h ? 2 : 4);
 }
 
 static const void *walk_kernel(int variant, int mode) {
-    const void *k[3][6] = {
+    const void *k[4][6] = {
         {reinterpret_cast<const void *>(walk_z2_kernel), reinterpret_cast<const void *>(minimize_z2_kernel),
          reinterpret_cast<const void *>(learn_z2_kernel), nullptr, reinterpret_cast<const void *>(resize_z2_kernel),
          reinterpret_cast<const void *>(walk_meta_z2_kernel)},
@@ -288,8 +305,10 @@ static const void *walk_kernel(int variant, int mode) {
          reinterpret_cast<const void *>(walk_meta_zt_kernel)},
         {reinterpret_cast<const void *>(walk_z2h_kernel), reinterpret_cast<const void *>(minimize_z2h_kernel),
          reinterpret_cast<const void *>(learn_z2h_kernel), nullptr, nullptr, nullptr},
+        {reinterpret_cast<const void *>(walk_z2m_kernel), reinterpret_cast<const void *>(minimize_z2m_kernel),
+         reinterpret_cast<const void *>(learn_z2m_kernel), nullptr, nullptr, nullptr},
     };
-    if (variant < 0 || variant > 2 || mode < 0 || mode > 5) return nullptr;
+    if (variant < 0 || variant > 3 || mode < 0 || mode > 5) return nullptr;
     return k[variant][mode];
 }
 

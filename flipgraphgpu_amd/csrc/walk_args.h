@@ -116,6 +116,17 @@ __host__ __device__ inline size_t walk_lds_base_bytes(int max_m, int word_bytes 
 }
 __host__ __device__ inline size_t z2_lds_bytes(int max_m, int word_bytes = 4) { return walk_lds_base_bytes(max_m, word_bytes); }
 
+// Class-mask layout (Z2, at most kMaskMaxTerms terms, 16-bit factor words):
+// F, H, then CM[c][t] (32-bit class masks).  The CM term stride is odd, so
+// the rows of two components that the lanes of a pair touch in step fall in
+// different halves of the banks.
+constexpr int kMaskMaxTerms = 32;
+__host__ __device__ inline int lds_mask_stride(int max_m) { return max_m | 1; }
+__host__ __device__ inline size_t z2m_lds_bytes(int max_m) {
+    return walk_lds_f_bytes(max_m, 2) + (size_t)kWalkersPerGroup * 3 * (size_t)h_words(max_m) * 4 +
+           (size_t)kWalkersPerGroup * 3 * (size_t)lds_mask_stride(max_m) * 4;
+}
+
 // Integer ring: the candidate pairs one tryFlip visits one at a time
 // (before its exhaustive pass) are remembered in LDS (16-bit positions,
 // kZtMaxVisits per walker, kept ascending).

@@ -59,9 +59,9 @@ FGK_HD uint32_t gf2_right(uint32_t x, const Gf2Mat &R, int a, int b) {
     return y;
 }
 
-template <typename WordT>
-struct Z2LaneT : LaneBaseT<WordT> {
-    using B = LaneBaseT<WordT>;
+template <typename WordT, bool kMaskT = false>
+struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
+    using B = LaneBaseT<WordT, kMaskT>;
     using typename B::Word;
     using typename B::JoinInfo;
     using B::F;
@@ -92,6 +92,9 @@ struct Z2LaneT : LaneBaseT<WordT> {
     using B::perm3;
     using B::nn;
     using B::select_pair;
+    using B::cm;
+    using B::h_add;
+    using B::popc;
     using B::n0;
     using B::n1;
     using B::n2;
@@ -181,6 +184,10 @@ struct Z2LaneT : LaneBaseT<WordT> {
             flip_literal(i, j, k, i1, i2, check_reduce);
             return;
         }
+        if constexpr (kMaskT) {
+            flip_masks(i, j, k, i1, i2, A, B, fj1, fk2, check_reduce);
+            return;
+        }
         const uint32_t fi = f(i, i1);
 #if defined(__HIP_DEVICE_COMPILE__)
         const int mc = side ? k : j, x = side ? i2 : i1, oc = side ? j : k;
@@ -215,6 +222,80 @@ struct Z2LaneT : LaneBaseT<WordT> {
         leave(k, i2);
         if (hb.any >= 0) link_join(k, i2, hb.ji);
         f(k, i2) = B;
+#endif
+    }
+
+    // flip with class masks (every involved factor non-zero).  A half of the
+    // flip (term x of component mc gets factor nv) reads x's masks before
+    // anything changes: its old class in mc and its classes in i and in the
+    // other changed component oc; the new class is the mask of any member
+    // found by the scan.  The half's reduction candidates are the new class
+    // members that share component i with x (rule 1) or component oc (rule 2);
+    // the lowest one is its first match (scheme_z2.cu:544-576).  The flip's
+    // other changed term is never in either new class (its factor there
+    // differs from the new one), so masks read before the flip are the ones
+    // the reference's scan sees after it.  Without a reduction, x leaves its
+    // old class and joins the new one: one mask store per member and H counts
+    // adjusted with stores that return nothing, no walk along the class.
+    struct MaskHalf {
+        uint32_t old, nw;
+        int t, rule;
+    };
+    FGK_HD MaskHalf mask_half(int mc, int x, uint32_t nv, int i, int oc, bool check_reduce) {
+        MaskHalf h;
+        h.old = cm(mc, x);
+        const uint32_t ci = cm(i, x), co = cm(oc, x);
+        const int z = this->find_last(mc, nv);
+        h.nw = z >= 0 ? cm(mc, z) : 0u;
+        const uint32_t red = check_reduce ? (h.nw & (ci | co)) : 0u;
+        h.t = red ? __builtin_ctz(red) : (1 << 30);
+        h.rule = red ? (((ci >> h.t) & 1u) ? 1 : 2) : 0;
+        return h;
+    }
+    // x leaves class `old` of component mc and joins `nw`; its factor becomes nv.
+    FGK_HD void mask_move(int mc, int x, uint32_t old, uint32_t nw, uint32_t nv) {
+        const uint32_t xb = 1u << x, below = xb - 1u;
+        const uint32_t rest = old & ~xb, joined = nw | xb;
+        for (uint32_t q = rest | nw; q; q &= q - 1u) {
+            const int y = __builtin_ctz(q);
+            const bool in_new = (nw >> y) & 1u;
+            cm(mc, y) = in_new ? joined : rest;
+            if ((below >> y) & 1u) h_add(mc, y, in_new ? 1 : -1);
+        }
+        cm(mc, x) = joined;
+        this->h(mc, x) = (uint8_t)popc(nw & ~(below | xb));
+        f(mc, x) = (Word)nv;
+        addS(mc, popc(nw) - popc(rest));
+    }
+    FGK_HD void flip_masks(int i, int j, int k, int i1, int i2, uint32_t A, uint32_t Bv, uint32_t fj1, uint32_t fk2,
+                           bool check_reduce) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const int mc = side ? k : j, x = side ? i2 : i1, oc = side ? j : k;
+        const uint32_t nv = side ? Bv : A;
+        const MaskHalf mine = mask_half(mc, x, nv, i, oc, check_reduce);
+        const int ot = pair_exchange(mine.t, side), orule = pair_exchange(mine.rule, side);
+        const int tA = side ? ot : mine.t, rA = side ? orule : mine.rule;
+        const int tB = side ? mine.t : ot, rB = side ? mine.rule : orule;
+#else
+        const MaskHalf ha = mask_half(j, i1, A, i, k, check_reduce);
+        const MaskHalf hb = mask_half(k, i2, Bv, i, j, check_reduce);
+        const int tA = ha.t, rA = ha.rule, tB = hb.t, rB = hb.rule;
+#endif
+        if (rA || rB) {
+            f(j, i1) = A;
+            f(k, i2) = Bv;
+            if (rA && tA <= tB) reduce(rA == 1 ? k : i, tA, i1);
+            else reduce(rB == 1 ? j : i, tB, i2);
+            return;
+        }
+        pop_update(fj1, A, fk2, Bv);
+#if defined(__HIP_DEVICE_COMPILE__)
+        const uint32_t before = Sc(mc);
+        mask_move(mc, x, mine.old, mine.nw, nv);
+        addS(oc, pair_exchange((int)(Sc(mc) - before), side));
+#else
+        mask_move(j, i1, ha.old, ha.nw, A);
+        mask_move(k, i2, hb.old, hb.nw, Bv);
 #endif
     }
 
@@ -324,6 +405,20 @@ struct Z2LaneT : LaneBaseT<WordT> {
 
     // tryReduce (scheme_z2.cu:1127-1154) over the fresh-list order.
     FGK_HD bool try_reduce() {
+        if constexpr (kMaskT) {
+            for (int a = 0; a < m; ++a)
+                for (uint32_t q = cm(0, a) & ~((2u << a) - 1u); q; q &= q - 1u) {
+                    const int y = __builtin_ctz(q);
+                    if (f(1, a) == f(1, y)) { reduce(2, a, y); return true; }
+                    if (f(2, a) == f(2, y)) { reduce(1, a, y); return true; }
+                }
+            for (int a = 0; a < m; ++a)
+                for (uint32_t q = cm(1, a) & ~((2u << a) - 1u); q; q &= q - 1u) {
+                    const int y = __builtin_ctz(q);
+                    if (f(2, a) == f(2, y)) { reduce(0, a, y); return true; }
+                }
+            return false;
+        }
         for (int a = 0; a < m; ++a) {
             int g = 0;
             for (int y = nx(0, a); y > a && g <= m; y = nx(0, y), ++g) {
@@ -422,6 +517,7 @@ struct Z2LaneT : LaneBaseT<WordT> {
 
 using Z2Lane = Z2LaneT<uint32_t>;    // Z2 factors of up to 32 coefficients
 using Z2Lane16 = Z2LaneT<uint16_t>;  // Z2 factors of up to 16 coefficients
+using Z2LaneM = Z2LaneT<uint16_t, true>;  // up to 16 coefficients and 32 terms, class masks
 
 }  // namespace lane
 }  // namespace fgk

@@ -47,6 +47,10 @@ typedef struct {
     int32_t word_bits;          /* Z2 LDS factor words: 16 or 32; 0 = 16 only when it lets more walker blocks run at once */
     double sandwiching_probability; /* --sandwiching-probability (divided by max iterations; Z2 needs 32-bit words; empty for ZT as in the reference) */
     double basis_probability;       /* --basis-probability (divided by max iterations; Z2 needs 32-bit words) */
+    int32_t partner_layout;     /* Z2 flip-partner bookkeeping: 0 = class masks when the scheme has at most 32
+                                   terms and 16-coefficient factors (and word_bits is 0), else partner lists;
+                                   1 = partner lists; 2 = class masks */
+    int32_t reserved0;
 } fg_config;
 
 /* Per-walker state, for tests and checkpointing. */

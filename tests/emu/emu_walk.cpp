@@ -78,9 +78,14 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
     int nn_max = 0;
     for (int q = 0; q < count; ++q)
         for (int c = 0; c < 3; ++c) nn_max = std::max(nn_max, recs[q].n[c] * recs[q].n[(c + 1) % 3]);
-    const bool half = !ring && nn_max <= 16 && !force_wide && mode != fgk::lane::kModeResize &&
-                      mode != fgk::lane::kModeMeta;
-    std::vector<uint8_t> lds(ring ? fgk::zt_lds_bytes(maxm) : fgk::z2_lds_bytes(maxm, half ? 2 : 4), 0xCD);
+    // force_wide: 0 the runtime's automatic choice, 1 32-bit words with partner
+    // lists, 2 16-bit words with partner lists, 3 class masks
+    const bool small = !ring && nn_max <= 16 && mode != fgk::lane::kModeResize && mode != fgk::lane::kModeMeta;
+    const bool masks = small && maxm <= fgk::kMaskMaxTerms && (force_wide == 0 || force_wide == 3);
+    const bool half = small && !masks && force_wide != 1;
+    std::vector<uint8_t> lds(ring ? fgk::zt_lds_bytes(maxm)
+                                  : (masks ? fgk::z2m_lds_bytes(maxm) : fgk::z2_lds_bytes(maxm, half ? 2 : 4)),
+                             0xCD);
     const int G = (W + fgk::kWalkersPerGroup - 1) / fgk::kWalkersPerGroup;
     for (int l = 0; l < launches; ++l) {
         if (l > 0) a.forced = nullptr;  // proposals apply to the first launch
@@ -101,6 +106,10 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                     walk_lane<Z2Lane, kModeResize>(a, L, lane, w, W);
                 } else if (mode == kModeMeta) {
                     walk_lane<Z2Lane, kModeMeta>(a, L, lane, w, W);
+                } else if (masks) {
+                    if (mode == kModeMinimize) walk_lane<Z2LaneM, kModeMinimize>(a, L, lane, w, W);
+                    else if (mode == kModeLearn) walk_lane<Z2LaneM, kModeLearn>(a, L, lane, w, W);
+                    else walk_lane<Z2LaneM, kModeWalk>(a, L, lane, w, W);
                 } else if (half) {
                     if (mode == kModeMinimize) walk_lane<Z2Lane16, kModeMinimize>(a, L, lane, w, W);
                     else if (mode == kModeLearn) walk_lane<Z2Lane16, kModeLearn>(a, L, lane, w, W);

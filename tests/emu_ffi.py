@@ -28,12 +28,20 @@ def lib():
     return _lib
 
 
-def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=False, wide=False, resize=None):
+LAYOUTS = {"auto": 0, "wide": 1, "half": 2, "masks": 3}
+
+
+def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=False, wide=False, resize=None,
+         layout="auto"):
     """Emulated launches; with learn (Z2) the first launch takes the proposed
     pairs `forced` [walkers][4] and the result adds (log_meta, minm).  Z2
-    factors of up to 16 coefficients run the 16-bit-word lane code unless
-    wide is set.  resize = (thr_resize, max_rank, max_coef) runs the resize
-    mode instead of the walk."""
+    layouts as the runtime picks them (auto: class masks up to 32 terms and
+    16 coefficients, else 16-bit words with partner lists up to 16
+    coefficients, else 32-bit words), or forced: "wide", "half", "masks"
+    (wide=True is "wide").  resize = (thr_resize, max_rank, max_coef) runs the
+    resize mode instead of the walk."""
+    if wide:
+        layout = "wide"
     recs = np.ascontiguousarray(recs.copy())
     err = ctypes.c_int32()
     p = params
@@ -48,7 +56,7 @@ def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=Fals
                                 p.random_iterations, p.plus_iterations, p.use_reduce, p.thr_reduce,
                                 p.use_expand, p.thr_expand, p.thr_best_eq, p.check_reduce, launches,
                                 p.zt_visit_cap, mode, None if fz is None else fz.ctypes.data,
-                                lm.ctypes.data, mm.ctypes.data, int(wide), thr, rmax, cmax,
+                                lm.ctypes.data, mm.ctypes.data, LAYOUTS[layout], thr, rmax, cmax,
                                 p.use_sandwich, p.thr_sandwich, p.use_basis, p.thr_basis, ctypes.byref(err))
     assert rc == 0
     if learn:

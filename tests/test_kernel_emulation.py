@@ -10,11 +10,11 @@ from walk_compare import first_mismatch
 
 
 def _check(n, walkers, iterations, seed, launches=1, plus=8000, p_expand=0.0, random_iterations=False,
-           check_reduce=True, ring=O.RING_Z2, p_reduce=0.0):
+           check_reduce=True, ring=O.RING_Z2, p_reduce=0.0, layout="auto"):
     start = O.naive_records(*n, walkers, ring=ring)
     p = O.make_params(iterations=iterations, plus_iterations=plus, p_expand=p_expand, p_reduce=p_reduce,
                       random_iterations=random_iterations, check_reduce=check_reduce)
-    emu, err = E.walk(start, p, seed, launches=launches, ring=ring)
+    emu, err = E.walk(start, p, seed, launches=launches, ring=ring, layout=layout)
     ora = O.walk_records(start, p, seed, launches=launches, nthreads=8, ring=ring)
     assert err == 0
     mism = first_mismatch(emu, ora)
@@ -36,18 +36,28 @@ def test_emulated_kernel_plus_expand_paths(n):
     _check(n, 64, 2000, seed=7, launches=2, plus=150, p_expand=200.0)
 
 
+# schemes of up to 32 terms run the class-mask lanes by default; these run
+# the same cases on the partner-list lanes (16- and 32-bit factor words) too
+@pytest.mark.parametrize("layout", ["masks", "half", "wide"])
+@pytest.mark.parametrize("n", [(2, 2, 2), (2, 3, 4), (3, 3, 3)])
+def test_emulated_layouts_match_oracle(n, layout):
+    _check(n, 64, 2000, seed=17, launches=2, plus=150, p_expand=200.0, layout=layout)
+
+
 def test_emulated_kernel_random_iterations():
     _check((3, 3, 3), 64, 1200, seed=99, launches=3, random_iterations=True, p_expand=20.0)
 
 
-def test_emulated_kernel_without_reductions():
+@pytest.mark.parametrize("layout", ["masks", "half"])
+def test_emulated_kernel_without_reductions(layout):
     # complexity-minimizer mode: tryFlip(checkReduce = false)
-    ora = _check((3, 3, 3), 64, 2000, seed=3, check_reduce=False)
+    ora = _check((3, 3, 3), 64, 2000, seed=3, check_reduce=False, layout=layout)
     assert (ora["m"] >= 27).all() or (ora["m"] < 27).any()
 
 
-def test_emulated_kernel_reduce_probability():
-    _check((3, 3, 3), 64, 1500, seed=11, p_reduce=300.0, plus=200, p_expand=50.0)
+@pytest.mark.parametrize("layout", ["masks", "half"])
+def test_emulated_kernel_reduce_probability(layout):
+    _check((3, 3, 3), 64, 1500, seed=11, p_reduce=300.0, plus=200, p_expand=50.0, layout=layout)
 
 
 @pytest.mark.parametrize("n,walkers,iterations", [

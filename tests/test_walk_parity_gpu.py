@@ -13,12 +13,14 @@ pytestmark = pytest.mark.gpu
 
 
 def _run_both(n, walkers, iterations, seed, launches=1, plus=8000, p_expand=0.0, random_iterations=False,
-              ring=O.RING_Z2, p_reduce=0.0, word_bits=0):
+              ring=O.RING_Z2, p_reduce=0.0, word_bits=0, partner_layout=0):
     print(f"[parity] ring={ring} n={n} walkers={walkers} iterations={iterations} launches={launches} seed={seed} "
-          f"plus={plus} p_expand={p_expand} p_reduce={p_reduce} random={random_iterations}", flush=True)
+          f"plus={plus} p_expand={p_expand} p_reduce={p_reduce} random={random_iterations} "
+          f"word_bits={word_bits} layout={partner_layout}", flush=True)
     with fg.Engine(walkers, seed, ring=ring, max_iterations=iterations, plus_iterations=plus,
                    expand_probability=p_expand, reduce_probability=p_reduce,
-                   random_iterations=random_iterations, zt_visit_cap=16, word_bits=word_bits) as e:
+                   random_iterations=random_iterations, zt_visit_cap=16, word_bits=word_bits,
+                   partner_layout=partner_layout) as e:
         e.init_naive(*n)
         init = e.read_walkers()
         ora0 = O.naive_records(*n, walkers, ring=ring)
@@ -57,6 +59,31 @@ def test_walk_16bit_words_bit_identical(n):
     # Z2 factors of up to 16 coefficients held in 16-bit LDS words
     gpu, ora = _run_both(n, 128, 2000, seed=31, launches=2, plus=200, p_expand=100.0, word_bits=16)
     assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
+
+
+@pytest.mark.parametrize("n", [(2, 2, 2), (3, 3, 3), (2, 3, 4)])
+@pytest.mark.parametrize("layout,word_bits", [(1, 0), (1, 32), (2, 0)])
+def test_walk_partner_layouts_bit_identical(n, layout, word_bits):
+    # schemes of up to 32 terms: class masks (the default) and partner lists
+    # with 16- or 32-bit words give the same trajectories
+    gpu, ora = _run_both(n, 128, 2000, seed=57, launches=2, plus=150, p_expand=100.0, word_bits=word_bits,
+                         partner_layout=layout)
+    assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
+
+
+@pytest.mark.parametrize("n,layout", [((3, 3, 3), 2), ((3, 3, 3), 1), ((4, 4, 4), 0)])
+def test_walk_reduce_probability_bit_identical(n, layout):
+    # --reduce-probability > 0: tryReduce after flips (flip_graph.cu:550-551)
+    gpu, ora = _run_both(n, 128, 2000, seed=66, launches=2, plus=200, p_expand=50.0, p_reduce=400.0,
+                         partner_layout=layout)
+    assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
+
+
+def test_walk_5x5x5_bit_identical():
+    # 25-coefficient factors (32-bit words), 125 terms, plus and expand
+    gpu, ora = _run_both((5, 5, 5), 64, 600, seed=125, launches=2, plus=200, p_expand=200.0)
+    assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
+    assert (ora["total_flips"] > 0).all()
 
 
 def test_random_iterations_bit_identical():
