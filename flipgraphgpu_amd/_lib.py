@@ -95,6 +95,8 @@ def load_library():
     L.fg_walker_scheme.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int64]
     L.fg_verify.argtypes = [ctypes.c_int, i32p, ctypes.c_int64, i32p]
     L.fg_verify_noalloc.argtypes = [ctypes.c_int, i32p, ctypes.c_int64, i32p]
+    L.fg_verify_host.argtypes = [ctypes.c_int, i32p, ctypes.c_int64, i32p]
+    L.fg_complexity_host.argtypes = [ctypes.c_int, i32p, ctypes.c_int64, i32p]
     L.fg_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                               ctypes.POINTER(ctypes.c_uint64)]
     L.fg_last_walk_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
@@ -340,6 +342,27 @@ class Engine:
                                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cap),
                    "fg_walker_scheme")
         return out[:n]
+
+
+def verify_host(blobs, ring=RING_Z2):
+    """The library's host Brent check (fg_verify_host) of single-scheme blobs; bool array."""
+    L = load_library()
+    flat = np.concatenate([np.array([len(blobs)], dtype=np.int32)] + [np.asarray(b, dtype=np.int32) for b in blobs])
+    f, p = _i32(flat)
+    ok = np.zeros(len(blobs), dtype=np.int32)
+    _check(L.fg_verify_host(ring, p, len(f), ok.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))), "fg_verify_host")
+    return ok.astype(bool)
+
+
+def complexity_host(blobs, ring=RING_Z2):
+    """getComplexity of single-scheme blobs on the host (fg_complexity_host)."""
+    L = load_library()
+    flat = np.concatenate([np.array([len(blobs)], dtype=np.int32)] + [np.asarray(b, dtype=np.int32) for b in blobs])
+    f, p = _i32(flat)
+    out = np.zeros(len(blobs), dtype=np.int32)
+    _check(L.fg_complexity_host(ring, p, len(f), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+           "fg_complexity_host")
+    return out
 
 
 def verify(blobs, ring=RING_Z2, noalloc=False):

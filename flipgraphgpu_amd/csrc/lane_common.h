@@ -69,6 +69,14 @@ FGK_HD uint32_t byte_sum(uint32_t w) {
     return (w & 0xFFu) + ((w >> 8) & 0xFFu) + ((w >> 16) & 0xFFu) + (w >> 24);
 #endif
 }
+// acc plus the four bytes of w (one v_sad_u8, its accumulator operand).
+FGK_HD uint32_t byte_sum_acc(uint32_t w, uint32_t acc) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sad_u8(w, 0u, acc);
+#else
+    return acc + byte_sum(w);
+#endif
+}
 
 namespace lane {
 
@@ -403,7 +411,7 @@ struct LaneBaseT {
             int cnt = 0;
 #pragma unroll
             for (int e = 0; e < NB; ++e) {
-                acc += byte_sum(wd[e]);
+                acc = byte_sum_acc(wd[e], acc);
                 const bool le = acc <= r;
                 below = le ? acc : below;
                 cnt += le ? 1 : 0;
@@ -425,6 +433,36 @@ struct LaneBaseT {
         }
     }
 
+    // select_scan for planes of exactly NB words, which always hold position
+    // r: no loop, and the word holding r is chosen with the compares the
+    // running sums already made.
+    template <int NB>
+    FGK_HD void select_block(int c, uint32_t &r, int &t1) {
+        uint32_t wd[NB];
+#pragma unroll
+        for (int e = 0; e < NB; ++e) wd[e] = hword(c, e);
+        bool le[NB];
+        uint32_t acc = 0, below = 0;
+#pragma unroll
+        for (int e = 0; e < NB; ++e) {
+            acc = byte_sum_acc(wd[e], acc);
+            le[e] = acc <= r;
+            below = le[e] ? acc : below;
+        }
+        uint32_t word = wd[NB - 1];
+        int idx = NB - 1;
+#pragma unroll
+        for (int e = NB - 2; e >= 0; --e) {
+            word = le[e] ? word : wd[e];
+            idx = le[e] ? idx : e;
+        }
+        r -= below;
+        const uint32_t c0 = word & 0xFFu, c1 = c0 + ((word >> 8) & 0xFFu), c2 = c1 + ((word >> 16) & 0xFFu);
+        const int b = (r >= c0 ? 1 : 0) + (r >= c1 ? 1 : 0) + (r >= c2 ? 1 : 0);
+        r -= b == 0 ? 0u : (b == 1 ? c0 : (b == 2 ? c1 : c2));
+        t1 = idx * 4 + b;
+    }
+
     // The unordered pair at position r (< S0 + S1 + S2) of the fresh-list
     // order: component, then t1, then t2.  H is scanned in blocks of 8 or 16
     // words (32 or 64 terms; the loads of a block are issued together) with
@@ -442,7 +480,8 @@ struct LaneBaseT {
         t1 = -1;
         // planes of 16 H words (64 terms) are scanned in one block, so every
         // H read of the selection goes out in one round trip
-        if (hw % 16 == 0) select_scan<16>(c, r, t1);
+        if constexpr (kMask) select_block<8>(c, r, t1);  // at most 32 terms: 8 H words
+        else if (hw % 16 == 0) select_scan<16>(c, r, t1);
         else select_scan<8>(c, r, t1);
         if (t1 < 0 || t1 >= m) {
             fail(kErrSelect);

@@ -147,6 +147,41 @@ namespace fgrt {
 void set_error(const std::string &msg) { g_err = msg; }
 }  // namespace fgrt
 
+namespace {
+// Walks a count-prefixed blob, calling fn(q, scheme) for each scheme.
+template <class Fn>
+int for_each_scheme(const char *what, int ring, const int32_t *blob, int64_t len, Fn fn) {
+    if (!blob || len < 1) return fail(std::string(what) + ": bad arguments");
+    if (ring != FG_RING_Z2 && ring != FG_RING_ZT) return fail(std::string(what) + ": unknown ring");
+    const int count = blob[0];
+    if (count < 0) return fail(std::string(what) + ": negative count");
+    int64_t pos = 1;
+    for (int q = 0; q < count; ++q) {
+        fg::Scheme s;
+        std::string err;
+        if (!fg::scheme_from_blob(ring, blob, len, &pos, &s, &err)) return fail(std::string(what) + ": " + err);
+        fn(q, s);
+    }
+    return count;
+}
+}  // namespace
+
+extern "C" int fg_verify_host(int ring, const int32_t *blob, int64_t len, int32_t *out_ok) {
+    if (!out_ok) return fail("fg_verify_host: null output");
+    int good = 0;
+    const int rc = for_each_scheme("fg_verify_host", ring, blob, len, [&](int q, const fg::Scheme &s) {
+        out_ok[q] = fg::validate(s) ? 1 : 0;
+        good += out_ok[q];
+    });
+    return rc < 0 ? rc : good;
+}
+
+extern "C" int fg_complexity_host(int ring, const int32_t *blob, int64_t len, int32_t *out) {
+    if (!out) return fail("fg_complexity_host: null output");
+    return for_each_scheme("fg_complexity_host", ring, blob, len,
+                           [&](int q, const fg::Scheme &s) { out[q] = fg::complexity(s); });
+}
+
 extern "C" int fg_device_info(int *cu_count, int *clock_khz, int *lds_per_block, char *name, int name_cap) {
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));

@@ -252,8 +252,9 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
         h.rule = red ? (((ci >> h.t) & 1u) ? 1 : 2) : 0;
         return h;
     }
-    // x leaves class `old` of component mc and joins `nw`; its factor becomes nv.
-    FGK_HD void mask_move(int mc, int x, uint32_t old, uint32_t nw, uint32_t nv) {
+    // x leaves class `old` of component mc and joins `nw`; its factor becomes
+    // nv.  Returns the change of component mc's pair count.
+    FGK_HD int mask_move(int mc, int x, uint32_t old, uint32_t nw, uint32_t nv) {
         const uint32_t xb = 1u << x, below = xb - 1u;
         const uint32_t rest = old & ~xb, joined = nw | xb;
         for (uint32_t q = rest | nw; q; q &= q - 1u) {
@@ -265,7 +266,7 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
         cm(mc, x) = joined;
         this->h(mc, x) = (uint8_t)popc(nw & ~(below | xb));
         f(mc, x) = (Word)nv;
-        addS(mc, popc(nw) - popc(rest));
+        return popc(nw) - popc(rest);
     }
     FGK_HD void flip_masks(int i, int j, int k, int i1, int i2, uint32_t A, uint32_t Bv, uint32_t fj1, uint32_t fk2,
                            bool check_reduce) {
@@ -290,13 +291,17 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
         }
         pop_update(fj1, A, fk2, Bv);
 #if defined(__HIP_DEVICE_COMPILE__)
-        const uint32_t before = Sc(mc);
-        mask_move(mc, x, mine.old, mine.nw, nv);
-        addS(oc, pair_exchange((int)(Sc(mc) - before), side));
+        const int d = mask_move(mc, x, mine.old, mine.nw, nv);
+        const int od = pair_exchange(d, side);
+        const int dj = side ? od : d, dk = side ? d : od;
 #else
-        mask_move(j, i1, ha.old, ha.nw, A);
-        mask_move(k, i2, hb.old, hb.nw, Bv);
+        const int dj = mask_move(j, i1, ha.old, ha.nw, A);
+        const int dk = mask_move(k, i2, hb.old, hb.nw, Bv);
 #endif
+        // component i keeps its classes; j and k change by dj and dk
+        S0 += (uint32_t)(j == 0 ? dj : (k == 0 ? dk : 0));
+        S1 += (uint32_t)(j == 1 ? dj : (k == 1 ? dk : 0));
+        S2 += (uint32_t)(j == 2 ? dj : (k == 2 ? dk : 0));
     }
 
     // plus (scheme_z2.cu:579-612)

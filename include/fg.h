@@ -193,6 +193,17 @@ FG_API int fg_walker_scheme(fg_ctx *ctx, int walker, int which, int32_t *out_blo
  * Returns the number of valid schemes or < 0. */
 FG_API int fg_verify(int ring, const int32_t *blob, int64_t len, int32_t *out_ok);
 
+/* The Brent check on the host (Scheme::validate, src/schemes/scheme_z2.cu:3-29,
+ * scheme_integer.cu:3-38): the library's own re-verification of GPU results,
+ * for a count-prefixed blob as fg_verify takes; needs no GPU.  Returns the
+ * number of valid schemes or < 0. */
+FG_API int fg_verify_host(int ring, const int32_t *blob, int64_t len, int32_t *out_ok);
+
+/* getComplexity (src/schemes/scheme_z2.cu:127-139, scheme_integer.cu:139-147)
+ * of every scheme of a count-prefixed blob, on the host.  Returns the count
+ * or < 0. */
+FG_API int fg_complexity_host(int ring, const int32_t *blob, int64_t len, int32_t *out);
+
 /* fg_verify without any allocation (no counterpart in the reference; for
  * callers that cannot allocate device memory: code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  The staging (a copy
