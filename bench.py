@@ -13,8 +13,12 @@ updates and the equality test of the two new factors against all m terms);
 achieved = sum(W) / (sum of walk-kernel durations from HIP events), peak =
 CUs * clock * 64 lanes.
 cpu_baseline: the oracle's restatement of the reference walk (history-
-ordered FlipSet, as the reference keeps it) on the host cores, on a bounded
-sample of the same workload.
+ordered FlipSet, as the reference keeps it), built -O3 -march=native on this
+host, on every CPU this process may use; median of 5 bounded runs of the same
+workload.
+ttt: the second half of the metric -- time to a verified 4x4x4 Z2 scheme of
+rank <= 49, uniform walks vs learned + uniform proposals (3 seeds per arm in
+the default line, 10 with --workload ttt).
 """
 import argparse
 import json
@@ -41,6 +45,9 @@ def parse():
                     help="Z2 LDS factor words (0: the runtime's choice)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ttt", action="store_true", help="leave the time-to-target A/B out of the default line")
+    ap.add_argument("--ttt-seeds-default", type=int, default=3, help="seeds per arm of the A/B in the default line")
+    ap.add_argument("--ttt-train-default", type=int, default=600, help="training steps of the A/B in the default line")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "ttt", "propose"],
@@ -80,27 +87,67 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}))
 
 
-def cpu_baseline(n, plus_iterations, target_seconds):
-    """Oracle (reference FlipSet order) on the host cores; returns flips/s."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n, plus_iterations, target_seconds, reps=5):
+    """The oracle's restatement of the reference walk (history-ordered FlipSet,
+    as the reference keeps it) on every CPU this process may use, one thread
+    per CPU with the walkers split between them.  It is built with -O3
+    -march=native for this host first (the prebuilt library is the fallback).
+    Returns the median flips/s of `reps` runs of about target_seconds / reps
+    each, from the naive scheme with the workload's flags."""
+    import statistics
+    import subprocess
+    import tempfile
+
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ffi as O
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    build = "-O3 -march=native, built on this host"
+    path = os.path.join(tempfile.mkdtemp(prefix="fg_cpu_baseline_"), "liboracle_native.so")
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", f"NATIVE_OUT={path}"],
+                       check=True, capture_output=True, timeout=120)
+        O.use_library(path)
+    except (subprocess.SubprocessError, OSError, RuntimeError):
+        build = "prebuilt oracle/liboracle.so (-O3 -march=x86-64-v2): the native build failed"
+    threads = len(os.sched_getaffinity(0))
     walkers = threads * 8
-    iters = 1000
-    while True:
+
+    def run(iters, seed):
         p = O.make_params(iterations=iters, plus_iterations=plus_iterations, select_mode=O.SELECT_REFERENCE)
         recs = O.naive_records(*n, walkers)
         t0 = time.perf_counter()
-        recs = O.walk_records(recs, p, seed=1, nthreads=threads)
+        recs = O.walk_records(recs, p, seed=seed, nthreads=threads)
         dt = time.perf_counter() - t0
-        if dt >= target_seconds * 0.5 or iters >= 1 << 26:
+        return int(recs["total_flips"].sum()), dt
+
+    iters = 2000  # calibration: grow until a run takes a measurable time
+    while True:
+        flips, dt = run(iters, 99)
+        if dt >= 1.0 or iters >= 1 << 26:
             break
-        iters = int(iters * min(16.0, max(2.0, target_seconds / max(dt, 1e-3))))
-    flips = int(recs["total_flips"].sum())
-    return {"value": flips / dt, "unit": "flips/s", "cores": threads, "kind": "port",
-            "sample": f"{walkers} walkers x {iters} iterations of the {n[0]}x{n[1]}x{n[2]} Z2 walk "
-                      f"from the naive scheme, {dt:.1f} s wall"}
+        iters *= 4
+    iters = max(1000, int(iters * (target_seconds / reps) / max(dt, 1e-3)))
+    rates, secs = [], []
+    for rep in range(reps):
+        flips, dt = run(iters, 1 + rep)
+        rates.append(flips / dt)
+        secs.append(dt)
+    return {"value": statistics.median(rates), "unit": "flips/s", "cores": threads, "kind": "port",
+            "sample": f"{walkers} walkers x {iters} iterations of the {n[0]}x{n[1]}x{n[2]} Z2 walk from the naive "
+                      f"scheme, plus-iterations {plus_iterations}, {reps} runs of {statistics.median(secs):.1f} s "
+                      f"(median), {threads} threads",
+            "runs": [round(x, 1) for x in rates], "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "threads_available": threads, "build": build}
 
 
 def profiled_traffic(path):
@@ -162,19 +209,19 @@ def heldout_ranking(a, sc, W, L):
     return res
 
 
-def time_to_target(a):
+def time_to_target(a, seeds=None, train_steps=None, emit=True):
     """Median wall time over seeds to a verified n=4 Z2 scheme of rank <= target,
     uniform walks vs learned + uniform proposals.  The scorer is trained once,
     on trajectories logged by uniform walks of the same shape (seed 1000);
     its training time is reported separately.  A run ends at the first launch
     after which the best scheme has rank <= target and passes the GPU Brent
-    check and the host re-check."""
+    check and the library's host re-check.  Returns the result (and prints it
+    as one JSON line when emit)."""
+    seeds = seeds or a.seeds
+    train_steps = train_steps or a.train_steps
     import numpy as np
 
     import flipgraphgpu_amd as fg
-
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_ffi as O
 
     W, L = a.ttt_walkers, a.ttt_launch
 
@@ -196,7 +243,7 @@ def time_to_target(a):
             launches += 1
             blob, best, _ = e.best()
             if best <= a.ttt_target:
-                ok = fg.verify([blob])[0] and O.validate(blob) == 1
+                ok = fg.verify([blob])[0] and fg.verify_host([blob])[0]
                 if ok:
                     break
         dt = time.perf_counter() - t0
@@ -216,14 +263,14 @@ def time_to_target(a):
     progress(f"[ttt] logged {info['rows']} flips ({info['positives']} positive) in {t_log:.1f} s")
     sc = fg.Scorer(4, 4, 4, 64, seed=7)
     t0 = time.perf_counter()
-    losses = sc.train(e, steps=a.train_steps, batch=a.train_batch, lr=a.lr, seed=11)
+    losses = sc.train(e, steps=train_steps, batch=a.train_batch, lr=a.lr, seed=11)
     t_train = time.perf_counter() - t0
     e.close()
-    progress(f"[ttt] trained {a.train_steps} steps in {t_train:.1f} s, loss {losses[:20].mean():.4f} -> {losses[-50:].mean():.4f}")
+    progress(f"[ttt] trained {train_steps} steps in {t_train:.1f} s, loss {losses[:20].mean():.4f} -> {losses[-50:].mean():.4f}")
     heldout = heldout_ranking(a, sc, W, L)
     progress(f"[ttt] held-out AUC: trained {heldout['auc_trained']:.4f}, untrained {heldout['auc_untrained']:.4f}")
     res = {"uniform": [], "learned": []}
-    for seed in range(1, a.seeds + 1):
+    for seed in range(1, seeds + 1):
         for arm, scorer in (("uniform", None), ("learned", sc)):
             dt, best, launches = run(seed, scorer)
             res[arm].append({"seed": seed, "seconds": round(dt, 3), "best_rank": int(best), "launches": launches,
@@ -237,7 +284,7 @@ def time_to_target(a):
 
     mu, ml = med("uniform"), med("learned")
     out = {
-        "metric": f"time to verified 4x4x4 Z2 rank-{a.ttt_target} (median over {a.seeds} seeds)",
+        "metric": f"time to verified 4x4x4 Z2 rank-{a.ttt_target} (median over {seeds} seeds)",
         "value": ml, "unit": "s", "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
         "uniform_median_s": mu, "learned_median_s": ml,
         "speedup_learned_vs_uniform": (mu / ml) if ml > 0 else None,
@@ -248,11 +295,13 @@ def time_to_target(a):
         "heldout": heldout,
         "training": {"logged_flips": info["rows"], "positives": info["positives"],
                      "log_seconds": round(t_log, 2), "train_seconds": round(t_train, 2),
-                     "steps": a.train_steps, "batch": a.train_batch,
+                     "steps": train_steps, "batch": a.train_batch,
                      "loss_first": float(losses[:20].mean()), "loss_last": float(losses[-50:].mean())},
         "runs": res,
     }
-    print(json.dumps(out))
+    if emit:
+        print(json.dumps(out))
+    return out
 
 
 def propose_flop(n, M, K, D=128, Z=64):
@@ -333,9 +382,6 @@ def workload_c1(a):
     per walker)."""
     import flipgraphgpu_amd as fg
 
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_ffi as O
-
     e = fg.Engine(1024, a.seed, max_iterations=1000, plus_iterations=10000, expand_probability=0.0)
     e.init_naive(2, 2, 2)
     c0 = e.counters()
@@ -346,7 +392,7 @@ def workload_c1(a):
             raise RuntimeError("invalid schemes")
         launches += 1
         blob, best, _ = e.best()
-        if best <= 7 and fg.verify([blob])[0] and O.validate(blob) == 1:
+        if best <= 7 and fg.verify([blob])[0] and fg.verify_host([blob])[0]:
             break
     dt = time.perf_counter() - t0
     flips = e.counters()["flips"] - c0["flips"]
@@ -410,9 +456,6 @@ def workload_c4(a):
 
     import flipgraphgpu_amd as fg
 
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_ffi as O
-
     out = {"metric": "additions of integer-ring schemes before / after reduction", "n_gpus": 1, "formats": {}}
     for n, launches in (((3, 3, 3), a.launches or 30), ((4, 4, 4), a.launches or 30)):
         W = 65536
@@ -427,13 +470,16 @@ def workload_c4(a):
         dt = time.perf_counter() - t0
         flips = e.counters()["flips"] - c0["flips"]
         blob, rank, _ = e.best()
-        assert fg.verify([blob], ring=fg.RING_ZT)[0] and O.validate(blob, O.RING_ZT) == 1
+        assert fg.verify([blob], ring=fg.RING_ZT)[0] and fg.verify_host([blob], fg.RING_ZT)[0]
         recs = e.read_walkers()
         cands = [w for w in range(W) if recs[w]["best_m"] == rank][:256]
         blobs = [e.walker_scheme(w, best=True) for w in cands]
         e.close()
-        naive_cx = O.complexity(O.naive_blob(*n, ring=O.RING_ZT), O.RING_ZT)
-        cx_found = min(O.complexity(b, O.RING_ZT) for b in blobs)
+        with fg.Engine(1, 1, ring=fg.RING_ZT) as ne:
+            ne.init_naive(*n)
+            naive_blob = ne.walker_scheme(0, best=False)
+        naive_cx = int(fg.complexity_host([naive_blob], fg.RING_ZT)[0])
+        cx_found = int(fg.complexity_host(blobs, fg.RING_ZT).min())
         # complexity minimizer over the walkers' best schemes of that rank
         t1 = time.perf_counter()
         z = fg.Engine(4096, a.seed, ring=fg.RING_ZT, max_iterations=10000, check_reduce=False)
@@ -537,7 +583,9 @@ def main():
     traffic = profiled_traffic(os.path.join(ROOT, traffic_src)) if default_shape else None
     achieved = ops / (kernel_ms / 1e3) if kernel_ms > 0 else 0.0
     out = {
-        "metric": f"flips/s ({'integer-ring' if zt else 'Z2'} flip-graph random walk, {n[0]}x{n[1]}x{n[2]}, {a.walkers} walkers)",
+        "metric": ("flips/s; time to verified 4x4x4 Z2 rank-49, learned vs uniform flips" if default_shape else
+                   f"flips/s ({'integer-ring' if zt else 'Z2'} flip-graph random walk, {n[0]}x{n[1]}x{n[2]}, "
+                   f"{a.walkers} walkers)"),
         "value": flips / dt,
         "unit": "flips/s",
         "n_gpus": 1,
@@ -574,6 +622,12 @@ def main():
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, a.plus_iterations, a.cpu_seconds)
+    if not a.no_ttt:
+        t = time_to_target(a, seeds=a.ttt_seeds_default, train_steps=a.ttt_train_default, emit=False)
+        out["ttt"] = {"metric": t["metric"], "uniform_median_s": t["uniform_median_s"],
+                      "learned_median_s": t["learned_median_s"], "speedup": t["speedup_learned_vs_uniform"],
+                      "config": t["config"]["workload"], "heldout": t["heldout"], "training": t["training"],
+                      "runs": t["runs"]}
     print(json.dumps(out))
 
 

@@ -90,6 +90,7 @@ def load_library():
     L.fg_complexities.argtypes = [vp, i32p, i32p]
     L.fg_broadcast_best.argtypes = [vp, ctypes.c_int, i32p, ctypes.c_int]
     L.fg_autotune.argtypes = [vp, ctypes.c_int, i32p]
+    L.fg_layout_info.argtypes = [vp, i32p, i32p, ctypes.c_int]
     L.fg_walk.argtypes = [vp, ctypes.c_int]
     L.fg_best.argtypes = [vp, i32p, ctypes.c_int64, i32p, i32p]
     L.fg_walker_scheme.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int64]
@@ -296,6 +297,16 @@ class Engine:
         g = ctypes.c_int32()
         _check(self.L.fg_autotune(self.ctx, iterations, ctypes.byref(g)), "fg_autotune")
         return g.value
+
+    LAYOUTS = {0: "z2 lists, 32-bit words", 1: "integer ring", 2: "z2 lists, 16-bit words", 3: "z2 class masks"}
+
+    def layout_info(self):
+        """(layout, trials of the last autotune as [n][3] = layout, grid, flips/s / 1e6)."""
+        v = ctypes.c_int32()
+        buf = np.zeros((64, 3), dtype=np.int32)
+        n = self.L.fg_layout_info(self.ctx, ctypes.byref(v), buf.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), 64)
+        _check(n, "fg_layout_info")
+        return int(v.value), buf[:n].copy()
 
     def summary(self):
         W = self.walkers

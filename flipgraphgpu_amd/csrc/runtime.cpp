@@ -79,7 +79,11 @@ struct fg_ctx {
     int nitems = 0;
     bool minimize = false;  // pool set up by fg_init_minimizer
     bool flips = false;     // pool set up by fg_init_flips
-    int variant = 0;        // walk kernel variant (kernels.h: kVarZ2 / kVarZT / kVarZ2h)
+    int variant = 0;        // walk kernel variant (kernels.h: kVarZ2 / kVarZT / kVarZ2h / kVarZ2m)
+    int nn_max = 0;         // widest factor of the pool's formats
+    bool masks_fit = false; // the class-mask layout can hold the pool
+    bool meta = false;      // sandwiching / basis edges configured
+    std::vector<int32_t> trials;  // last fg_autotune: (variant, grid, flips per second / 1e6) triples
 
     // trajectory log: ring of the last `horizon` launches, labelled rows
     struct Log {
@@ -111,8 +115,9 @@ struct fg_ctx {
     int grid = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    int graph_iterations = -1;  // launch the captured graph replays: iterations and grid
+    int graph_iterations = -1;  // launch the captured graph replays: iterations, grid and kernel variant
     int graph_grid = -1;
+    int graph_variant = -1;
 
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -121,6 +126,7 @@ struct fg_ctx {
         graph = nullptr;
         graph_iterations = -1;
         graph_grid = -1;
+        graph_variant = -1;
     }
     void release() {
         drop_graph();
@@ -281,9 +287,13 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
         if (nn_max > 16) return fail("init: 16-bit factor words hold factors of up to 16 coefficients");
         variant = fgk::kVarZ2h;
     } else if (!ring && nn_max <= 16 && ctx->cfg.word_bits != 32 && !meta) {
+        // workgroups resident per CU as the occupancy query gives them
+        // (LDS and registers), for either word size
         const long blocks = (W + fgk::kWalkersPerGroup - 1) / fgk::kWalkersPerGroup;
-        const size_t l32 = fgk::walk_lds_bytes(fgk::kVarZ2, maxm), l16 = fgk::walk_lds_bytes(fgk::kVarZ2h, maxm);
-        const long fit32 = (long)(ctx->lds_per_cu / l32) * ctx->cus, fit16 = (long)(ctx->lds_per_cu / l16) * ctx->cus;
+        int g32 = 0, g16 = 0;
+        HIPCHK(fgk::prepare_walk(fgk::kVarZ2, maxm, 0, &g32));
+        HIPCHK(fgk::prepare_walk(fgk::kVarZ2h, maxm, 0, &g16));
+        const long fit32 = (long)g32 * ctx->cus, fit16 = (long)g16 * ctx->cus;
         const long rounds32 = fit32 > 0 ? (blocks + fit32 - 1) / fit32 : LONG_MAX;
         const long rounds16 = fit16 > 0 ? (blocks + fit16 - 1) / fit16 : LONG_MAX;
         if (rounds16 < rounds32) variant = fgk::kVarZ2h;
@@ -294,6 +304,10 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
 
     ctx->release();
     ctx->variant = variant;
+    ctx->nn_max = nn_max;
+    ctx->masks_fit = masks_fit;
+    ctx->meta = meta;
+    ctx->trials.clear();
     ctx->W = W;
     ctx->maxm = maxm;
     const size_t plane = (size_t)3 * maxm * W;
@@ -734,7 +748,7 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced, double resize) {
         return verify_planes(ctx, ctx->cfg.ring, ctx->cur, ctx->m, ctx->fmt, ctx->W, ctx->maxm, ctx->ok);
     };
     if (ctx->cfg.use_graph && mode != 2 && mode != 4) {
-        if (ctx->graph_iterations != a.iterations || ctx->graph_grid != ctx->grid) {
+        if (ctx->graph_iterations != a.iterations || ctx->graph_grid != ctx->grid || ctx->graph_variant != ctx->variant) {
             ctx->drop_graph();
             HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             const int rc = enqueue(false);
@@ -746,6 +760,7 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced, double resize) {
             HIPCHK(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
             ctx->graph_iterations = a.iterations;
             ctx->graph_grid = ctx->grid;
+            ctx->graph_variant = ctx->variant;
         }
         HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
         HIPCHK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
@@ -861,31 +876,83 @@ extern "C" int fg_get_forced(fg_ctx *ctx, int32_t *forced) {
 
 extern "C" int fg_walk_forced(fg_ctx *ctx, int iterations) { return fgrt::walk_forced(ctx, iterations); }
 
+namespace {
+// Makes `variant` the pool's walk kernel: sets the LDS limits of every mode
+// the pool may launch with it and returns its resident workgroups per CU.
+int use_variant(fg_ctx *ctx, int variant, int *groups_per_cu) {
+    const int mode = ctx->minimize ? 1 : (ctx->flips ? 3 : 0);
+    int g = 0;
+    if (fgk::prepare_walk(variant, ctx->maxm, mode, &g) != hipSuccess || g < 1) return -1;
+    if (mode == 0 && ctx->cfg.ring == FG_RING_Z2) {
+        int g2 = 0;
+        if (fgk::prepare_walk(variant, ctx->maxm, 2, &g2) != hipSuccess) return -1;
+    }
+    ctx->variant = variant;
+    ctx->groups_per_cu = g;
+    *groups_per_cu = g;
+    return 0;
+}
+}  // namespace
+
 extern "C" int fg_autotune(fg_ctx *ctx, int iterations, int32_t *grid) {
     if (!ctx || !ctx->ready) return fail("fg_autotune: context not initialised");
     if (iterations <= 0) return fail("fg_autotune: iterations must be positive");
-    const int maxg = std::max(1, std::min(ctx->groups_per_cu * ctx->cus, ctx->nitems));
-    int best = maxg;
-    double best_rate = -1;
-    for (int mult = 1;; ++mult) {
-        const int g = std::min(mult * ctx->cus, maxg);
-        ctx->grid = g;
-        uint64_t f0 = 0, f1 = 0;
-        if (fg_counters(ctx, &f0, nullptr, nullptr) < 0) return -1;
-        const int bad = fg_walk(ctx, iterations);
-        if (bad < 0) return bad;
-        if (fg_counters(ctx, &f1, nullptr, nullptr) < 0) return -1;
-        const double rate = (double)(f1 - f0) / std::max(1e-3, (double)ctx->last_ms);
-        if (rate > best_rate) {
-            best_rate = rate;
-            best = g;
-        }
-        if (g >= maxg) break;
+    // layouts worth timing: every Z2 layout the pool fits when the config
+    // leaves the choice open (the integer ring has one layout)
+    std::vector<int> variants;
+    const bool open = ctx->cfg.ring == FG_RING_Z2 && ctx->cfg.word_bits == 0 && ctx->cfg.partner_layout == 0 &&
+                      !ctx->meta && !ctx->log.on;
+    if (open) {
+        if (ctx->masks_fit) variants.push_back(fgk::kVarZ2m);
+        if (ctx->nn_max <= 16) variants.push_back(fgk::kVarZ2h);
+        variants.push_back(fgk::kVarZ2);
+    } else {
+        variants.push_back(ctx->variant);
     }
-    ctx->grid = best;
+    ctx->trials.clear();
+    int best_v = ctx->variant, best_g = ctx->grid, best_gpc = ctx->groups_per_cu;
+    double best_rate = -1;
+    for (int v : variants) {
+        int gpc = 0;
+        if (use_variant(ctx, v, &gpc)) continue;
+        const int maxg = std::max(1, std::min(gpc * ctx->cus, ctx->nitems));
+        for (int mult = 1;; ++mult) {
+            const int g = std::min(mult * ctx->cus, maxg);
+            ctx->grid = g;
+            ctx->drop_graph();
+            uint64_t f0 = 0, f1 = 0;
+            if (fg_counters(ctx, &f0, nullptr, nullptr) < 0) return -1;
+            const int bad = fg_walk(ctx, iterations);
+            if (bad < 0) return bad;
+            if (fg_counters(ctx, &f1, nullptr, nullptr) < 0) return -1;
+            const double rate = (double)(f1 - f0) / std::max(1e-3, (double)ctx->last_ms) * 1e3;
+            ctx->trials.insert(ctx->trials.end(), {v, g, (int32_t)std::min(rate / 1e6, 2e9)});
+            if (rate > best_rate) {
+                best_rate = rate;
+                best_v = v;
+                best_g = g;
+                best_gpc = gpc;
+            }
+            if (g >= maxg) break;
+        }
+    }
+    int gpc = 0;
+    if (use_variant(ctx, best_v, &gpc)) return fail("fg_autotune: could not restore the chosen layout");
+    ctx->groups_per_cu = best_gpc;
+    ctx->grid = best_g;
     ctx->drop_graph();
-    if (grid) *grid = best;
+    if (grid) *grid = best_g;
     return 0;
+}
+
+extern "C" int fg_layout_info(fg_ctx *ctx, int32_t *variant, int32_t *trials, int cap) {
+    if (!ctx || !ctx->ready) return fail("fg_layout_info: context not initialised");
+    if (variant) *variant = ctx->variant;
+    const int n = (int)ctx->trials.size() / 3;
+    if (trials)
+        for (int q = 0; q < n && q < cap; ++q)
+            for (int e = 0; e < 3; ++e) trials[3 * q + e] = ctx->trials[3 * q + e];
+    return n;
 }
 
 extern "C" int fg_last_walk_ms(fg_ctx *ctx, float *ms) {

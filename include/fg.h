@@ -147,10 +147,23 @@ FG_API int fg_broadcast_best(fg_ctx *ctx, int src, const int32_t *dst, int count
  * contiguous, in the given order. */
 FG_API int fg_init_batched(fg_ctx *ctx, int nformats, const int32_t *formats, const int32_t *weights);
 
-/* Picks the persistent grid size: times one short launch of `iterations`
- * per candidate (1, 2, ... times the CU count, up to the occupancy limit),
- * keeps the fastest in flips per second and returns it in *grid. */
+/* Autotuner (no reference counterpart; the reference's --block-size, threads
+ * per block, is fixed here at 32 walkers per workgroup -- one wave, two lanes
+ * per walker).  Times one launch of `iterations` per candidate and keeps the
+ * fastest in flips per second: persistent grids of 1, 2, ... times the CU
+ * count up to the occupancy limit, and, for Z2 pools whose config leaves the
+ * layout open (word_bits 0, partner_layout 0, no basis edges, no trajectory
+ * log), every LDS layout the pool fits (class masks, 16-bit words with
+ * partner lists, 32-bit words with partner lists).  The trials are ordinary
+ * walk launches; every layout and grid gives the same trajectories.  *grid
+ * gets the chosen grid. */
 FG_API int fg_autotune(fg_ctx *ctx, int iterations, int32_t *grid);
+
+/* The pool's walk layout (0: Z2, 32-bit words and partner lists; 1: integer
+ * ring; 2: Z2, 16-bit words and partner lists; 3: Z2 class masks) and the
+ * trials of the last fg_autotune, [n][3] = layout, grid, flips per second /
+ * 1e6 (at most cap rows).  Returns the number of trials. */
+FG_API int fg_layout_info(fg_ctx *ctx, int32_t *layout, int32_t *trials, int cap);
 
 /* FlipGraph::randomWalk / randomWalkKernel (src/entities/flip_graph.cu:235-240,
  * 520-573): one launch.  iterations <= 0 uses max_iterations (random in

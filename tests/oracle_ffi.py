@@ -86,6 +86,16 @@ def _build():
 _lib = None
 
 
+def use_library(path):
+    """Load the oracle from `path` (a build of the same sources, e.g. with
+    -march=native for a timing) instead of oracle/liboracle.so; call before
+    the first lib()."""
+    global LIB_PATH, _lib
+    if _lib is not None and LIB_PATH != path:
+        raise RuntimeError("the oracle library is already loaded")
+    LIB_PATH = path
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -59,23 +59,36 @@ def test_graph_replay_matches_plain_launches():
     assert first_mismatch(out[1], out[0]) is None
 
 
-def test_autotune_keeps_results_exact():
-    W, seed = 2048, 9
-    with fg.Engine(W, seed, max_iterations=400, plus_iterations=200) as e:
-        e.init_naive(3, 3, 3)
+@pytest.mark.parametrize("n,W,use_graph", [((3, 3, 3), 2048, False), ((3, 3, 3), 2048, True),
+                                            ((4, 4, 4), 512, True)])
+def test_autotune_keeps_results_exact(n, W, use_graph):
+    """The autotuner times every layout the pool fits (class masks, 16- and
+    32-bit partner lists) and grid with ordinary 100-iteration launches; the
+    walk then continues in the chosen layout, bit-identical to the oracle
+    replaying all those launches."""
+    seed = 9
+    print(f"[autotune] n={n} W={W} use_graph={use_graph}", flush=True)
+    with fg.Engine(W, seed, max_iterations=400, plus_iterations=200, use_graph=use_graph) as e:
+        e.init_naive(*n)
         start = e.read_walkers()
+        assert e.walk(400) == 0  # a launch before tuning (a cached graph of the first layout)
         g = e.autotune(100)
-        assert g >= 1
+        layout, trials = e.layout_info()
+        assert g >= 1 and len(trials) >= 1
+        assert layout in set(int(x) for x in trials[:, 0])
+        expect_layouts = {0, 2} | ({3} if n[0] * n[1] * n[2] <= 32 else set())
+        assert set(int(x) for x in trials[:, 0]) == expect_layouts
+        # each trial launched its own grid: every (layout, grid) pair is distinct
+        assert len({(int(a), int(b)) for a, b, _ in trials}) == len(trials)
         assert e.walk(400) == 0
         gpu = e.read_walkers()
         c = e.counters()
     assert (gpu["it"] == gpu["it"][0]).all()
     iters_done = int(gpu["it"][0])
     assert c["iterations"] == W * iters_done
-    # 64 blocks of 32 walkers fit in one grid size, so the autotuner ran
-    # exactly one 100-iteration launch; replay 100 + 400 on the oracle
-    assert iters_done == 500
-    mid = O.walk_records(start, O.make_params(iterations=100, max_iterations=400, plus_iterations=200), seed,
-                         nthreads=8)
-    ora = O.walk_records(mid, O.make_params(iterations=400, plus_iterations=200), seed, nthreads=8)
+    assert iters_done == 800 + 100 * len(trials)
+    ora = O.walk_records(start, O.make_params(iterations=400, plus_iterations=200), seed, nthreads=8)
+    ora = O.walk_records(ora, O.make_params(iterations=100, max_iterations=400, plus_iterations=200), seed,
+                         launches=len(trials), nthreads=8)
+    ora = O.walk_records(ora, O.make_params(iterations=400, plus_iterations=200), seed, nthreads=8)
     assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)

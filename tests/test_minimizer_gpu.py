@@ -23,6 +23,7 @@ def _blob_of(schemes):
     (O.RING_Z2, (4, 4, 4), 256, 2000, 0),
     (O.RING_Z2, (3, 3, 3), 256, 2000, 16),
     (O.RING_ZT, (3, 3, 3), 512, 3000, 0),
+    (O.RING_ZT, (4, 4, 4), 256, 2000, 0),
 ])
 def test_minimizer_bit_identical_to_oracle(ring, n, walkers, iterations, bits):
     print(f"[minimizer] ring={ring} n={n} walkers={walkers} iterations={iterations} bits={bits}", flush=True)
