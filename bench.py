@@ -50,10 +50,11 @@ def parse():
     ap.add_argument("--ttt-train-default", type=int, default=600, help="training steps of the A/B in the default line")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "ttt", "propose"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "ttt", "propose", "descent"],
                     help="c2: the throughput bench (default); c1: 2x2x2 time to rank 7; c3: batched uneven formats; "
                          "c4: integer-ring walks + additions reducer + complexity minimizer; ttt: learned vs uniform")
-    ap.add_argument("--launches", type=int, default=0, help="launch budget of the c1/c3/c4 workloads (0: default)")
+    ap.add_argument("--launches", type=int, default=0, help="launch budget of the c1/c3/c4/descent workloads (0: default)")
+    ap.add_argument("--expand-probability", type=float, default=0.0, help="--expand-probability of the descent workload")
     tt = ap.add_argument_group("time to target (learned vs uniform proposals, 4x4x4 Z2)")
     tt.add_argument("--ttt", action="store_true", help="run the time-to-target study instead of the throughput bench")
     tt.add_argument("--seeds", type=int, default=10)
@@ -524,10 +525,44 @@ def workload_c4(a):
     print(json.dumps(out))
 
 
+def workload_descent(a):
+    """Rank descent of 4x4x4 walks from the naive rank-64 scheme over both
+    rings (65536 walkers, launches of 1000 iterations, --plus-iterations and
+    --expand-probability as given): best and mean rank every 20 launches."""
+    import numpy as np
+
+    import flipgraphgpu_amd as fg
+
+    launches = a.launches or 200
+    out = {"metric": "rank descent of 4x4x4 walks from the naive scheme", "n_gpus": 1, "rings": {}}
+    for name, ring in (("z2", fg.RING_Z2), ("zt", fg.RING_ZT)):
+        e = fg.Engine(65536, a.seed, ring=ring, max_iterations=1000, plus_iterations=a.plus_iterations,
+                      expand_probability=a.expand_probability)
+        e.init_naive(4, 4, 4)
+        c0 = e.counters()
+        t0 = time.perf_counter()
+        traj = []
+        for q in range(launches):
+            if e.walk(1000):
+                raise RuntimeError("invalid schemes")
+            if (q + 1) % 20 == 0:
+                s = e.summary()
+                traj.append({"launches": q + 1, "best_rank": int(s["best_m"].min()),
+                             "mean_rank": round(float(np.mean(s["m"])), 4)})
+                print(f"[descent] {name} {traj[-1]}", file=sys.stderr, flush=True)
+        dt = time.perf_counter() - t0
+        out["rings"][name] = {"seconds": round(dt, 2), "flips_per_s": (e.counters()["flips"] - c0["flips"]) / dt,
+                              "trajectory": traj}
+        e.close()
+    out["config"] = {"workload": f"4x4x4, 65536 walkers, launches of 1000 iterations, plus-iterations "
+                                 f"{a.plus_iterations}, expand-probability {a.expand_probability}"}
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
-    if a.workload in ("c1", "c3", "c4") and a.gpus == 1:
-        {"c1": workload_c1, "c3": workload_c3, "c4": workload_c4}[a.workload](a)
+    if a.workload in ("c1", "c3", "c4", "descent") and a.gpus == 1:
+        {"c1": workload_c1, "c3": workload_c3, "c4": workload_c4, "descent": workload_descent}[a.workload](a)
         return
     if a.workload == "propose" and a.gpus == 1:
         workload_propose(a)

@@ -105,6 +105,7 @@ def load_library():
     L.fg_summary.argtypes = [vp, i32p, i32p, i32p, i32p, i32p]
     L.fg_write_walkers.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     L.fg_log_enable.argtypes = [vp, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
+    L.fg_log_window.argtypes = [vp, ctypes.c_int]
     L.fg_log_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                               ctypes.POINTER(ctypes.c_int64)]
     L.fg_log_read.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, vp, vp, vp, vp]
@@ -260,8 +261,12 @@ class Engine:
         _check(self.L.fg_broadcast_best(self.ctx, int(src), p, len(d)), "fg_broadcast_best")
 
     # ---- learned proposals ----
-    def log_enable(self, max_batches, every=1, horizon=10):
+    def log_enable(self, max_batches, every=1, horizon=10, window=0):
+        """Trajectory log; with window > 0 (horizon 1) a flip is labelled by the
+        first `window` iterations of its own launch."""
         _check(self.L.fg_log_enable(self.ctx, max_batches, every, horizon), "fg_log_enable")
+        if window:
+            _check(self.L.fg_log_window(self.ctx, window), "fg_log_window")
 
     def log_info(self):
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

@@ -78,6 +78,9 @@ FGK_HD uint32_t byte_sum_acc(uint32_t w, uint32_t acc) {
 #endif
 }
 
+#if defined(FGK_PHASE_CLOCKS) && defined(__HIP_DEVICE_COMPILE__)
+extern __device__ unsigned long long g_phase[16];  // phase profile build: cycles per phase, [15] iterations
+#endif
 namespace lane {
 
 constexpr int WS = kWalkersPerGroup;  // LDS stride between consecutive terms of one walker
@@ -140,6 +143,27 @@ struct LaneBaseT {
     int pop;        // coefficient count of the scheme, kept for getComplexity
     bool track_pop; // keep pop current through flips (the minimizer reads it every flip)
     int sel_c, sel_t1, sel_t2;  // pair of the last tryFlip (component, index1 < index2)
+#if defined(FGK_PHASE_CLOCKS) && defined(__HIP_DEVICE_COMPILE__)
+    // Phase profile build only: cycles (s_memtime) spent since the previous
+    // mark, summed per phase: 0 RNG block, 1 pair selection, 2 factor reads
+    // and class-member scan, 3 class walk or mask logic, 4 exchange of the
+    // flip halves, 5 leave / join, 6 bookkeeping and best copy, 7 loop tail,
+    // 8 reduction with rebuild, 9 zero-factor flip.
+    uint32_t ph[10];
+    uint64_t pht;
+    FGK_HD void mark(int k) {
+        const uint64_t t = __builtin_readcyclecounter();
+        ph[k] += (uint32_t)(t - pht);
+        pht = t;
+    }
+    FGK_HD void mark_start() {
+        for (int q = 0; q < 10; ++q) ph[q] = 0;
+        pht = __builtin_readcyclecounter();
+    }
+#else
+    FGK_HD void mark(int) {}
+    FGK_HD void mark_start() {}
+#endif
 
     // Extra per-lane LDS beyond the common planes (none by default).
     FGK_HD void lane_extra(uint8_t *, int) {}
@@ -828,6 +852,7 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     uint32_t nb0 = 0, nb1 = 0, nb2 = 0, nb3 = 0;
     bool have_next = false;
 #endif
+    s.mark_start();
     for (int iter = 0; iter < trips; ++iter) {
         bool flipped = false, copy = false;
         uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
@@ -861,6 +886,7 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
             philox10((uint32_t)it, (uint32_t)(it >> 32), (uint32_t)w, 0u, s.k0, s.k1, b0, b1, b2, b3);
 #endif
             ++it;
+            s.mark(0);
             const int rank = s.m;
             bool ok = false;
             if constexpr (kMode == kModeLearn) {
@@ -899,6 +925,7 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
             }
         }
         copy_best(s, A, copy, lane, w);
+        s.mark(6);
         if (flipped) {
             if (A.use_reduce && s.aux32() < A.thr_reduce) s.try_reduce();  // :550-551
             if (A.use_expand && b3 < A.thr_expand && s.m <= best_m + 2) {   // :553-556
@@ -910,9 +937,16 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
             }
             if (flips_count >= A.plus_iterations && s.try_plus()) flips_count = 0;  // :564-565
         }
-        min_m = s.m < min_m ? s.m : min_m;
+        if (A.min_window <= 0 || iter < A.min_window) min_m = s.m < min_m ? s.m : min_m;
+        s.mark(7);
     }
     if (kMode == kModeLearn && active && s.side == 0 && A.minm) A.minm[w] = min_m;
+#if defined(FGK_PHASE_CLOCKS) && defined(__HIP_DEVICE_COMPILE__)
+    if (active && kMode == kModeWalk) {
+        for (int q = 0; q < 10; ++q) atomicAdd(&g_phase[q], (unsigned long long)s.ph[q]);
+        atomicAdd(&g_phase[15], (unsigned long long)iters);
+    }
+#endif
     }
 
     if (active && s.side == 0) {

@@ -89,6 +89,7 @@ struct fg_ctx {
     struct Log {
         bool on = false;
         int every = 1, horizon = 10, nslot = 0;
+        int window = 0;  // iterations of the logging launch whose ranks label its flip (0: whole launches)
         long long max_batches = 0, launches = 0, labelled = 0, positives = 0;
         uint32_t *ring_words = nullptr, *words = nullptr;
         int32_t *ring_meta = nullptr, *ring_minm = nullptr, *m = nullptr, *pairs = nullptr;
@@ -107,6 +108,7 @@ struct fg_ctx {
             vpairs.clear();
             vy.clear();
             on = false;
+            window = 0;
             launches = labelled = positives = 0;
         }
     } log;
@@ -666,6 +668,7 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced, double resize) {
     const bool learn = resize <= 0 && !ctx->minimize && !ctx->flips && (use_forced || ctx->log.on);
     if (learn && ctx->cfg.ring != FG_RING_Z2) return fail("fg_walk: proposals and trajectory logs are built for Z2 walks");
     fgk::WalkArgs a{};
+    a.args_bytes = (int32_t)sizeof(fgk::WalkArgs);
     a.cur = ctx->cur;
     a.best = ctx->best;
     a.m = ctx->m;
@@ -722,6 +725,7 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced, double resize) {
             a.log_words = ctx->log.ring_words + (size_t)slot * ctx->log.nslot * 3 * ctx->maxm;
             a.log_meta = ctx->log.ring_meta + (size_t)slot * ctx->log.nslot * 4;
             a.minm = ctx->log.ring_minm + (size_t)slot * ctx->W;
+            a.min_window = ctx->log.window;
         }
     }
     if (ctx->minimize) {
@@ -773,6 +777,7 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced, double resize) {
     HIPCHK(hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1));
     int32_t err = 0;
     HIPCHK(hipMemcpy(&err, ctx->err, sizeof err, hipMemcpyDeviceToHost));
+    if (err & fgk::kErrArgs) return fail("fg_walk: the walk kernel's argument layout differs from the host's (a stale build)", -3);
     if (err) return fail("fg_walk: kernel invariant violation, flags " + std::to_string(err), -3);
     std::vector<int32_t> ok(ctx->W);
     HIPCHK(hipMemcpy(ok.data(), ctx->ok, ok.size() * 4, hipMemcpyDeviceToHost));
@@ -830,6 +835,15 @@ extern "C" int fg_log_enable(fg_ctx *ctx, int64_t max_batches, int every, int ho
     HIPCHK(fgk::launch_fill(L.ring_minm, INT_MAX, (int)std::min<size_t>((size_t)horizon * ctx->W, INT_MAX), ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     L.on = true;
+    return 0;
+}
+
+extern "C" int fg_log_window(fg_ctx *ctx, int iterations) {
+    if (!ctx || !ctx->log.on) return fail("fg_log_window: no trajectory log");
+    if (iterations < 0) return fail("fg_log_window: iterations must be >= 0");
+    if (iterations > 0 && ctx->log.horizon != 1)
+        return fail("fg_log_window: an in-launch window needs a log horizon of one launch");
+    ctx->log.window = iterations;
     return 0;
 }
 

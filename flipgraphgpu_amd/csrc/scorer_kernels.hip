@@ -193,23 +193,34 @@ hipError_t launch_head_out(const float *Z, const float *wo, const float *bo, con
     return hipGetLastError();
 }
 
+// One workgroup: the mean BCE loss summed in a fixed order (each thread a
+// fixed stride of pairs, then a fixed tree), so training is reproducible.
 __global__ void bce_kernel(const float *s, const float *y, int P, float *ds, float *loss) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    const float x = s[p];
-    const float sp = x > 0.f ? x + log1pf(expf(-x)) : log1pf(expf(x));  // softplus
-    atomicAdd(loss, (sp - y[p] * x) / (float)P);
-    const float sig = 1.f / (1.f + expf(-x));
-    ds[p] = (sig - y[p]) / (float)P;
+    float acc = 0.f;
+    for (int p = threadIdx.x; p < P; p += blockDim.x) {
+        const float x = s[p];
+        const float sp = x > 0.f ? x + log1pf(expf(-x)) : log1pf(expf(x));  // softplus
+        acc += (sp - y[p] * x) / (float)P;
+        const float sig = 1.f / (1.f + expf(-x));
+        ds[p] = (sig - y[p]) / (float)P;
+    }
+    __shared__ float part[256];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss += part[0];
 }
 
 hipError_t launch_bce(const float *s, const float *y, int P, float *ds, float *loss, hipStream_t st) {
-    hipLaunchKernelGGL(bce_kernel, dim3(blocks(P)), dim3(256), 0, st, s, y, P, ds, loss);
+    hipLaunchKernelGGL(bce_kernel, dim3(1), dim3(256), 0, st, s, y, P, ds, loss);
     return hipGetLastError();
 }
 
 __global__ void head_back_kernel(const float *Z, const float *ds, const float *wo, const int32_t *pairs, int P, int Zd,
-                                 float *dZ, bf16 *dZb, float *dwo, float *dbo) {
+                                 float *dZ, bf16 *dZb) {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long)P * Zd) return;
     const int p = (int)(g / Zd), q = (int)(g % Zd);
@@ -218,20 +229,35 @@ __global__ void head_back_kernel(const float *Z, const float *ds, const float *w
     if (c >= 0) {
         const float z = Z[g];
         d = z > 0.f ? ds[p] * wo[c * Zd + q] : 0.f;
-        atomicAdd(&dwo[c * Zd + q], ds[p] * z);
-        if (q == 0) atomicAdd(&dbo[c], ds[p]);
     }
     dZ[g] = d;
     dZb[g] = (bf16)d;
 }
 
+// Output-layer gradients: one thread per (component c, column q) sums over
+// the pairs in order (q == Zd: the bias), so the sums are reproducible.
+__global__ void head_grad_kernel(const float *Z, const float *ds, const int32_t *pairs, int P, int Zd, float *dwo,
+                                 float *dbo) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= 3 * (Zd + 1)) return;
+    const int c = g / (Zd + 1), q = g % (Zd + 1);
+    float acc = 0.f;
+    for (int p = 0; p < P; ++p)
+        if (pairs[3 * p] == c) acc += q < Zd ? ds[p] * Z[(long)p * Zd + q] : ds[p];
+    if (q < Zd) dwo[c * Zd + q] += acc;
+    else dbo[c] += acc;
+}
+
 hipError_t launch_head_back(const float *Z, const float *ds, const float *wo, const int32_t *pairs, int P, int Zd,
                             float *dZ, bf16 *dZb, float *dwo, float *dbo, hipStream_t st) {
-    hipLaunchKernelGGL(head_back_kernel, dim3(blocks((long)P * Zd)), dim3(256), 0, st, Z, ds, wo, pairs, P, Zd, dZ, dZb,
-                       dwo, dbo);
+    hipLaunchKernelGGL(head_back_kernel, dim3(blocks((long)P * Zd)), dim3(256), 0, st, Z, ds, wo, pairs, P, Zd, dZ, dZb);
+    hipLaunchKernelGGL(head_grad_kernel, dim3(blocks(3 * (Zd + 1))), dim3(256), 0, st, Z, ds, pairs, P, Zd, dwo, dbo);
     return hipGetLastError();
 }
 
+// Backward passes score one pair per tuple (index1 < index2), so the two
+// rows a thread updates are written by no other thread: plain adds, no
+// atomics, and the result does not depend on scheduling.
 __global__ void pair_scatter_kernel(const float *dQ, const float *G, const int32_t *tup, const int32_t *pairs, int P,
                                     int M, int D, float *dG) {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -241,8 +267,8 @@ __global__ void pair_scatter_kernel(const float *dQ, const float *G, const int32
     const long b = tup[p];
     const long r1 = b * M + pairs[3 * p + 1], r2 = b * M + pairs[3 * p + 2];
     const float d1 = dQ[(long)p * 2 * D + d], d2 = dQ[(long)p * 2 * D + D + d];
-    atomicAdd(&dG[r1 * D + d], d1 + d2 * G[r2 * D + d]);
-    atomicAdd(&dG[r2 * D + d], d1 + d2 * G[r1 * D + d]);
+    dG[r1 * D + d] += d1 + d2 * G[r2 * D + d];
+    dG[r2 * D + d] += d1 + d2 * G[r1 * D + d];
 }
 
 hipError_t launch_pair_scatter(const float *dQ, const float *G, const int32_t *tup, const int32_t *pairs, int P, int M,
@@ -469,6 +495,7 @@ __global__ void __launch_bounds__(256) propose_kernel(ProposeArgs a) {
     __shared__ __attribute__((aligned(16))) bf16 Mv[kPD];
     __shared__ float Qv[kPD];
     __shared__ float sc[kProposeMaxK];
+    __shared__ float scw[4][kProposeMaxK];  // head partial sums of each wave, added in wave order
     bf16 *Es = reinterpret_cast<bf16 *>(smem);               // [64][kPEs]
     bf16 *Xs = Es + kProposeMaxTerms * kPEs;                  // [64][kPXs]
     float *Gs = reinterpret_cast<float *>(smem);             // [64][128], after the last layer
@@ -606,7 +633,6 @@ __global__ void __launch_bounds__(256) propose_kernel(ProposeArgs a) {
 #pragma unroll
             for (int e = 0; e < 8; ++e) qs[e] = qm[e] = zero;
         }
-        if (tid < kProposeMaxK) sc[tid] = 0.f;
         __syncthreads();  // every G read is done
         *reinterpret_cast<bf16x8 *>(Qs + k * kPQs + i0) = qs;
         *reinterpret_cast<bf16x8 *>(Qs + k * kPQs + kPD + i0) = qm;
@@ -635,13 +661,14 @@ __global__ void __launch_bounds__(256) propose_kernel(ProposeArgs a) {
         }
         if ((lane & 15) == 0)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) atomicAdd(&sc[4 * (lane >> 4) + q], part[q]);
+            for (int q = 0; q < 4; ++q) scw[wave][4 * (lane >> 4) + q] = part[q];
     }
     __syncthreads();
     if (tid < a.K) {
         const long p = (long)b * a.K + tid;
         const int c = a.pairs[3 * p];
-        const float s = c >= 0 ? a.bo[c] + sc[tid] : -1e30f;
+        const float hs = ((scw[0][tid] + scw[1][tid]) + scw[2][tid]) + scw[3][tid];
+        const float s = c >= 0 ? a.bo[c] + hs : -1e30f;
         sc[tid] = s;
         if (a.scores) a.scores[p] = s;
     }

@@ -11,6 +11,9 @@
 #include "resize_ops.h"
 
 namespace fgk {
+#if defined(FGK_PHASE_CLOCKS)
+__device__ unsigned long long g_phase[16];
+#endif
 
 // Persistent walk: each workgroup (one wave, 32 walkers) claims blocks of
 // walkers from the queue until none is left.  Blocks never straddle two
@@ -21,6 +24,10 @@ namespace fgk {
 template <class Lane, int kMode>
 __device__ __attribute__((always_inline)) void walk_persistent(const WalkArgs &A, uint8_t *lds) {
     const int lane = threadIdx.x;
+    if (A.args_bytes != (int32_t)sizeof(WalkArgs)) {
+        if (lane == 0) atomicOr(A.err, kErrArgs);
+        return;
+    }
     for (;;) {
         int item = 0;
         if (lane == 0) item = atomicAdd(A.queue, 1);
@@ -378,3 +385,15 @@ hipError_t launch_verify(int ring, const VerifyArgs &v, hipStream_t stream) {
 }
 
 }  // namespace fgk
+
+#if defined(FGK_PHASE_CLOCKS)
+// Phase profile build only (make phases): read and clear the per-phase cycle
+// sums of the Z2 walk kernels.
+extern "C" __attribute__((visibility("default"))) int fg_phase_read(unsigned long long *out16) {
+    return hipMemcpyFromSymbol(out16, HIP_SYMBOL(fgk::g_phase), 16 * 8) == hipSuccess ? 0 : -1;
+}
+extern "C" __attribute__((visibility("default"))) int fg_phase_reset(void) {
+    unsigned long long z[16] = {0};
+    return hipMemcpyToSymbol(HIP_SYMBOL(fgk::g_phase), z, 16 * 8) == hipSuccess ? 0 : -1;
+}
+#endif

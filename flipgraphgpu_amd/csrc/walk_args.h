@@ -18,6 +18,12 @@ constexpr int kWalkersPerGroup = kThreadsPerGroup / kLanesPerWalker;
 // word(c, t, w) = words[(c * max_m + t) * walkers + w], so a wave touching
 // term t of 64 consecutive walkers reads 64 consecutive words.
 struct WalkArgs {
+    // The first two fields keep their offsets whatever is added below: the
+    // kernel compares args_bytes with its own sizeof(WalkArgs) and, on a
+    // mismatch (a stale object of a different layout), only raises
+    // kErrArgs through err and returns.
+    int32_t args_bytes;
+    int32_t *err;         // invariant violations (bit flags), host checks after each launch
     uint32_t *cur;        // Z2 factor masks / ZT coefficient magnitudes
     uint32_t *cur_s;      // ZT signs (null for Z2)
     uint32_t *best;       // best schemes, per walker [c][t]: best[(w * 3 + c) * max_m + t]
@@ -47,7 +53,6 @@ struct WalkArgs {
     uint32_t thr_best_eq;
     int32_t check_reduce;
     int32_t zt_visit_cap;
-    int32_t *err;         // invariant violations (bit flags), host checks after each launch
     // persistent launch: workgroups take blocks [item_begin[q], item_end[q])
     // of at most kWalkersPerGroup walkers from a queue until it runs dry
     const int32_t *item_begin;
@@ -60,7 +65,8 @@ struct WalkArgs {
     int32_t log_every;      // walkers w with w % log_every == 0 log their first flip (0: none)
     uint32_t *log_words;    // [walkers / log_every][3][max_m] scheme before that flip
     int32_t *log_meta;      // [walkers / log_every][4]: component (-1: no flip), index1, index2, rank
-    int32_t *minm;          // [walkers] lowest rank during the launch
+    int32_t *minm;          // [walkers] lowest rank during the launch (or its first min_window iterations)
+    int32_t min_window;     // iterations whose ranks count toward minm (0: the whole launch)
     // resize launches (Z2, resizeKernel)
     int32_t *bfmt;            // [walkers][4] format of each walker's best scheme (null: same as fmt)
     uint64_t thr_resize;      // a walker resizes when its first aux draw is below this (at most 2^32)
@@ -77,6 +83,7 @@ struct WalkArgs {
 constexpr int32_t kErrSelect = 1;   // pair draw found no term (partner counts inconsistent)
 constexpr int32_t kErrWalk = 2;     // class list walk exceeded the term count
 constexpr int32_t kErrCapacity = 4; // a term would exceed the LDS capacity
+constexpr int32_t kErrArgs = 8;     // the argument block's layout differs from the kernel's
 
 // Padding of the packed H / N planes: a multiple of 32 terms (8 words of 4
 // terms) so the selection scan reads whole 8-word blocks.

@@ -163,10 +163,12 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
         hf.t = 1 << 30;
         hf.rule = 0;
         hf.any = this->find_last(mc, nv);
+        this->mark(2);
         if (hf.any >= 0)
             class_walk(mc, x, hf.any, i, oc, hf.ji,
                        [&](uint32_t a, uint32_t b) { return !check_reduce ? 0 : (a == fi ? 1 : (b == ov ? 2 : 0)); },
                        hf.t, hf.rule);
+        this->mark(3);
         return hf;
     }
 
@@ -182,6 +184,7 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
         const uint32_t B = fk2 ^ fk1;
         if (!(fj1 && fj2 && fk1 && fk2 && A && B)) {
             flip_literal(i, j, k, i1, i2, check_reduce);
+            this->mark(9);
             return;
         }
         if constexpr (kMaskT) {
@@ -196,6 +199,7 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
         const int ot = pair_exchange(mine.t, side), orule = pair_exchange(mine.rule, side);
         const int tA = side ? ot : mine.t, rA = side ? orule : mine.rule;
         const int tB = side ? mine.t : ot, rB = side ? mine.rule : orule;
+        this->mark(4);
 #else
         const Half ha = flip_half(j, i1, A, i, k, fi, fk1, check_reduce);
         const Half hb = flip_half(k, i2, B, i, j, fi, fj2, check_reduce);
@@ -206,6 +210,7 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
             f(k, i2) = B;
             if (rA && tA <= tB) reduce(rA == 1 ? k : i, tA, i1);
             else reduce(rB == 1 ? j : i, tB, i2);
+            this->mark(8);
             return;
         }
         pop_update(fj1, A, fk2, B);
@@ -215,6 +220,7 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
         if (mine.any >= 0) link_join(mc, x, mine.ji);
         f(mc, x) = nv;
         addS(oc, pair_exchange((int)(Sc(mc) - before), side));
+        this->mark(5);
 #else
         leave(j, i1);
         if (ha.any >= 0) link_join(j, i1, ha.ji);
@@ -246,10 +252,12 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
         h.old = cm(mc, x);
         const uint32_t ci = cm(i, x), co = cm(oc, x);
         const int z = this->find_last(mc, nv);
+        this->mark(2);
         h.nw = z >= 0 ? cm(mc, z) : 0u;
         const uint32_t red = check_reduce ? (h.nw & (ci | co)) : 0u;
         h.t = red ? __builtin_ctz(red) : (1 << 30);
         h.rule = red ? (((ci >> h.t) & 1u) ? 1 : 2) : 0;
+        this->mark(3);
         return h;
     }
     // x leaves class `old` of component mc and joins `nw`; its factor becomes
@@ -277,6 +285,7 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
         const int ot = pair_exchange(mine.t, side), orule = pair_exchange(mine.rule, side);
         const int tA = side ? ot : mine.t, rA = side ? orule : mine.rule;
         const int tB = side ? mine.t : ot, rB = side ? mine.rule : orule;
+        this->mark(4);
 #else
         const MaskHalf ha = mask_half(j, i1, A, i, k, check_reduce);
         const MaskHalf hb = mask_half(k, i2, Bv, i, j, check_reduce);
@@ -287,6 +296,7 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
             f(k, i2) = Bv;
             if (rA && tA <= tB) reduce(rA == 1 ? k : i, tA, i1);
             else reduce(rB == 1 ? j : i, tB, i2);
+            this->mark(8);
             return;
         }
         pop_update(fj1, A, fk2, Bv);
@@ -294,6 +304,7 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
         const int d = mask_move(mc, x, mine.old, mine.nw, nv);
         const int od = pair_exchange(d, side);
         const int dj = side ? od : d, dk = side ? d : od;
+        this->mark(5);
 #else
         const int dj = mask_move(j, i1, ha.old, ha.nw, A);
         const int dk = mask_move(k, i2, hb.old, hb.nw, Bv);
@@ -447,6 +458,7 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
         if (!total) return false;
         int c, t1, t2, j, k;
         if (!select_pair(draw_below(r, total), c, t1, t2)) return false;
+        this->mark(1);
         sel_c = c;
         sel_t1 = t1;
         sel_t2 = t2;

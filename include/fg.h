@@ -257,6 +257,11 @@ FG_API int fg_write_walkers(fg_ctx *ctx, int begin, int count, const fg_walker_r
  * the walker's rank fell below the rank at that flip in the meantime, else
  * 0.  At most max_batches launches are kept. */
 FG_API int fg_log_enable(fg_ctx *ctx, int64_t max_batches, int every, int horizon);
+/* Labels from a window of the logging launch itself (log horizon 1): a
+ * logged flip is labelled 1 when the walker's rank falls below the rank at
+ * the flip within the launch's first `iterations` iterations (the flip's
+ * own included); 0 restores whole-launch labels. */
+FG_API int fg_log_window(fg_ctx *ctx, int iterations);
 /* Labelled rows, rows labelled 1, labelled launches. */
 FG_API int fg_log_info(fg_ctx *ctx, int64_t *rows, int64_t *positives, int64_t *batches);
 /* Labelled rows begin..begin+count-1: words [count][3][max_terms] (factor

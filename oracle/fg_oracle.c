@@ -949,7 +949,7 @@ void fgo_walk(fgo_walker *w, const fgo_walk_params *p) {
         }
         if (!ok) { /* :535-537 */
             try_expand(s, 1 + (int)(b[1] % 2u), &w->rng);
-            if (s->m < w->min_m) w->min_m = s->m;
+            if ((p->min_window <= 0 || it < p->min_window) && s->m < w->min_m) w->min_m = s->m;
             continue;
         }
         w->total_flips++;
@@ -968,7 +968,7 @@ void fgo_walk(fgo_walker *w, const fgo_walk_params *p) {
         if (p->use_sandwich && fgo_rng_aux32(&w->rng) < p->thr_sandwich && s->ring == FGO_RING_Z2) sandwiching(s, &w->rng);
         if (p->use_basis && fgo_rng_aux32(&w->rng) < p->thr_basis) swap_basis(s, &w->rng);
         if (w->flips_count >= p->plus_iterations && try_plus(s, &w->rng)) w->flips_count = 0; /* :564-565 */
-        if (s->m < w->min_m) w->min_m = s->m;
+        if ((p->min_window <= 0 || it < p->min_window) && s->m < w->min_m) w->min_m = s->m;
     }
 }
 

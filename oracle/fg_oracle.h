@@ -91,6 +91,7 @@ typedef struct {
     uint32_t thr_sandwich;
     int use_basis;         /* basis-change edge (Z2) */
     uint32_t thr_basis;
+    int min_window;        /* iterations whose ranks count toward min_m (0: the whole launch) */
 } fgo_walk_params;
 
 typedef struct {

@@ -94,6 +94,50 @@ def test_trajectory_log_matches_oracle():
         assert pr in record_pairs(st)
 
 
+def test_window_labels_match_oracle():
+    # labels from the first WIN iterations of the logging launch itself
+    print("[learned] window labels", flush=True)
+    W, L, WIN, n = 128, 150, 20, (3, 3, 3)
+    p = O.make_params(iterations=L, plus_iterations=400, min_window=WIN)
+    with fg.Engine(W, 6, max_iterations=L, plus_iterations=400) as e:
+        e.init_naive(*n)
+        e.log_enable(max_batches=8, every=1, horizon=1, window=WIN)
+        for _ in range(6):
+            assert e.walk(L) == 0
+        info = e.log_info()
+        assert info["batches"] == 6
+        words, m, pairs, y = e.log_read(0, info["rows"], 27)
+    recs = O.naive_records(*n, W)
+    rows = []
+    for s in range(6):
+        st = recs.copy()
+        recs, log, mm = O.walk_records_forced(recs, p, 6, None)
+        for w in range(W):
+            if log[w, 0] >= 0:
+                rows.append((st[w], tuple(log[w, :3]), int(log[w, 3]), float(mm[w] < log[w, 3])))
+    assert len(rows) == info["rows"] and info["positives"] == sum(r[3] for r in rows) > 0
+    for q, (st, pr, rank, lab) in enumerate(rows):
+        assert tuple(pairs[q]) == pr and m[q] == rank and y[q] == lab
+        assert np.array_equal(words[q][:, :rank], st["cur_v"][:, :rank].astype(np.uint32))
+
+
+def test_training_is_deterministic():
+    # fixed-order reductions everywhere: the same log, seeds and steps give
+    # the same weights, bit for bit
+    print("[learned] deterministic training", flush=True)
+    with fg.Engine(2048, 4, max_iterations=100, plus_iterations=2000) as e:
+        e.init_naive(4, 4, 4)
+        e.log_enable(max_batches=6, every=1, horizon=2)
+        for _ in range(7):
+            assert e.walk(100) == 0
+        out = []
+        for _ in range(2):
+            with fg.Scorer(4, 4, 4, 64, seed=2) as sc:
+                losses = sc.train(e, steps=30, batch=256, lr=2e-3, seed=5)
+                out.append((sc.get_params(), losses))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_training_and_proposals():
     print("[learned] train + propose 4x4x4", flush=True)
     W = 4096
