@@ -55,6 +55,8 @@ class FgConfig(ctypes.Structure):
         ("sandwiching_probability", ctypes.c_double),
         ("basis_probability", ctypes.c_double),
         ("partner_layout", ctypes.c_int32),
+        ("lookahead_period", ctypes.c_int32),
+        ("lookahead_candidates", ctypes.c_int32),
         ("reserved0", ctypes.c_int32),
     ]
 
@@ -169,13 +171,14 @@ class Engine:
     def __init__(self, walkers, seed, ring=RING_Z2, max_iterations=10000, plus_iterations=8000,
                  random_iterations=False, check_reduce=True, expand_probability=0.0,
                  reduce_probability=0.0, max_rank=0, zt_visit_cap=16, use_graph=False, grid=0, word_bits=0,
-                 sandwiching_probability=0.0, basis_probability=0.0, partner_layout=0):
+                 sandwiching_probability=0.0, basis_probability=0.0, partner_layout=0, lookahead_period=0,
+                 lookahead_candidates=0):
         self.L = load_library()
         self.ring = ring
         self.cfg = FgConfig(ring, walkers, max_iterations, plus_iterations, int(random_iterations),
                             int(check_reduce), expand_probability, reduce_probability, seed, max_rank,
                             zt_visit_cap, int(use_graph), grid, word_bits, sandwiching_probability,
-                            basis_probability, partner_layout, 0)
+                            basis_probability, partner_layout, lookahead_period, lookahead_candidates, 0)
         self.ctx = ctypes.c_void_p()
         _check(self.L.fg_create(ctypes.byref(self.cfg), ctypes.byref(self.ctx)), "fg_create")
         self.walkers = walkers

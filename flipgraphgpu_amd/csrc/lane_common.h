@@ -670,6 +670,7 @@ constexpr int kModeLearn = 2;     // walk + proposed first flip + trajectory log
 constexpr int kModeFlips = 3;     // flipSchemesKernel of the additions reducer
 constexpr int kModeResize = 4;    // resizeKernel (Z2)
 constexpr int kModeMeta = 5;      // walk with the sandwiching and basis-change edges (Z2)
+constexpr int kModeLook = 6;      // walk with reduction-seeking lookahead proposals (Z2)
 
 // Resize draw thresholds: a uniform compare u < q is x < floor(q * 2^32).
 constexpr uint32_t kThrHalf = 2147483648u, kThr005 = 214748364u, kThr055 = 2362232012u, kThr085 = 3650722201u;
@@ -906,6 +907,9 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
                     lm[2] = s.sel_t2;
                     lm[3] = rank;
                 }
+            } else if constexpr (kMode == kModeLook) {
+                if (iter % A.look_period == A.look_period - 1) ok = s.try_lookahead(A.look_candidates, check_reduce);
+                if (!ok) ok = s.try_flip(b0, b1, check_reduce);
             } else {
                 ok = s.try_flip(b0, b1, check_reduce);
             }

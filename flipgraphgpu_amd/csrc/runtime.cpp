@@ -213,6 +213,9 @@ extern "C" int fg_create(const fg_config *cfg, fg_ctx **out) {
     if (cfg->max_iterations <= 0) return fail("fg_create: max_iterations must be positive");
     if (cfg->plus_iterations < 0) return fail("fg_create: plus_iterations must be >= 0");
     if (cfg->word_bits != 0 && cfg->word_bits != 16 && cfg->word_bits != 32) return fail("fg_create: word_bits must be 0, 16 or 32");
+    if (cfg->lookahead_period < 0 || cfg->lookahead_candidates < 0 ||
+        (cfg->lookahead_candidates > 0 && (cfg->lookahead_period < 1 || cfg->ring != FG_RING_Z2)))
+        return fail("fg_create: lookahead proposals need a positive period and a Z2 pool");
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
@@ -397,6 +400,7 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
     if (mode == 0 && ring == FG_RING_Z2) {
         int g2 = 0;
         HIPCHK(fgk::prepare_walk(variant, maxm, 2, &g2));
+        HIPCHK(fgk::prepare_walk(variant, maxm, 6, &g2));
         if (variant == fgk::kVarZ2) HIPCHK(fgk::prepare_walk(variant, maxm, 4, &g2));
         if (variant == fgk::kVarZ2) HIPCHK(fgk::prepare_walk(variant, maxm, 5, &g2));
     }
@@ -404,6 +408,11 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
         int g2 = 0;
         HIPCHK(fgk::prepare_walk(variant, maxm, 4, &g2));
         HIPCHK(fgk::prepare_walk(variant, maxm, 5, &g2));
+    }
+    if (mode == 0 && ring == FG_RING_Z2 && ctx->cfg.lookahead_candidates > 0) {
+        int g6 = 0;  // the lookahead kernel is the one the pool launches
+        HIPCHK(fgk::prepare_walk(variant, maxm, 6, &g6));
+        ctx->groups_per_cu = std::min(ctx->groups_per_cu, g6);
     }
     if (ctx->groups_per_cu < 1) return fail("init: the walk kernel does not fit on a CU with this scheme size");
     ctx->grid = ctx->cfg.grid > 0 ? ctx->cfg.grid : ctx->groups_per_cu * ctx->cus;
@@ -704,7 +713,11 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced, double resize) {
     a.cx = ctx->cx;
     const bool meta = ctx->cfg.sandwiching_probability > 0 || ctx->cfg.basis_probability > 0;
     if (meta && learn) return fail("fg_walk: proposals and trajectory logs do not combine with basis-change edges");
-    const int mode = resize > 0 ? 4 : (ctx->minimize ? 1 : (ctx->flips ? 3 : (learn ? 2 : (meta ? 5 : 0))));
+    const bool look = ctx->cfg.lookahead_candidates > 0 && ctx->cfg.ring == FG_RING_Z2;
+    if (look && (learn || meta)) return fail("fg_walk: lookahead proposals do not combine with learned proposals or basis edges");
+    const int mode = resize > 0 ? 4 : (ctx->minimize ? 1 : (ctx->flips ? 3 : (learn ? 2 : (meta ? 5 : (look ? 6 : 0)))));
+    a.look_period = ctx->cfg.lookahead_period;
+    a.look_candidates = ctx->cfg.lookahead_candidates;
     a.use_sandwich = ctx->cfg.sandwiching_probability > 0;
     a.thr_sandwich = threshold(ctx->cfg.sandwiching_probability, ctx->cfg.max_iterations);
     a.use_basis = ctx->cfg.basis_probability > 0;
@@ -900,6 +913,8 @@ int use_variant(fg_ctx *ctx, int variant, int *groups_per_cu) {
     if (mode == 0 && ctx->cfg.ring == FG_RING_Z2) {
         int g2 = 0;
         if (fgk::prepare_walk(variant, ctx->maxm, 2, &g2) != hipSuccess) return -1;
+        if (fgk::prepare_walk(variant, ctx->maxm, 6, &g2) != hipSuccess) return -1;
+        if (ctx->cfg.lookahead_candidates > 0) g = std::min(g, g2);
     }
     ctx->variant = variant;
     ctx->groups_per_cu = g;

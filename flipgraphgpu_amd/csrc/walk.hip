@@ -117,6 +117,22 @@ FGK_WALK_KERNEL walk_meta_zt_kernel(WalkArgs A) {
     walk_persistent<lane::ZTLane, lane::kModeMeta>(A, lds);
 }
 
+// Walk with reduction-seeking lookahead proposals (Z2, every layout).
+FGK_WALK_KERNEL look_z2_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2Lane, lane::kModeLook>(A, lds);
+}
+
+FGK_WALK_KERNEL look_z2h_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2Lane16, lane::kModeLook>(A, lds);
+}
+
+FGK_WALK_KERNEL look_z2m_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2LaneM, lane::kModeLook>(A, lds);
+}
+
 // Walk with a proposed first flip and trajectory logging (Z2).
 FGK_WALK_KERNEL learn_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
@@ -303,19 +319,21 @@ size_t walk_lds_bytes(int variant, int max_m) {
 }
 
 static const void *walk_kernel(int variant, int mode) {
-    const void *k[4][6] = {
+    const void *k[4][7] = {
         {reinterpret_cast<const void *>(walk_z2_kernel), reinterpret_cast<const void *>(minimize_z2_kernel),
          reinterpret_cast<const void *>(learn_z2_kernel), nullptr, reinterpret_cast<const void *>(resize_z2_kernel),
-         reinterpret_cast<const void *>(walk_meta_z2_kernel)},
+         reinterpret_cast<const void *>(walk_meta_z2_kernel), reinterpret_cast<const void *>(look_z2_kernel)},
         {reinterpret_cast<const void *>(walk_zt_kernel), reinterpret_cast<const void *>(minimize_zt_kernel), nullptr,
          reinterpret_cast<const void *>(flips_zt_kernel), reinterpret_cast<const void *>(resize_zt_kernel),
-         reinterpret_cast<const void *>(walk_meta_zt_kernel)},
+         reinterpret_cast<const void *>(walk_meta_zt_kernel), nullptr},
         {reinterpret_cast<const void *>(walk_z2h_kernel), reinterpret_cast<const void *>(minimize_z2h_kernel),
-         reinterpret_cast<const void *>(learn_z2h_kernel), nullptr, nullptr, nullptr},
+         reinterpret_cast<const void *>(learn_z2h_kernel), nullptr, nullptr, nullptr,
+         reinterpret_cast<const void *>(look_z2h_kernel)},
         {reinterpret_cast<const void *>(walk_z2m_kernel), reinterpret_cast<const void *>(minimize_z2m_kernel),
-         reinterpret_cast<const void *>(learn_z2m_kernel), nullptr, nullptr, nullptr},
+         reinterpret_cast<const void *>(learn_z2m_kernel), nullptr, nullptr, nullptr,
+         reinterpret_cast<const void *>(look_z2m_kernel)},
     };
-    if (variant < 0 || variant > 3 || mode < 0 || mode > 5) return nullptr;
+    if (variant < 0 || variant > 3 || mode < 0 || mode > 6) return nullptr;
     return k[variant][mode];
 }
 

@@ -67,6 +67,10 @@ struct WalkArgs {
     int32_t *log_meta;      // [walkers / log_every][4]: component (-1: no flip), index1, index2, rank
     int32_t *minm;          // [walkers] lowest rank during the launch (or its first min_window iterations)
     int32_t min_window;     // iterations whose ranks count toward minm (0: the whole launch)
+    // lookahead launches (Z2): every look_period-th iteration of a launch first
+    // tries look_candidates extra pairs and applies the first that reduces
+    int32_t look_period;
+    int32_t look_candidates;
     // resize launches (Z2, resizeKernel)
     int32_t *bfmt;            // [walkers][4] format of each walker's best scheme (null: same as fmt)
     uint64_t thr_resize;      // a walker resizes when its first aux draw is below this (at most 2^32)

@@ -315,6 +315,64 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
         S2 += (uint32_t)(j == 2 ? dj : (k == 2 ? dk : 0));
     }
 
+    // ---- reduction-seeking lookahead proposals (no reference counterpart) ----
+    // Whether flip(i, j, k, i1, i2) would lower the rank: a zero factor, or a
+    // term that afterwards matches i1 or i2 in two components (the reduction
+    // flip() takes).  Read-only.  One half per lane of the pair, as in flip.
+    FGK_HD bool half_reduces(int mc, int x, uint32_t nv, int i, int oc, uint32_t fi, uint32_t ov) {
+        const int z = this->find_last(mc, nv);
+        if (z < 0) return false;
+        if constexpr (kMaskT) {
+            return (cm(mc, z) & (cm(i, x) | cm(oc, x))) != 0u;
+        } else {
+            int y = z, steps = 0;
+            do {
+                if (f(i, y) == fi || f(oc, y) == ov) return true;
+                y = nx(mc, y);
+            } while (y != z && ++steps <= m);
+            return false;
+        }
+    }
+    FGK_HD bool flip_reduces(int i, int j, int k, int i1, int i2) {
+        const uint32_t fj1 = f(j, i1), fj2 = f(j, i2), fk1 = f(k, i1), fk2 = f(k, i2);
+        const uint32_t A = fj1 ^ fj2, B = fk2 ^ fk1;
+        if (!(fj1 && fj2 && fk1 && fk2 && A && B)) return true;
+        const uint32_t fi = f(i, i1);
+#if defined(__HIP_DEVICE_COMPILE__)
+        const int mc = side ? k : j, x = side ? i2 : i1, oc = side ? j : k;
+        const int mine = half_reduces(mc, x, side ? B : A, i, oc, fi, side ? fj2 : fk1) ? 1 : 0;
+        return (mine | pair_exchange(mine, side)) != 0;
+#else
+        return half_reduces(j, i1, A, i, k, fi, fk1) || half_reduces(k, i2, B, i, j, fi, fj2);
+#endif
+    }
+    // `count` candidates drawn as tryFlip draws them (an aux word for the
+    // pair in fresh-list order, an aux word for the orientation coins); the
+    // first whose flip lowers the rank is applied.  False when none does.
+    FGK_HD bool try_lookahead(int count, bool check_reduce) {
+        const uint32_t total = S0 + S1 + S2;
+        if (!total) return false;
+        for (int q = 0; q < count; ++q) {
+            const uint32_t x = aux32(), coin = aux32();
+            int c, t1, t2, j, k;
+            if (!select_pair(draw_below(x, total), c, t1, t2)) return false;
+            orientation(c, coin, j, k);
+            if (coin & 2u) {
+                const int t = t1;
+                t1 = t2;
+                t2 = t;
+            }
+            if (flip_reduces(c, j, k, t1, t2)) {
+                sel_c = c;
+                sel_t1 = t1 < t2 ? t1 : t2;
+                sel_t2 = t1 < t2 ? t2 : t1;
+                flip(c, j, k, t1, t2, check_reduce);
+                return true;
+            }
+        }
+        return false;
+    }
+
     // plus (scheme_z2.cu:579-612)
     FGK_HD void plus(int i, int j, int k, int i1, int i2, int variant) {
         const uint32_t a1 = f(i, i1), b1 = f(j, i1), c1 = f(k, i1);

@@ -50,6 +50,11 @@ typedef struct {
     int32_t partner_layout;     /* Z2 flip-partner bookkeeping: 0 = class masks when the scheme has at most 32
                                    terms and 16-coefficient factors (and word_bits is 0), else partner lists;
                                    1 = partner lists; 2 = class masks */
+    int32_t lookahead_period;   /* Z2 reduction-seeking proposals (no reference counterpart; 0 = off): every
+                                   lookahead_period-th iteration of a launch first draws lookahead_candidates
+                                   pairs with orientation coins from the aux stream and applies the first whose
+                                   flip lowers the rank, else flips uniformly as usual */
+    int32_t lookahead_candidates;
     int32_t reserved0;
 } fg_config;
 

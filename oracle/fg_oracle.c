@@ -581,6 +581,40 @@ static int try_flip_z2(fgo_scheme *s, const uint32_t b[4], const fgo_walk_params
     return 1;
 }
 
+/* Reduction-seeking lookahead (no reference counterpart): whether flip(i, j,
+ * k, i1, i2) of the Z2 scheme would lower the rank -- a zero factor (the term
+ * disappears) or a term that afterwards matches i1 or i2 in two components
+ * (the reduction flip() takes, scheme_z2.cu:544-576). */
+static int z2_flip_reduces(const fgo_scheme *s, int i, int j, int k, int i1, int i2) {
+    const uint64_t A = s->v[j][i1] ^ s->v[j][i2], B = s->v[k][i2] ^ s->v[k][i1];
+    if (!A || !B || !s->v[j][i1] || !s->v[j][i2] || !s->v[k][i1] || !s->v[k][i2]) return 1;
+    for (int t = 0; t < s->m; t++) {
+        if (t != i1 && s->v[j][t] == A && (s->v[i][t] == s->v[i][i1] || s->v[k][t] == s->v[k][i1])) return 1;
+        if (t != i2 && s->v[k][t] == B && (s->v[i][t] == s->v[i][i2] || s->v[j][t] == s->v[j][i2])) return 1;
+    }
+    return 0;
+}
+
+/* Lookahead proposals: `count` candidates drawn as tryFlip draws them (one
+ * aux word picks the pair in fresh-list order, one gives the orientation
+ * coins); the first whose flip lowers the rank is applied.  0 when none does. */
+static int try_lookahead_z2(fgo_scheme *s, fgo_rng *r, int count, int check_reduce) {
+    const int size = fgo_count_pairs(s);
+    if (!size) return 0;
+    for (int q = 0; q < count; q++) {
+        const uint32_t x = fgo_rng_aux32(r), coin = fgo_rng_aux32(r);
+        int c, t1, t2, i, j, k;
+        fgo_pick_canonical(s, draw_below(x, (uint32_t)size), &c, &t1, &t2);
+        pick_orientation(c, (int)coin, &i, &j, &k);
+        if (coin & 2) { int t = t1; t1 = t2; t2 = t; }
+        if (z2_flip_reduces(s, i, j, k, t1, t2)) {
+            flip_z2(s, i, j, k, t1, t2, check_reduce);
+            return 1;
+        }
+    }
+    return 0;
+}
+
 /* A given pair flipped with tryFlip's orientation coins; 0 when the two
  * factors differ (the pair is no longer a flip). */
 static int try_flip_pair_z2(fgo_scheme *s, int c, int t1, int t2, uint32_t coin, int check_reduce) {
@@ -939,6 +973,9 @@ void fgo_walk(fgo_walker *w, const fgo_walk_params *p) {
         int ok = 0;
         if (it == 0 && w->forced[0] >= 0 && s->ring == FGO_RING_Z2)
             ok = try_flip_pair_z2(s, w->forced[0], w->forced[1], w->forced[2], b[1], p->check_reduce);
+        if (!ok && p->look_candidates > 0 && p->look_period > 0 && s->ring == FGO_RING_Z2 &&
+            it % p->look_period == p->look_period - 1)
+            ok = try_lookahead_z2(s, &w->rng, p->look_candidates, p->check_reduce);
         if (!ok) ok = s->ring == FGO_RING_Z2 ? try_flip_z2(s, b, p) : try_flip_zt(s, b, &w->rng, p);
         if (it == 0) {
             w->forced[0] = -1;

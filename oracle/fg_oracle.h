@@ -92,6 +92,8 @@ typedef struct {
     int use_basis;         /* basis-change edge (Z2) */
     uint32_t thr_basis;
     int min_window;        /* iterations whose ranks count toward min_m (0: the whole launch) */
+    int look_period;       /* Z2 lookahead proposals: every look_period-th iteration of a launch tries */
+    int look_candidates;   /* look_candidates extra pairs and takes the first that reduces (0: off) */
 } fgo_walk_params;
 
 typedef struct {

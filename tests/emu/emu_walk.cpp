@@ -22,7 +22,7 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                                    int mode, const int32_t *forced, int32_t *log_meta, int32_t *minm,
                                    int force_wide, uint64_t thr_resize, int resize_max_rank, int resize_max_coef,
                                    int use_sandwich, uint32_t thr_sandwich, int use_basis, uint32_t thr_basis,
-                                   int32_t *err_out) {
+                                   int look_period, int look_candidates, int32_t *err_out) {
     int maxm = 1;
     for (int q = 0; q < count; ++q) {
         maxm = std::max(maxm, std::max(recs[q].m, recs[q].best_m));
@@ -65,6 +65,8 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
     a.thr_sandwich = thr_sandwich;
     a.use_basis = use_basis;
     a.thr_basis = thr_basis;
+    a.look_period = look_period;
+    a.look_candidates = look_candidates;
     std::vector<uint32_t> log_words((size_t)W * 3 * maxm);
     std::vector<int32_t> lm((size_t)W * 4, -1), mm(W, 0);
     if (mode == fgk::lane::kModeLearn) {
@@ -106,6 +108,10 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                     walk_lane<Z2Lane, kModeResize>(a, L, lane, w, W);
                 } else if (mode == kModeMeta) {
                     walk_lane<Z2Lane, kModeMeta>(a, L, lane, w, W);
+                } else if (mode == kModeLook) {
+                    if (masks) walk_lane<Z2LaneM, kModeLook>(a, L, lane, w, W);
+                    else if (half) walk_lane<Z2Lane16, kModeLook>(a, L, lane, w, W);
+                    else walk_lane<Z2Lane, kModeLook>(a, L, lane, w, W);
                 } else if (masks) {
                     if (mode == kModeMinimize) walk_lane<Z2LaneM, kModeMinimize>(a, L, lane, w, W);
                     else if (mode == kModeLearn) walk_lane<Z2LaneM, kModeLearn>(a, L, lane, w, W);

@@ -23,7 +23,7 @@ def lib():
                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                   ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32,
-                                  ctypes.POINTER(ctypes.c_int32)]
+                                  ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
 
@@ -48,6 +48,8 @@ def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=Fals
     mode = 2 if learn else (4 if resize is not None else p.mode)
     if mode == 0 and (p.use_sandwich or p.use_basis):
         mode = 5  # the meta-walk kernel
+    if mode == 0 and p.look_candidates > 0:
+        mode = 6  # the lookahead kernel
     thr, rmax, cmax = resize if resize is not None else (0, 0, 0)
     fz = None if forced is None else np.ascontiguousarray(forced, dtype=np.int32)
     lm = np.zeros((len(recs), 4), dtype=np.int32)
@@ -57,7 +59,8 @@ def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=Fals
                                 p.use_expand, p.thr_expand, p.thr_best_eq, p.check_reduce, launches,
                                 p.zt_visit_cap, mode, None if fz is None else fz.ctypes.data,
                                 lm.ctypes.data, mm.ctypes.data, LAYOUTS[layout], thr, rmax, cmax,
-                                p.use_sandwich, p.thr_sandwich, p.use_basis, p.thr_basis, ctypes.byref(err))
+                                p.use_sandwich, p.thr_sandwich, p.use_basis, p.thr_basis, p.look_period,
+                                p.look_candidates, ctypes.byref(err))
     assert rc == 0
     if learn:
         return recs, err.value, lm, mm

@@ -57,6 +57,8 @@ class WalkParams(ctypes.Structure):
         ("use_basis", ctypes.c_int),
         ("thr_basis", ctypes.c_uint32),
         ("min_window", ctypes.c_int),
+        ("look_period", ctypes.c_int),
+        ("look_candidates", ctypes.c_int),
     ]
 
 
@@ -71,14 +73,15 @@ THR_BEST_EQ = 42949673  # 0.01 * 2^32, rounded up
 
 def make_params(iterations=1000, max_iterations=None, plus_iterations=8000, p_expand=0.0,
                 p_reduce=0.0, random_iterations=False, select_mode=SELECT_CANONICAL,
-                check_reduce=True, zt_visit_cap=16, mode=0, p_sandwich=0.0, p_basis=0.0, min_window=0):
+                check_reduce=True, zt_visit_cap=16, mode=0, p_sandwich=0.0, p_basis=0.0, min_window=0,
+                look_period=0, look_candidates=0):
     mi = max_iterations if max_iterations is not None else iterations
     return WalkParams(iterations, mi, int(random_iterations), plus_iterations,
                       int(p_reduce > 0), threshold(p_reduce, mi),
                       int(p_expand > 0), threshold(p_expand, mi),
                       THR_BEST_EQ, select_mode, int(check_reduce), zt_visit_cap, mode,
                       int(p_sandwich > 0), threshold(p_sandwich, mi), int(p_basis > 0), threshold(p_basis, mi),
-                      min_window)
+                      min_window, look_period, look_candidates)
 
 
 def _build():

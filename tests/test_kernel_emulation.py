@@ -83,3 +83,21 @@ def test_emulated_wide_words_match_oracle(n):
     ora = O.walk_records(start, p, 77, launches=2, nthreads=8)
     assert err == 0
     assert first_mismatch(emu, ora) is None, first_mismatch(emu, ora)
+
+
+@pytest.mark.parametrize("layout", ["masks", "half", "wide"])
+@pytest.mark.parametrize("n", [(3, 3, 3), (4, 4, 4)])
+def test_emulated_lookahead_matches_oracle(n, layout):
+    # reduction-seeking lookahead proposals every 2nd iteration, 3 candidates
+    if n == (4, 4, 4) and layout == "masks":
+        pytest.skip("class masks hold at most 32 terms")
+    start = O.naive_records(*n, 32)
+    p = O.make_params(iterations=1500, plus_iterations=300, p_expand=50.0, look_period=2, look_candidates=3)
+    emu, err = E.walk(start, p, 21, launches=2, layout=layout)
+    ora = O.walk_records(start, p, 21, launches=2, nthreads=8)
+    assert err == 0
+    assert first_mismatch(emu, ora) is None, first_mismatch(emu, ora)
+    if n == (3, 3, 3):  # the proposals do find reductions
+        base = O.walk_records(start, O.make_params(iterations=1500, plus_iterations=300, p_expand=50.0), 21,
+                              launches=2, nthreads=8)
+        assert int(ora["m"].sum()) < int(base["m"].sum())

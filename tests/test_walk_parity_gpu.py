@@ -13,14 +13,14 @@ pytestmark = pytest.mark.gpu
 
 
 def _run_both(n, walkers, iterations, seed, launches=1, plus=8000, p_expand=0.0, random_iterations=False,
-              ring=O.RING_Z2, p_reduce=0.0, word_bits=0, partner_layout=0):
+              ring=O.RING_Z2, p_reduce=0.0, word_bits=0, partner_layout=0, look=(0, 0)):
     print(f"[parity] ring={ring} n={n} walkers={walkers} iterations={iterations} launches={launches} seed={seed} "
           f"plus={plus} p_expand={p_expand} p_reduce={p_reduce} random={random_iterations} "
-          f"word_bits={word_bits} layout={partner_layout}", flush=True)
+          f"word_bits={word_bits} layout={partner_layout} look={look}", flush=True)
     with fg.Engine(walkers, seed, ring=ring, max_iterations=iterations, plus_iterations=plus,
                    expand_probability=p_expand, reduce_probability=p_reduce,
                    random_iterations=random_iterations, zt_visit_cap=16, word_bits=word_bits,
-                   partner_layout=partner_layout) as e:
+                   partner_layout=partner_layout, lookahead_period=look[0], lookahead_candidates=look[1]) as e:
         e.init_naive(*n)
         init = e.read_walkers()
         ora0 = O.naive_records(*n, walkers, ring=ring)
@@ -29,7 +29,7 @@ def _run_both(n, walkers, iterations, seed, launches=1, plus=8000, p_expand=0.0,
             assert e.walk(iterations) == 0
         gpu = e.read_walkers()
     p = O.make_params(iterations=iterations, plus_iterations=plus, p_expand=p_expand, p_reduce=p_reduce,
-                      random_iterations=random_iterations)
+                      random_iterations=random_iterations, look_period=look[0], look_candidates=look[1])
     ora = O.walk_records(ora0, p, seed, launches=launches, nthreads=8, ring=ring)
     return gpu, ora
 
@@ -76,6 +76,14 @@ def test_walk_reduce_probability_bit_identical(n, layout):
     # --reduce-probability > 0: tryReduce after flips (flip_graph.cu:550-551)
     gpu, ora = _run_both(n, 128, 2000, seed=66, launches=2, plus=200, p_expand=50.0, p_reduce=400.0,
                          partner_layout=layout)
+    assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
+
+
+@pytest.mark.parametrize("n,layout,look", [((3, 3, 3), 0, (2, 3)), ((3, 3, 3), 1, (1, 2)), ((4, 4, 4), 0, (2, 3)),
+                                           ((2, 3, 4), 0, (4, 8))])
+def test_walk_lookahead_bit_identical(n, layout, look):
+    # reduction-seeking lookahead proposals (period, candidates)
+    gpu, ora = _run_both(n, 128, 2000, seed=81, launches=2, plus=200, p_expand=50.0, partner_layout=layout, look=look)
     assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
 
 
