@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ttt", action="store_true", help="leave the time-to-target A/B out of the default line")
     ap.add_argument("--ttt-seeds-default", type=int, default=3, help="seeds per arm of the A/B in the default line")
-    ap.add_argument("--ttt-train-default", type=int, default=600, help="training steps of the A/B in the default line")
+    ap.add_argument("--ttt-train-default", type=int, default=1000, help="training steps of the A/B in the default line")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "ttt", "propose", "descent"],
@@ -65,9 +65,10 @@ def parse():
     tt.add_argument("--ttt-plus", type=int, default=20000)
     tt.add_argument("--learned-every", type=int, default=2, help="every k-th walker takes the learned proposal")
     tt.add_argument("--candidates", type=int, default=8, help="uniform candidates scored per proposal")
-    tt.add_argument("--log-launches", type=int, default=86, help="uniform launches logged for training")
-    tt.add_argument("--log-horizon", type=int, default=10, help="launches until a logged flip is labelled")
-    tt.add_argument("--train-steps", type=int, default=1500)
+    tt.add_argument("--log-launches", type=int, default=8, help="uniform launches logged for training")
+    tt.add_argument("--log-walkers", type=int, default=16384, help="walkers of the logged uniform walk")
+    tt.add_argument("--log-warm", type=int, default=30, help="launches of descent before logging starts")
+    tt.add_argument("--train-steps", type=int, default=1000)
     tt.add_argument("--train-batch", type=int, default=1024)
     tt.add_argument("--lr", type=float, default=1e-3)
     return ap.parse_args()
@@ -177,22 +178,33 @@ def auc(scores, labels):
     return float((r[pos].sum() - P * (P + 1) / 2) / (P * N))
 
 
-def heldout_ranking(a, sc, W, L):
-    """How well the trained scorer ranks flips it has not seen: flips logged by a
-    fresh uniform walk (seed 2000, every 16th walker, the training horizon),
-    scored by the trained network and by its untrained initial state (same
-    seed); AUC against the logged labels."""
+def descent_log(seed, W, L, warm, launches, plus):
+    """A uniform 4x4x4 walk's trajectory log: `warm` launches of descent from the
+    naive scheme first, then `launches` logged launches, each walker's first
+    flip labelled 1 when its rank falls within that launch."""
+    import flipgraphgpu_amd as fg
+
+    e = fg.Engine(W, seed, max_iterations=L, plus_iterations=plus, expand_probability=0.0)
+    e.init_naive(4, 4, 4)
+    for _ in range(warm):
+        if e.walk(L):
+            raise RuntimeError("invalid schemes while logging")
+    e.log_enable(max_batches=launches, every=1, horizon=1)
+    for _ in range(launches):
+        if e.walk(L):
+            raise RuntimeError("invalid schemes while logging")
+    return e
+
+
+def heldout_ranking(a, sc, L):
+    """How well the trained scorer ranks flips it has not seen: flips logged the
+    same way by a fresh uniform walk (seed 2000), scored by the trained network
+    and by its untrained initial state (same seed); AUC against the labels."""
     import numpy as np
 
     import flipgraphgpu_amd as fg
 
-    e = fg.Engine(W, 2000, max_iterations=L, plus_iterations=a.ttt_plus, expand_probability=0.0)
-    e.init_naive(4, 4, 4)
-    batches = 4
-    e.log_enable(max_batches=batches, every=16, horizon=a.log_horizon)
-    for q in range(batches + a.log_horizon - 1):
-        if e.walk(L):
-            raise RuntimeError("invalid schemes while logging")
+    e = descent_log(2000, 4096, L, a.log_warm, 4, a.ttt_plus)
     n = e.log_info()["rows"]
     words, m, pairs, y = e.log_read(0, n, 64)
     e.close()
@@ -251,14 +263,10 @@ def time_to_target(a, seeds=None, train_steps=None, emit=True):
         e.close()
         return dt, best, launches
 
-    # training data: uniform walks with the trajectory log
+    # training data: uniform walks with the trajectory log, from the descent
+    # phase (after log_warm launches), labelled by the flip's own launch
     t0 = time.perf_counter()
-    e = fg.Engine(W, 1000, max_iterations=L, plus_iterations=a.ttt_plus, expand_probability=0.0)
-    e.init_naive(4, 4, 4)
-    e.log_enable(max_batches=a.log_launches, every=1, horizon=a.log_horizon)
-    for q in range(a.log_launches + a.log_horizon - 1):
-        if e.walk(L):
-            raise RuntimeError("invalid schemes while logging")
+    e = descent_log(1000, a.log_walkers, L, a.log_warm, a.log_launches, a.ttt_plus)
     info = e.log_info()
     t_log = time.perf_counter() - t0
     progress(f"[ttt] logged {info['rows']} flips ({info['positives']} positive) in {t_log:.1f} s")
@@ -268,7 +276,7 @@ def time_to_target(a, seeds=None, train_steps=None, emit=True):
     t_train = time.perf_counter() - t0
     e.close()
     progress(f"[ttt] trained {train_steps} steps in {t_train:.1f} s, loss {losses[:20].mean():.4f} -> {losses[-50:].mean():.4f}")
-    heldout = heldout_ranking(a, sc, W, L)
+    heldout = heldout_ranking(a, sc, L)
     progress(f"[ttt] held-out AUC: trained {heldout['auc_trained']:.4f}, untrained {heldout['auc_untrained']:.4f}")
     res = {"uniform": [], "learned": []}
     for seed in range(1, seeds + 1):
