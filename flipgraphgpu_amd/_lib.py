@@ -114,6 +114,7 @@ def load_library():
     L.fg_set_forced.argtypes = [vp, vp]
     L.fg_get_forced.argtypes = [vp, vp]
     L.fg_walk_forced.argtypes = [vp, ctypes.c_int]
+    L.fg_walk_trace.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
     L.fg_scorer_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                    ctypes.POINTER(vp)]
     L.fg_scorer_destroy.argtypes = [vp]
@@ -327,6 +328,18 @@ class Engine:
     def walk(self, iterations=0):
         """One launch; returns the number of current schemes failing the GPU Brent check."""
         return _check(self.L.fg_walk(self.ctx, iterations), "fg_walk")
+
+    def walk_trace(self, iterations=0, cap=4096):
+        """fg_walk that also records every operator applied to each walker's
+        scheme: returns (ops [walkers][cap][8] int32, counts [walkers], the
+        number of schemes failing the GPU Brent check).  A count above cap
+        means the walker's trace was cut."""
+        W = self.walkers
+        ops = np.zeros((W, max(cap, 1), 8), dtype=np.int32)
+        counts = np.zeros(W, dtype=np.int32)
+        bad = _check(self.L.fg_walk_trace(self.ctx, iterations, cap, ops.ctypes.data, counts.ctypes.data),
+                     "fg_walk_trace")
+        return ops[:, :cap], counts, bad
 
     def last_walk_ms(self):
         x = ctypes.c_float()

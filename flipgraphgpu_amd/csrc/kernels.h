@@ -17,7 +17,7 @@ size_t walk_lds_bytes(int variant, int max_m);
 // mode: 0 walk, 1 complexity minimizer, 2 walk with proposals and logging (Z2 only),
 // 3 the additions reducer's scheme flips (integer ring only), 4 resize, 5 walk with sandwiching
 // and basis changes (4 and 5: Z2 with 32-bit words or the integer ring), 6 walk with lookahead
-// proposals (Z2).
+// proposals (Z2), 7 walk that records its operators (every layout).
 hipError_t prepare_walk(int variant, int max_m, int mode, int *groups_per_cu);
 // Resets a.queue and launches `grid` persistent workgroups.
 hipError_t launch_walk(int variant, int mode, const WalkArgs &a, int grid, hipStream_t stream);

@@ -143,6 +143,30 @@ struct LaneBaseT {
     int pop;        // coefficient count of the scheme, kept for getComplexity
     bool track_pop; // keep pop current through flips (the minimizer reads it every flip)
     int sel_c, sel_t1, sel_t2;  // pair of the last tryFlip (component, index1 < index2)
+    // Flip-trace launches only (null in every other kernel, where the
+    // recording folds away): the operators applied to this walker's scheme,
+    // 8 int32 each, [code, a0..a6] with the codes of an explicit trace:
+    // 0 flip(i, j, k, i1, i2, checkReduce), 1 plus(i, j, k, i1, i2, variant),
+    // 2 split(i, j, k, index, a, s) (Z2: a = the new factor, s = 0; integer
+    // ring: a = magnitudes, s = signs), 6 tryReduce (fresh-list order).  trc_n counts every
+    // operator, also those past trc_cap, which are not stored.
+    int32_t *trc;
+    int trc_cap, trc_n;
+    FGK_HD void rec(int code, int x0, int x1, int x2, int x3, int x4, int x5, int x6) {
+        if (!trc) return;
+        if (side == 0 && trc_n < trc_cap) {
+            int32_t *o = trc + (size_t)trc_n * 8;
+            o[0] = code;
+            o[1] = x0;
+            o[2] = x1;
+            o[3] = x2;
+            o[4] = x3;
+            o[5] = x4;
+            o[6] = x5;
+            o[7] = x6;
+        }
+        ++trc_n;
+    }
 #if defined(FGK_PHASE_CLOCKS) && defined(__HIP_DEVICE_COMPILE__)
     // Phase profile build only: cycles (s_memtime) spent since the previous
     // mark, summed per phase: 0 RNG block, 1 pair selection, 2 factor reads
@@ -671,6 +695,7 @@ constexpr int kModeFlips = 3;     // flipSchemesKernel of the additions reducer
 constexpr int kModeResize = 4;    // resizeKernel (Z2)
 constexpr int kModeMeta = 5;      // walk with the sandwiching and basis-change edges (Z2)
 constexpr int kModeLook = 6;      // walk with reduction-seeking lookahead proposals (Z2)
+constexpr int kModeTrace = 7;     // plain walk that records every operator it applies
 
 // Resize draw thresholds: a uniform compare u < q is x < floor(q * 2^32).
 constexpr uint32_t kThrHalf = 2147483648u, kThr005 = 214748364u, kThr055 = 2362232012u, kThr085 = 3650722201u;
@@ -695,7 +720,9 @@ constexpr int kMaxExtensionN = 16; // src/config.cuh:37
 // to three random walkers' best schemes, else projects, multiplies by a
 // random best scheme, doubles a dimension or extends one.  kModeMeta is the
 // walk with the sandwiching and basis-change checks of randomWalkKernel
-// (:558-562), kept out of the plain walk kernel.  `lds` is the workgroup's LDS block, `lane`
+// (:558-562), kept out of the plain walk kernel.  kModeTrace is the plain walk
+// that also writes every operator it applies to A.trace (LaneBaseT::rec), an
+// explicit trace that replays the launch without its random draws.  `lds` is the workgroup's LDS block, `lane`
 // the lane in the workgroup (walker slot lane / 2, flip side lane % 2), `w`
 // the walker index.  The host emulation calls it once per walker with an
 // even lane and does both sides.
@@ -737,6 +764,15 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     s.vmask = Lane::kValueMask;
     s.pop = 0;
     s.track_pop = kMode == kModeMinimize;
+    s.trc = nullptr;
+    s.trc_cap = 0;
+    s.trc_n = 0;
+    if constexpr (kMode == kModeTrace) {
+        if (active) {
+            s.trc = A.trace + (size_t)w * A.trace_cap * 8;
+            s.trc_cap = A.trace_cap;
+        }
+    }
     int best_rank = 0, best_m = 0, flips_count = 0;
     uint64_t it = 0, total_flips = 0, sum_m = 0;
     s.aux = 0;
@@ -931,7 +967,10 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
         copy_best(s, A, copy, lane, w);
         s.mark(6);
         if (flipped) {
-            if (A.use_reduce && s.aux32() < A.thr_reduce) s.try_reduce();  // :550-551
+            if (A.use_reduce && s.aux32() < A.thr_reduce) {  // :550-551
+                s.rec(6, 0, 0, 0, 0, 0, 0, 0);
+                s.try_reduce();
+            }
             if (A.use_expand && b3 < A.thr_expand && s.m <= best_m + 2) {   // :553-556
                 if (s.try_expand(1 + (int)(s.aux32() % 2u))) flips_count = 0;
             }
@@ -954,6 +993,7 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     }
 
     if (active && s.side == 0) {
+        if constexpr (kMode == kModeTrace) A.trace_n[w] = s.trc_n;
         A.m[w] = s.m;
         A.best_m[w] = best_m;
         A.best_rank[w] = best_rank;

@@ -76,6 +76,9 @@ struct fg_ctx {
     int32_t *item_begin = nullptr, *item_end = nullptr, *queue = nullptr;
     int32_t *cx = nullptr, *dst = nullptr;
     int32_t *forced = nullptr;  // [W][4] proposed first flips (learned walks)
+    // set only for the duration of an fg_walk_trace launch
+    int32_t *trace = nullptr, *trace_n = nullptr;
+    int trace_cap = 0;
     int nitems = 0;
     bool minimize = false;  // pool set up by fg_init_minimizer
     bool flips = false;     // pool set up by fg_init_flips
@@ -612,6 +615,37 @@ extern "C" int fg_walk(fg_ctx *ctx, int iterations) {
     return walk_launch(ctx, iterations, false);
 }
 
+extern "C" int fg_walk_trace(fg_ctx *ctx, int iterations, int cap, int32_t *ops, int32_t *counts) {
+    if (!ctx || !ctx->ready) return fail("fg_walk_trace: context not initialised");
+    if (cap < 0 || (cap > 0 && !ops) || !counts) return fail("fg_walk_trace: bad arguments");
+    if (ctx->minimize || ctx->flips) return fail("fg_walk_trace: traces are recorded for plain walks");
+    const size_t slots = (size_t)ctx->W * (size_t)std::max(cap, 1) * 8;
+    int g = 0;
+    if (fgk::prepare_walk(ctx->variant, ctx->maxm, 7, &g) != hipSuccess || g < 1)
+        return fail("fg_walk_trace: the trace kernel does not fit the pool's layout");
+    int32_t *d_ops = nullptr, *d_n = nullptr;
+    HIPCHK(dalloc(&d_ops, slots));
+    if (dalloc(&d_n, (size_t)ctx->W) != hipSuccess) {
+        (void)hipFree(d_ops);
+        return fail("fg_walk_trace: out of device memory");
+    }
+    ctx->trace = d_ops;
+    ctx->trace_n = d_n;
+    ctx->trace_cap = cap;
+    const int rc = walk_launch(ctx, iterations, false);
+    ctx->trace = ctx->trace_n = nullptr;
+    ctx->trace_cap = 0;
+    int out = rc;
+    if (rc >= 0) {
+        if ((cap > 0 && hipMemcpy(ops, d_ops, (size_t)ctx->W * cap * 8 * 4, hipMemcpyDeviceToHost) != hipSuccess) ||
+            hipMemcpy(counts, d_n, (size_t)ctx->W * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            out = fail("fg_walk_trace: copy of the trace failed");
+    }
+    (void)hipFree(d_ops);
+    (void)hipFree(d_n);
+    return out;
+}
+
 extern "C" int fg_resize(fg_ctx *ctx, double probability) {
     if (!ctx || !ctx->ready) return fail("fg_resize: context not initialised");
     if (ctx->variant == fgk::kVarZ2h || ctx->variant == fgk::kVarZ2m || ctx->minimize || ctx->flips)
@@ -715,7 +749,14 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced, double resize) {
     if (meta && learn) return fail("fg_walk: proposals and trajectory logs do not combine with basis-change edges");
     const bool look = ctx->cfg.lookahead_candidates > 0 && ctx->cfg.ring == FG_RING_Z2;
     if (look && (learn || meta)) return fail("fg_walk: lookahead proposals do not combine with learned proposals or basis edges");
-    const int mode = resize > 0 ? 4 : (ctx->minimize ? 1 : (ctx->flips ? 3 : (learn ? 2 : (meta ? 5 : (look ? 6 : 0)))));
+    const bool trace = ctx->trace != nullptr;
+    if (trace && (resize > 0 || ctx->minimize || ctx->flips || learn || meta || look))
+        return fail("fg_walk_trace: traces are recorded for plain walks");
+    const int mode = trace ? 7
+                           : (resize > 0 ? 4 : (ctx->minimize ? 1 : (ctx->flips ? 3 : (learn ? 2 : (meta ? 5 : (look ? 6 : 0))))));
+    a.trace = ctx->trace;
+    a.trace_n = ctx->trace_n;
+    a.trace_cap = ctx->trace_cap;
     a.look_period = ctx->cfg.lookahead_period;
     a.look_candidates = ctx->cfg.lookahead_candidates;
     a.use_sandwich = ctx->cfg.sandwiching_probability > 0;
@@ -764,7 +805,7 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced, double resize) {
         if (timed) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         return verify_planes(ctx, ctx->cfg.ring, ctx->cur, ctx->m, ctx->fmt, ctx->W, ctx->maxm, ctx->ok);
     };
-    if (ctx->cfg.use_graph && mode != 2 && mode != 4) {
+    if (ctx->cfg.use_graph && mode != 2 && mode != 4 && mode != 7) {
         if (ctx->graph_iterations != a.iterations || ctx->graph_grid != ctx->grid || ctx->graph_variant != ctx->variant) {
             ctx->drop_graph();
             HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));

@@ -133,6 +133,27 @@ FGK_WALK_KERNEL look_z2m_kernel(WalkArgs A) {
     walk_persistent<lane::Z2LaneM, lane::kModeLook>(A, lds);
 }
 
+// Plain walk that records every operator it applies (flip-trace launches).
+FGK_WALK_KERNEL trace_z2_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2Lane, lane::kModeTrace>(A, lds);
+}
+
+FGK_WALK_KERNEL trace_zt_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::ZTLane, lane::kModeTrace>(A, lds);
+}
+
+FGK_WALK_KERNEL trace_z2h_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2Lane16, lane::kModeTrace>(A, lds);
+}
+
+FGK_WALK_KERNEL trace_z2m_kernel(WalkArgs A) {
+    extern __shared__ uint8_t lds[];
+    walk_persistent<lane::Z2LaneM, lane::kModeTrace>(A, lds);
+}
+
 // Walk with a proposed first flip and trajectory logging (Z2).
 FGK_WALK_KERNEL learn_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
@@ -319,21 +340,22 @@ size_t walk_lds_bytes(int variant, int max_m) {
 }
 
 static const void *walk_kernel(int variant, int mode) {
-    const void *k[4][7] = {
+    const void *k[4][8] = {
         {reinterpret_cast<const void *>(walk_z2_kernel), reinterpret_cast<const void *>(minimize_z2_kernel),
          reinterpret_cast<const void *>(learn_z2_kernel), nullptr, reinterpret_cast<const void *>(resize_z2_kernel),
-         reinterpret_cast<const void *>(walk_meta_z2_kernel), reinterpret_cast<const void *>(look_z2_kernel)},
+         reinterpret_cast<const void *>(walk_meta_z2_kernel), reinterpret_cast<const void *>(look_z2_kernel),
+         reinterpret_cast<const void *>(trace_z2_kernel)},
         {reinterpret_cast<const void *>(walk_zt_kernel), reinterpret_cast<const void *>(minimize_zt_kernel), nullptr,
          reinterpret_cast<const void *>(flips_zt_kernel), reinterpret_cast<const void *>(resize_zt_kernel),
-         reinterpret_cast<const void *>(walk_meta_zt_kernel), nullptr},
+         reinterpret_cast<const void *>(walk_meta_zt_kernel), nullptr, reinterpret_cast<const void *>(trace_zt_kernel)},
         {reinterpret_cast<const void *>(walk_z2h_kernel), reinterpret_cast<const void *>(minimize_z2h_kernel),
          reinterpret_cast<const void *>(learn_z2h_kernel), nullptr, nullptr, nullptr,
-         reinterpret_cast<const void *>(look_z2h_kernel)},
+         reinterpret_cast<const void *>(look_z2h_kernel), reinterpret_cast<const void *>(trace_z2h_kernel)},
         {reinterpret_cast<const void *>(walk_z2m_kernel), reinterpret_cast<const void *>(minimize_z2m_kernel),
          reinterpret_cast<const void *>(learn_z2m_kernel), nullptr, nullptr, nullptr,
-         reinterpret_cast<const void *>(look_z2m_kernel)},
+         reinterpret_cast<const void *>(look_z2m_kernel), reinterpret_cast<const void *>(trace_z2m_kernel)},
     };
-    if (variant < 0 || variant > 3 || mode < 0 || mode > 6) return nullptr;
+    if (variant < 0 || variant > 3 || mode < 0 || mode > 7) return nullptr;
     return k[variant][mode];
 }
 

@@ -81,6 +81,11 @@ struct WalkArgs {
     uint32_t thr_sandwich;
     int32_t use_basis;
     uint32_t thr_basis;
+    // flip-trace launches: walker w writes its operators to
+    // trace[w][trace_cap][8] and their count (stored or not) to trace_n[w]
+    int32_t *trace;
+    int32_t *trace_n;
+    int32_t trace_cap;
 };
 
 // Error bits written by the walk kernels.

@@ -178,6 +178,7 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
     // emulation does both.  Reduction order as in the reference: the lowest
     // term first, the index1 half before the index2 half at equal terms.
     FGK_HD void flip(int i, int j, int k, int i1, int i2, bool check_reduce) {
+        this->rec(0, i, j, k, i1, i2, check_reduce ? 1 : 0, 0);
         const uint32_t fj1 = f(j, i1), fj2 = f(j, i2);
         const uint32_t fk1 = f(k, i1), fk2 = f(k, i2);
         const uint32_t A = fj1 ^ fj2;
@@ -375,6 +376,7 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
 
     // plus (scheme_z2.cu:579-612)
     FGK_HD void plus(int i, int j, int k, int i1, int i2, int variant) {
+        this->rec(1, i, j, k, i1, i2, variant, 0);
         const uint32_t a1 = f(i, i1), b1 = f(j, i1), c1 = f(k, i1);
         const uint32_t a2 = f(i, i2), b2 = f(j, i2), c2 = f(k, i2);
         const uint32_t a = a1 ^ a2, b = b1 ^ b2, c = c1 ^ c2;
@@ -397,6 +399,7 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
 
     // split (scheme_z2.cu:614-619)
     FGK_HD void split(int i, int j, int k, int index, uint32_t x1) {
+        this->rec(2, i, j, k, index, (int)x1, 0, 0);
         const uint32_t x2 = f(i, index) ^ x1;
         f(i, index) = x1;
         f(i, m) = x2;

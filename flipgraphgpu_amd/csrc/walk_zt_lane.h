@@ -182,6 +182,7 @@ struct ZTLane : LaneBaseT<uint32_t> {
     // non-zero; the two halves run on the two lanes of the pair (both on the
     // host), as in the Z2 lane.
     FGK_HD void flip(int i, int j, int k, int i1, int i2, bool check_reduce) {
+        rec(0, i, j, k, i1, i2, check_reduce ? 1 : 0, 0);
         const uint32_t fj1 = f(j, i1), fj2 = f(j, i2);
         const uint32_t fk1 = f(k, i1), fk2 = f(k, i2);
         bool ok;
@@ -230,6 +231,7 @@ struct ZTLane : LaneBaseT<uint32_t> {
 
     // plus (scheme_integer.cu:503-539)
     FGK_HD void plus(int i, int j, int k, int i1, int i2, int variant) {
+        rec(1, i, j, k, i1, i2, variant, 0);
         const uint32_t a1 = f(i, i1), b1 = f(j, i1), c1 = f(k, i1);
         const uint32_t a2 = f(i, i2), b2 = f(j, i2), c2 = f(k, i2);
         bool oaA, obA, ocA, oaS, obS, ocS;
@@ -256,6 +258,7 @@ struct ZTLane : LaneBaseT<uint32_t> {
 
     // split (scheme_integer.cu:541-547)
     FGK_HD void split(int i, int j, int k, int index, uint32_t a) {
+        rec(2, i, j, k, index, (int)(a & 0xFFFFu), (int)(a >> 16), 0);
         bool ok;
         f(i, m) = tern::sub(f(i, index), a, ok);
         f(j, m) = f(j, index);

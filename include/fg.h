@@ -177,6 +177,24 @@ FG_API int fg_layout_info(fg_ctx *ctx, int32_t *layout, int32_t *trials, int cap
  * number of invalid schemes found (0 expected) or < 0 on error. */
 FG_API int fg_walk(fg_ctx *ctx, int iterations);
 
+/* fg_walk that also records, per walker, every operator the launch applies
+ * to its scheme (the flip, plus, split and tryReduce calls of
+ * randomWalkKernel, src/entities/flip_graph.cu:520-573, with the arguments
+ * its random draws chose; src/schemes/scheme_z2.cu:531-619,1127-1154,
+ * scheme_integer.cu:428-547,1062-1101).  The trajectory is the one fg_walk
+ * takes.  ops[w][cap][8] gets walker w's first cap operators as
+ * [code, a0..a6]: 0 flip(i, j, k, index1, index2, checkReduce),
+ * 1 plus(i, j, k, index1, index2, variant), 2 split(i, j, k, index, a, s)
+ * (Z2: a = the new factor mask, s = 0; integer ring: a = magnitudes,
+ * s = signs), 6 tryReduce (over the fresh-list pair order, the walk's);
+ * counts[w] gets the number of operators applied,
+ * which may exceed cap (the rest are not stored).  Replaying a walker's
+ * trace from its scheme before the launch gives its scheme after it.  Plain
+ * walk pools only (no minimizer, flips, logging, lookahead or basis edges).
+ * No reference counterpart: the reference records nothing.  Returns what
+ * fg_walk returns. */
+FG_API int fg_walk_trace(fg_ctx *ctx, int iterations, int cap, int32_t *ops, int32_t *counts);
+
 /* resizeKernel (src/entities/flip_graph.cu:575-613) on a walk pool (Z2 with
  * 32-bit factor words, or the integer ring): each walker resizes with the
  * given probability -- half the

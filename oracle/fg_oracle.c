@@ -1250,6 +1250,7 @@ int fgo_blob_replay(int ring, const int32_t *in, const int32_t *ops, int nops, i
         case 3: fgo_reduce(s, o[1], o[2], o[3], o[4]); break;
         case 4: fgo_try_reduce(s); break;
         case 5: fgo_fix_signs(s); break;
+        case 6: try_reduce_mode(s, FGO_SELECT_CANONICAL); break;
         default: free(s); return -1;
         }
     }

@@ -187,7 +187,9 @@ int fgo_blob_naive(int ring, int n1, int n2, int n3, int32_t *out);
 /* Replay an explicit operator trace.  ops: 8 int32 per op:
  * [code, a0..a6].  code 0 flip(i,j,k,i1,i2,checkReduce); 1 plus(i,j,k,i1,i2,variant);
  * 2 split(i,j,k,index, a_lo, a_hi[, s_lo]) (ZT: a = values, s = signs in a5/a6 low bits);
- * 3 reduce(i,i1,i2,sub); 4 tryReduce; 5 fixSigns.  Returns new blob length or -1. */
+ * 3 reduce(i,i1,i2,sub); 4 tryReduce over the stored pair lists; 5 fixSigns;
+ * 6 tryReduce over the fresh-list order (the walk's canonical order).
+ * Returns new blob length or -1. */
 int fgo_blob_replay(int ring, const int32_t *in, const int32_t *ops, int nops, int32_t *out);
 int fgo_blob_pairs(int ring, const int32_t *blob, int mode, uint32_t *out, int cap);
 int fgo_blob_try_flip_zt(const int32_t *in, uint64_t seed, uint32_t walker, uint64_t it, int select_mode,

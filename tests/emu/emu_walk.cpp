@@ -22,7 +22,8 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                                    int mode, const int32_t *forced, int32_t *log_meta, int32_t *minm,
                                    int force_wide, uint64_t thr_resize, int resize_max_rank, int resize_max_coef,
                                    int use_sandwich, uint32_t thr_sandwich, int use_basis, uint32_t thr_basis,
-                                   int look_period, int look_candidates, int32_t *err_out) {
+                                   int look_period, int look_candidates, int32_t *trace, int32_t *trace_n,
+                                   int trace_cap, int32_t *err_out) {
     int maxm = 1;
     for (int q = 0; q < count; ++q) {
         maxm = std::max(maxm, std::max(recs[q].m, recs[q].best_m));
@@ -67,6 +68,9 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
     a.thr_basis = thr_basis;
     a.look_period = look_period;
     a.look_candidates = look_candidates;
+    a.trace = trace;  // the trace mode (7) writes [W][trace_cap][8] operators and their counts
+    a.trace_n = trace_n;
+    a.trace_cap = trace_cap;
     std::vector<uint32_t> log_words((size_t)W * 3 * maxm);
     std::vector<int32_t> lm((size_t)W * 4, -1), mm(W, 0);
     if (mode == fgk::lane::kModeLearn) {
@@ -103,6 +107,7 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                     else if (mode == kModeFlips) walk_lane<ZTLane, kModeFlips>(a, L, lane, w, W);
                     else if (mode == kModeResize) walk_lane<ZTLane, kModeResize>(a, L, lane, w, W);
                     else if (mode == kModeMeta) walk_lane<ZTLane, kModeMeta>(a, L, lane, w, W);
+                    else if (mode == kModeTrace) walk_lane<ZTLane, kModeTrace>(a, L, lane, w, W);
                     else walk_lane<ZTLane, kModeWalk>(a, L, lane, w, W);
                 } else if (mode == kModeResize) {
                     walk_lane<Z2Lane, kModeResize>(a, L, lane, w, W);
@@ -115,14 +120,17 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
                 } else if (masks) {
                     if (mode == kModeMinimize) walk_lane<Z2LaneM, kModeMinimize>(a, L, lane, w, W);
                     else if (mode == kModeLearn) walk_lane<Z2LaneM, kModeLearn>(a, L, lane, w, W);
+                    else if (mode == kModeTrace) walk_lane<Z2LaneM, kModeTrace>(a, L, lane, w, W);
                     else walk_lane<Z2LaneM, kModeWalk>(a, L, lane, w, W);
                 } else if (half) {
                     if (mode == kModeMinimize) walk_lane<Z2Lane16, kModeMinimize>(a, L, lane, w, W);
                     else if (mode == kModeLearn) walk_lane<Z2Lane16, kModeLearn>(a, L, lane, w, W);
+                    else if (mode == kModeTrace) walk_lane<Z2Lane16, kModeTrace>(a, L, lane, w, W);
                     else walk_lane<Z2Lane16, kModeWalk>(a, L, lane, w, W);
                 } else {
                     if (mode == kModeMinimize) walk_lane<Z2Lane, kModeMinimize>(a, L, lane, w, W);
                     else if (mode == kModeLearn) walk_lane<Z2Lane, kModeLearn>(a, L, lane, w, W);
+                    else if (mode == kModeTrace) walk_lane<Z2Lane, kModeTrace>(a, L, lane, w, W);
                     else walk_lane<Z2Lane, kModeWalk>(a, L, lane, w, W);
                 }
             }

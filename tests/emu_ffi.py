@@ -23,7 +23,8 @@ def lib():
                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                   ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32,
-                                  ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
 
@@ -32,14 +33,15 @@ LAYOUTS = {"auto": 0, "wide": 1, "half": 2, "masks": 3}
 
 
 def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=False, wide=False, resize=None,
-         layout="auto"):
+         layout="auto", trace_cap=0):
     """Emulated launches; with learn (Z2) the first launch takes the proposed
     pairs `forced` [walkers][4] and the result adds (log_meta, minm).  Z2
     layouts as the runtime picks them (auto: class masks up to 32 terms and
     16 coefficients, else 16-bit words with partner lists up to 16
     coefficients, else 32-bit words), or forced: "wide", "half", "masks"
     (wide=True is "wide").  resize = (thr_resize, max_rank, max_coef) runs the
-    resize mode instead of the walk."""
+    resize mode instead of the walk.  trace_cap > 0 runs the trace mode (one
+    launch) and the result adds (ops [walkers][trace_cap][8], counts)."""
     if wide:
         layout = "wide"
     recs = np.ascontiguousarray(recs.copy())
@@ -50,6 +52,11 @@ def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=Fals
         mode = 5  # the meta-walk kernel
     if mode == 0 and p.look_candidates > 0:
         mode = 6  # the lookahead kernel
+    if trace_cap > 0:
+        assert mode == 0 and launches == 1, "traces are recorded for one plain launch"
+        mode = 7
+    tr = np.zeros((len(recs), max(trace_cap, 1), 8), dtype=np.int32)
+    tn = np.zeros(len(recs), dtype=np.int32)
     thr, rmax, cmax = resize if resize is not None else (0, 0, 0)
     fz = None if forced is None else np.ascontiguousarray(forced, dtype=np.int32)
     lm = np.zeros((len(recs), 4), dtype=np.int32)
@@ -60,8 +67,11 @@ def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=Fals
                                 p.zt_visit_cap, mode, None if fz is None else fz.ctypes.data,
                                 lm.ctypes.data, mm.ctypes.data, LAYOUTS[layout], thr, rmax, cmax,
                                 p.use_sandwich, p.thr_sandwich, p.use_basis, p.thr_basis, p.look_period,
-                                p.look_candidates, ctypes.byref(err))
+                                p.look_candidates, tr.ctypes.data if trace_cap > 0 else None,
+                                tn.ctypes.data if trace_cap > 0 else None, trace_cap, ctypes.byref(err))
     assert rc == 0
+    if trace_cap > 0:
+        return recs, err.value, tr[:, :trace_cap], tn
     if learn:
         return recs, err.value, lm, mm
     return recs, err.value
