@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--word-bits", type=int, default=0, choices=[0, 16, 32],
                     help="Z2 LDS factor words (0: the runtime's choice)")
+    ap.add_argument("--partner-layout", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="Z2 partner bookkeeping: 0 auto, 1 lists, 2 class masks, 3 class masks with one lane per walker")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ttt", action="store_true", help="leave the time-to-target A/B out of the default line")
@@ -595,7 +597,8 @@ def main():
     n = tuple(a.n)
     zt = a.ring == "zt"
     e = fg.Engine(a.walkers, a.seed, ring=fg.RING_ZT if zt else fg.RING_Z2, max_iterations=a.iterations,
-                  plus_iterations=a.plus_iterations, expand_probability=0.0, word_bits=0 if zt else a.word_bits)
+                  plus_iterations=a.plus_iterations, expand_probability=0.0, word_bits=0 if zt else a.word_bits,
+                  partner_layout=0 if zt else a.partner_layout)
     e.init_naive(*n)
     for _ in range(a.warmup):
         bad = e.walk(a.iterations)

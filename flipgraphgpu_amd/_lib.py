@@ -307,7 +307,8 @@ class Engine:
         _check(self.L.fg_autotune(self.ctx, iterations, ctypes.byref(g)), "fg_autotune")
         return g.value
 
-    LAYOUTS = {0: "z2 lists, 32-bit words", 1: "integer ring", 2: "z2 lists, 16-bit words", 3: "z2 class masks"}
+    LAYOUTS = {0: "z2 lists, 32-bit words", 1: "integer ring", 2: "z2 lists, 16-bit words", 3: "z2 class masks",
+               4: "z2 class masks, one lane per walker"}
 
     def layout_info(self):
         """(layout, trials of the last autotune as [n][3] = layout, grid, flips/s / 1e6)."""

@@ -9,8 +9,13 @@ namespace fgk {
 
 // Walk kernel variants: Z2 with 32-bit factor words, the integer ring
 // (packed ternary words), Z2 with 16-bit factor words in LDS (all three with
-// partner lists), Z2 with 16-bit words and class masks (at most 32 terms).
-constexpr int kVarZ2 = 0, kVarZT = 1, kVarZ2h = 2, kVarZ2m = 3;
+// partner lists), Z2 with 16-bit words and class masks (at most 32 terms),
+// and the last with one lane per walker (64 walkers per workgroup).
+constexpr int kVarZ2 = 0, kVarZT = 1, kVarZ2h = 2, kVarZ2m = 3, kVarZ2s = 4;
+// Class-mask layouts (kVarZ2m, kVarZ2s).
+inline bool mask_variant(int variant) { return variant == kVarZ2m || variant == kVarZ2s; }
+// The one-lane layout's kernel for a walk-family mode (walk_solo.hip), or null.
+const void *solo_walk_kernel(int mode);
 
 size_t walk_lds_bytes(int variant, int max_m);
 // Sets the kernel's LDS limit and reports how many workgroups fit per CU.

@@ -79,11 +79,20 @@ FGK_HD uint32_t byte_sum_acc(uint32_t w, uint32_t acc) {
 }
 
 #if defined(FGK_PHASE_CLOCKS) && defined(__HIP_DEVICE_COMPILE__)
-extern __device__ unsigned long long g_phase[16];  // phase profile build: cycles per phase, [15] iterations
+// phase profile build: cycles per phase, [15] iterations (one array per
+// kernel source file, named by FGK_PHASE_ARRAY)
+#ifndef FGK_PHASE_ARRAY
+#define FGK_PHASE_ARRAY g_phase
+#endif
+extern __device__ unsigned long long FGK_PHASE_ARRAY[16];
 #endif
 namespace lane {
 
-constexpr int WS = kWalkersPerGroup;  // LDS stride between consecutive terms of one walker
+#if defined(__HIP_DEVICE_COMPILE__)
+constexpr bool kOnDevice = true;
+#else
+constexpr bool kOnDevice = false;
+#endif
 constexpr uint32_t kPM0 = 0xD2511F53u, kPM1 = 0xCD9E8D57u;
 constexpr uint32_t kPW0 = 0x9E3779B9u, kPW1 = 0xBB67AE85u;
 constexpr int kMaxRank = 350;        // src/config.cuh:10
@@ -116,10 +125,36 @@ FGK_HD void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_
 // and u have equal component-c factors, t included), for schemes of at most
 // 32 terms.  With masks a class is known without walking it: a term leaves
 // or joins a class with one store per member and no dependent reads.
-template <typename WordT, bool kMaskT = false>
+//
+// kWSv is the number of walkers per workgroup and the LDS stride between
+// consecutive terms of one walker: kWalkersPerGroup (32) for the lane-pair
+// layout, kSoloWalkersPerGroup (64) for the one-lane layout, where a lane
+// runs both halves of a flip itself (the code path the host emulation
+// runs) and its walker's column never shares an LDS bank with another lane
+// of its half-wave.
+template <typename WordT, bool kMaskT = false, int kWSv = kWalkersPerGroup>
 struct LaneBaseT {
     using Word = WordT;
     static constexpr bool kMask = kMaskT;
+    static constexpr int kWS = kWSv;
+    // the two halves of a flip run on the two lanes of a pair (GPU, pair layout)
+    static constexpr bool kSplit = kOnDevice && kWSv == kWalkersPerGroup;
+    // The one-lane layout packs two 16-bit factor rows per dword: dword p of
+    // component c holds row 15 - p in its low half and row 31 - p in its high
+    // half (32 rows, kPackedPairs dwords per component, one dword column per
+    // walker).  A scan of the 16 dwords then yields a row mask in term order
+    // (packed_ne).
+    static constexpr bool kPacked = kWSv == kSoloWalkersPerGroup;
+    static constexpr int kPackedPairs = 16;
+    static_assert(!kPacked || (kMaskT && sizeof(WordT) == 2), "the one-lane layout keeps 16-bit words and class masks");
+    // Offset of walker slot `slot` in the F plane, in words.
+    FGK_HD static int fslot(int slot) { return kPacked ? 2 * slot : slot; }
+    // Address of row t of component c in a walker's F column based at Fb.
+    FGK_HD static Word *fat(Word *Fb, int fs, int c, int t) {
+        if constexpr (kPacked) return Fb + (c * kPackedPairs + 15 - (t & 15)) * (2 * kWS) + (t >> 4);
+        else return Fb + (c * fs + t) * kWS;
+    }
+    FGK_HD static const Word *fat(const Word *Fb, int fs, int c, int t) { return fat(const_cast<Word *>(Fb), fs, c, t); }
     Word *F;      // walker-slot-offset plane bases
     uint8_t *H;   // slot-offset byte base of the packed H plane
     uint8_t *N;
@@ -192,12 +227,16 @@ struct LaneBaseT {
     // Extra per-lane LDS beyond the common planes (none by default).
     FGK_HD void lane_extra(uint8_t *, int) {}
 
-    FGK_HD Word &f(int c, int t) { return F[(c * fs + t) * WS]; }
+    FGK_HD Word &f(int c, int t) { return *fat(F, fs, c, t); }
+    // Packed layout: dword p of component c's rows (rows 15 - p and 31 - p).
+    FGK_HD uint32_t fpair(int c, int p) const {
+        return reinterpret_cast<const uint32_t *>(F)[(c * kPackedPairs + p) * kWS];
+    }
     // H packs terms 4q..4q+3 of a walker into one dword: byte (t & 3) of word q.
-    FGK_HD uint8_t &h(int c, int t) { return H[((c * hw + (t >> 2)) * WS) * 4 + (t & 3)]; }
-    FGK_HD uint32_t &hword(int c, int q) { return reinterpret_cast<uint32_t *>(H)[(c * hw + q) * WS]; }
-    FGK_HD uint8_t &nx(int c, int t) { return N[((c * hw + (t >> 2)) * WS) * 4 + (t & 3)]; }
-    FGK_HD uint32_t &cm(int c, int t) { return CM[(c * cs + t) * WS]; }
+    FGK_HD uint8_t &h(int c, int t) { return H[((c * hw + (t >> 2)) * kWS) * 4 + (t & 3)]; }
+    FGK_HD uint32_t &hword(int c, int q) { return reinterpret_cast<uint32_t *>(H)[(c * hw + q) * kWS]; }
+    FGK_HD uint8_t &nx(int c, int t) { return N[((c * hw + (t >> 2)) * kWS) * 4 + (t & 3)]; }
+    FGK_HD uint32_t &cm(int c, int t) { return CM[(c * cs + t) * kWS]; }
     // H[c][t] += d (d = +-1) as one LDS add on t's dword that returns nothing,
     // so the wave does not wait for it (the byte never wraps: H counts are
     // below 256 and a decremented count is at least 1).
@@ -489,6 +528,11 @@ struct LaneBaseT {
         uint32_t wd[NB];
 #pragma unroll
         for (int e = 0; e < NB; ++e) wd[e] = hword(c, e);
+        select_words<NB>(wd, r, t1);
+    }
+    // select_block on H words already loaded.
+    template <int NB>
+    FGK_HD static void select_words(const uint32_t (&wd)[NB], uint32_t &r, int &t1) {
         bool le[NB];
         uint32_t acc = 0, below = 0;
 #pragma unroll
@@ -561,12 +605,11 @@ struct LaneBaseT {
     // each row costs a compare and a select of a constant index.
     static constexpr int kScanBatch = kScanRows;
     FGK_HD int find_last(int c, uint32_t v) const {
-        const Word *Fc = F + c * fs * WS;
         int any = -1;
         for (int base = 0; base < m; base += kScanBatch) {
             Word w[kScanBatch];
 #pragma unroll
-            for (int e = 0; e < kScanBatch; ++e) w[e] = Fc[(base + e) * WS];
+            for (int e = 0; e < kScanBatch; ++e) w[e] = *fat(F, fs, c, base + e);
             // four independent select chains (rows e mod 4), so the compiler
             // can interleave them; the last match is the largest of the four
             int l0 = -1, l1 = -1, l2 = -1, l3 = -1;
@@ -597,6 +640,43 @@ struct LaneBaseT {
         if (t != m) copy_term(t, m);
     }
 };
+
+// x in a vector register (an opaque copy on the GPU), so values derived
+// from it stay in vector registers too.
+FGK_HD uint32_t vgpr_copy(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+#else
+    return x;
+#endif
+}
+
+// Per 16-bit half of x: 1 when the half is non-zero, else 0 (one
+// v_pk_min_u16 on the GPU; written out, since the compiler turns a packed
+// minimum into two compares and selects per dword).
+FGK_HD uint32_t nonzero_halves(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t r;
+    asm("v_pk_min_u16 %0, %1, %2" : "=v"(r) : "v"(x), "s"(0x00010001u));
+    return r;
+#else
+    return ((x & 0xFFFFu) ? 1u : 0u) | ((x >> 16) ? 0x10000u : 0u);
+#endif
+}
+
+// Rows of a packed component (dword p: row 15 - p low, row 31 - p high)
+// that differ from v: bit t set when row t != v.  Shifting the two half
+// flags of dword p in at bits 0 and 16 leaves row 15 - p at bit 15 - p and
+// row 31 - p at bit 31 - p after the 16 dwords.
+FGK_HD uint32_t packed_ne(const uint32_t (&R)[16], uint32_t v) {
+    const uint32_t vv = v | (v << 16);
+    uint32_t acc = 0;
+#pragma unroll
+    for (int p = 0; p < 16; ++p) acc = (acc << 1) | nonzero_halves(R[p] ^ vv);
+    return acc;
+}
 
 // A uniform draw in [0, size): floor(x * size / 2^32) (one v_mul_hi_u32; the
 // reference's curand() % size needs an integer division).
@@ -652,17 +732,26 @@ FGK_HD void copy_best(Lane &s, const WalkArgs &A, bool want, int lane, int w) {
         const int ms = __builtin_amdgcn_readlane(s.m, src);
         const int f0 = __builtin_amdgcn_readlane(s.n0, src), f1 = __builtin_amdgcn_readlane(s.n1, src),
                   f2 = __builtin_amdgcn_readlane(s.n2, src);
-        const int slot = lane % kWalkersPerGroup, src_slot = src % kWalkersPerGroup;
+        // the source lane's walker (the one-lane layout takes two queue
+        // items per wave, so the walkers of a wave need not be consecutive)
+        const int ws = __builtin_amdgcn_readlane(w, src);
+        const int slot = lane % Lane::kWS, src_slot = src % Lane::kWS;
         if (A.bfmt && lane == 0) {
-            int32_t *bf = A.bfmt + 4 * (size_t)(w - slot + src_slot);
+            int32_t *bf = A.bfmt + 4 * (size_t)ws;
             bf[0] = f0;
             bf[1] = f1;
             bf[2] = f2;
         }
-        const typename Lane::Word *Fs = s.F - slot + src_slot;
-        uint32_t *dst = A.best + (size_t)(w - slot + src_slot) * 3 * A.max_m;
-        for (int c = 0; c < 3; ++c)
-            for (int t = lane; t < ms; t += kThreadsPerGroup) dst[c * A.max_m + t] = Fs[(c * s.fs + t) * WS];
+        const typename Lane::Word *Fs = s.F - Lane::fslot(slot) + Lane::fslot(src_slot);
+        uint32_t *dst = A.best + (size_t)ws * 3 * A.max_m;
+        // the three components' reads go out together: one LDS round trip per pass
+        for (int t = lane; t < ms; t += kThreadsPerGroup) {
+            const uint32_t x0 = *Lane::fat(Fs, s.fs, 0, t), x1 = *Lane::fat(Fs, s.fs, 1, t),
+                           x2 = *Lane::fat(Fs, s.fs, 2, t);
+            dst[t] = x0;
+            dst[A.max_m + t] = x1;
+            dst[2 * A.max_m + t] = x2;
+        }
     }
 #else
     if (want) {
@@ -735,11 +824,12 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     using Word = typename Lane::Word;
     Lane s;
     const int hw = h_words(maxm);
-    const int fs = lds_term_stride(maxm, (int)sizeof(Word));
-    const size_t fbytes = walk_lds_f_bytes(maxm, (int)sizeof(Word));
-    const int slot = lane % kWalkersPerGroup;
-    s.side = lane / kWalkersPerGroup;
-    s.F = reinterpret_cast<Word *>(lds) + slot;
+    const int fs = Lane::kPacked ? 2 * Lane::kPackedPairs : lds_term_stride(maxm, (int)sizeof(Word));
+    constexpr int WS = Lane::kWS;
+    const size_t fbytes = Lane::kPacked ? z2s_lds_f_bytes() : walk_lds_f_bytes(maxm, (int)sizeof(Word), WS);
+    const int slot = lane % WS;
+    s.side = lane / WS;
+    s.F = reinterpret_cast<Word *>(lds) + Lane::fslot(slot);
     s.H = lds + fbytes + (size_t)slot * 4;
     if constexpr (Lane::kMask) {
         s.N = nullptr;
@@ -755,8 +845,11 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     s.err = A.err;
     s.maxm = maxm;
     s.hw = hw;
-    s.k0 = A.key0;
-    s.k1 = A.key1;
+    // one-lane layout: the Philox key is kept in vector registers, so the
+    // twenty round keys the compiler hoists out of the loop do not crowd the
+    // scalar registers (the kernel's SGPRs spill otherwise)
+    s.k0 = Lane::kPacked ? vgpr_copy(A.key0) : A.key0;
+    s.k1 = Lane::kPacked ? vgpr_copy(A.key1) : A.key1;
     s.wid = (uint32_t)w;
     s.m = 0;
     s.n0 = s.n1 = s.n2 = 1;
@@ -876,26 +969,40 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
         if (A.random_iterations) iters = 1 + (int)(s.aux32() % (uint32_t)A.max_iterations);
     }
     const bool check_reduce = A.check_reduce != 0;
+    // loop-invariant thresholds (vector registers in the one-lane layout, as the key)
+    const uint32_t thr_best_eq = Lane::kPacked ? vgpr_copy(A.thr_best_eq) : A.thr_best_eq;
+    const uint32_t thr_reduce = Lane::kPacked ? vgpr_copy(A.thr_reduce) : A.thr_reduce;
+    const uint32_t thr_expand = Lane::kPacked ? vgpr_copy(A.thr_expand) : A.thr_expand;
+    const int plus_iterations = Lane::kPacked ? (int)vgpr_copy((uint32_t)A.plus_iterations) : A.plus_iterations;
     // every lane runs the same number of loop trips so the wave stays
     // converged at the cooperative best copy
     const int trips = wave_max(iters);
     int min_m = s.m;
     const int log_slot = (kMode == kModeLearn && A.log_every > 0 && w % A.log_every == 0) ? w / A.log_every : -1;
 
-#if defined(__HIP_DEVICE_COMPILE__)
-    // The two lanes of a pair draw the main blocks of two iterations at once
-    // (side 0 the block of `it`, side 1 that of `it + 1`) and swap them, so
-    // each lane runs one Philox per two iterations.
+    // On the GPU the two lanes of a pair draw the main blocks of two
+    // iterations at once (side 0 the block of `it`, side 1 that of `it + 1`)
+    // and swap them, so each lane runs one Philox per two iterations.
     uint32_t nb0 = 0, nb1 = 0, nb2 = 0, nb3 = 0;
     bool have_next = false;
-#endif
     s.mark_start();
     for (int iter = 0; iter < trips; ++iter) {
         bool flipped = false, copy = false;
         uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
         if (iter < iters) {
-#if defined(__HIP_DEVICE_COMPILE__)
-            if (have_next) {
+            if constexpr (!Lane::kSplit) {
+                // one-lane layout: the block may have been computed during
+                // the previous iteration's flip
+                if (have_next) {
+                    b0 = nb0;
+                    b1 = nb1;
+                    b2 = nb2;
+                    b3 = nb3;
+                    have_next = false;
+                } else {
+                    philox10((uint32_t)it, (uint32_t)(it >> 32), (uint32_t)w, 0u, s.k0, s.k1, b0, b1, b2, b3);
+                }
+            } else if (have_next) {
                 b0 = nb0;
                 b1 = nb1;
                 b2 = nb2;
@@ -919,9 +1026,6 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
                 nb3 = s.side ? x3 : y3;
                 have_next = true;
             }
-#else
-            philox10((uint32_t)it, (uint32_t)(it >> 32), (uint32_t)w, 0u, s.k0, s.k1, b0, b1, b2, b3);
-#endif
             ++it;
             s.mark(0);
             const int rank = s.m;
@@ -946,6 +1050,15 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
             } else if constexpr (kMode == kModeLook) {
                 if (iter % A.look_period == A.look_period - 1) ok = s.try_lookahead(A.look_candidates, check_reduce);
                 if (!ok) ok = s.try_flip(b0, b1, check_reduce);
+            } else if constexpr (Lane::kPacked) {
+                // the next iteration's block (the counter is already advanced)
+                // fills the flip's first wait; the block is used only if the
+                // launch goes on
+                ok = s.try_flip(b0, b1, check_reduce, [&]() {
+                    philox10((uint32_t)it, (uint32_t)(it >> 32), (uint32_t)w, 0u, s.k0, s.k1, nb0, nb1, nb2, nb3);
+                    Lane::keep_loaded(nb0, nb1, nb2, nb3);  // computed here, not sunk to its use
+                    have_next = true;
+                });
             } else {
                 ok = s.try_flip(b0, b1, check_reduce);
             }
@@ -957,7 +1070,7 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
                 sum_m += (uint64_t)rank;
                 if (s.m < rank) flips_count = 0;
                 ++flips_count;
-                if (s.m < best_rank || (s.m == best_rank && b2 < A.thr_best_eq)) {  // :545-548
+                if (s.m < best_rank || (s.m == best_rank && b2 < thr_best_eq)) {  // :545-548
                     best_rank = s.m;
                     best_m = s.m;
                     copy = true;
@@ -967,18 +1080,18 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
         copy_best(s, A, copy, lane, w);
         s.mark(6);
         if (flipped) {
-            if (A.use_reduce && s.aux32() < A.thr_reduce) {  // :550-551
+            if (A.use_reduce && s.aux32() < thr_reduce) {  // :550-551
                 s.rec(6, 0, 0, 0, 0, 0, 0, 0);
                 s.try_reduce();
             }
-            if (A.use_expand && b3 < A.thr_expand && s.m <= best_m + 2) {   // :553-556
+            if (A.use_expand && b3 < thr_expand && s.m <= best_m + 2) {   // :553-556
                 if (s.try_expand(1 + (int)(s.aux32() % 2u))) flips_count = 0;
             }
             if constexpr (kMode == kModeMeta) {
                 if (A.use_sandwich && s.aux32() < A.thr_sandwich) s.sandwiching();  // :558-559
                 if (A.use_basis && s.aux32() < A.thr_basis) rz_swap_basis(s);       // :561-562
             }
-            if (flips_count >= A.plus_iterations && s.try_plus()) flips_count = 0;  // :564-565
+            if (flips_count >= plus_iterations && s.try_plus()) flips_count = 0;  // :564-565
         }
         if (A.min_window <= 0 || iter < A.min_window) min_m = s.m < min_m ? s.m : min_m;
         s.mark(7);
@@ -986,8 +1099,8 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     if (kMode == kModeLearn && active && s.side == 0 && A.minm) A.minm[w] = min_m;
 #if defined(FGK_PHASE_CLOCKS) && defined(__HIP_DEVICE_COMPILE__)
     if (active && kMode == kModeWalk) {
-        for (int q = 0; q < 10; ++q) atomicAdd(&g_phase[q], (unsigned long long)s.ph[q]);
-        atomicAdd(&g_phase[15], (unsigned long long)iters);
+        for (int q = 0; q < 10; ++q) atomicAdd(&FGK_PHASE_ARRAY[q], (unsigned long long)s.ph[q]);
+        atomicAdd(&FGK_PHASE_ARRAY[15], (unsigned long long)iters);
     }
 #endif
     }

@@ -82,7 +82,7 @@ struct fg_ctx {
     int nitems = 0;
     bool minimize = false;  // pool set up by fg_init_minimizer
     bool flips = false;     // pool set up by fg_init_flips
-    int variant = 0;        // walk kernel variant (kernels.h: kVarZ2 / kVarZT / kVarZ2h / kVarZ2m)
+    int variant = 0;        // walk kernel variant (kernels.h: kVarZ2 / kVarZT / kVarZ2h / kVarZ2m / kVarZ2s)
     int nn_max = 0;         // widest factor of the pool's formats
     bool masks_fit = false; // the class-mask layout can hold the pool
     bool meta = false;      // sandwiching / basis edges configured
@@ -285,12 +285,26 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
     // class masks: Z2 schemes of at most 32 terms with factors of up to 16
     // coefficients, unless 32-bit words, partner lists or the meta edges are asked for
     const bool masks_fit = !ring && nn_max <= 16 && maxm <= fgk::kMaskMaxTerms && !meta && ctx->cfg.word_bits != 32;
-    if (ctx->cfg.partner_layout < 0 || ctx->cfg.partner_layout > 2) return fail("init: partner_layout must be 0, 1 or 2");
-    if (ctx->cfg.partner_layout == 2 && !masks_fit)
+    if (ctx->cfg.partner_layout < 0 || ctx->cfg.partner_layout > 3) return fail("init: partner_layout must be 0, 1, 2 or 3");
+    if (ctx->cfg.partner_layout >= 2 && !masks_fit)
         return fail("init: class masks need Z2 schemes of at most 32 terms, factors of up to 16 coefficients, "
                     "16-bit words and no sandwiching or basis edges");
-    if (ctx->cfg.partner_layout == 2 || (ctx->cfg.partner_layout == 0 && masks_fit && ctx->cfg.word_bits == 0)) {
+    if (ctx->cfg.partner_layout == 3) {
+        variant = fgk::kVarZ2s;
+    } else if (ctx->cfg.partner_layout == 2 || (ctx->cfg.partner_layout == 0 && masks_fit && ctx->cfg.word_bits == 0)) {
         variant = fgk::kVarZ2m;
+        if (ctx->cfg.partner_layout == 0) {
+            // One lane per walker issues about two thirds of the instructions
+            // per walker-iteration that a lane pair does, but a SIMD then
+            // runs one wave where lane pairs run two.  It wins once the pool
+            // gives every SIMD its wave (3x3x3 at 65536 walkers: 24.7 -> 21.8
+            // ms per 10000-iteration launch); smaller pools keep lane pairs,
+            // whose waves reach more SIMDs.
+            int gs = 0;
+            HIPCHK(fgk::prepare_walk(fgk::kVarZ2s, maxm, mode == 1 ? 1 : 0, &gs));
+            const long solo_waves = (W + fgk::kSoloWalkersPerGroup - 1) / fgk::kSoloWalkersPerGroup;
+            if (gs > 0 && solo_waves >= (long)gs * ctx->cus && (mode == 0 || mode == 1)) variant = fgk::kVarZ2s;
+        }
     } else if (!ring && ctx->cfg.word_bits == 16) {
         if (nn_max > 16) return fail("init: 16-bit factor words hold factors of up to 16 coefficients");
         variant = fgk::kVarZ2h;
@@ -648,7 +662,7 @@ extern "C" int fg_walk_trace(fg_ctx *ctx, int iterations, int cap, int32_t *ops,
 
 extern "C" int fg_resize(fg_ctx *ctx, double probability) {
     if (!ctx || !ctx->ready) return fail("fg_resize: context not initialised");
-    if (ctx->variant == fgk::kVarZ2h || ctx->variant == fgk::kVarZ2m || ctx->minimize || ctx->flips)
+    if (ctx->variant == fgk::kVarZ2h || fgk::mask_variant(ctx->variant) || ctx->minimize || ctx->flips)
         return fail("fg_resize: resize edges run on walk pools with 32-bit factor words (word_bits 32 for Z2)");
     if (!(probability > 0)) return 0;
     return walk_launch(ctx, 1, false, probability);
@@ -973,7 +987,10 @@ extern "C" int fg_autotune(fg_ctx *ctx, int iterations, int32_t *grid) {
     const bool open = ctx->cfg.ring == FG_RING_Z2 && ctx->cfg.word_bits == 0 && ctx->cfg.partner_layout == 0 &&
                       !ctx->meta && !ctx->log.on;
     if (open) {
-        if (ctx->masks_fit) variants.push_back(fgk::kVarZ2m);
+        if (ctx->masks_fit) {
+            variants.push_back(fgk::kVarZ2s);
+            variants.push_back(fgk::kVarZ2m);
+        }
         if (ctx->nn_max <= 16) variants.push_back(fgk::kVarZ2h);
         variants.push_back(fgk::kVarZ2);
     } else {

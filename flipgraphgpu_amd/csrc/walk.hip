@@ -9,40 +9,12 @@
 #include "walk_z2_lane.h"
 #include "walk_zt_lane.h"
 #include "resize_ops.h"
+#include "walk_persistent.h"
 
 namespace fgk {
 #if defined(FGK_PHASE_CLOCKS)
 __device__ unsigned long long g_phase[16];
 #endif
-
-// Persistent walk: each workgroup (one wave, 32 walkers) claims blocks of
-// walkers from the queue until none is left.  Blocks never straddle two
-// formats and are queued largest format first, so uneven formats share one
-// launch with little tail.  Every wave exits once the queue is exhausted.
-// Inlined into each kernel so the argument block is read straight from the
-// kernel-argument segment (an out-of-line call would copy it to scratch).
-template <class Lane, int kMode>
-__device__ __attribute__((always_inline)) void walk_persistent(const WalkArgs &A, uint8_t *lds) {
-    const int lane = threadIdx.x;
-    if (A.args_bytes != (int32_t)sizeof(WalkArgs)) {
-        if (lane == 0) atomicOr(A.err, kErrArgs);
-        return;
-    }
-    for (;;) {
-        int item = 0;
-        if (lane == 0) item = atomicAdd(A.queue, 1);
-        item = __builtin_amdgcn_readfirstlane(item);
-        if (item >= A.nitems) break;
-        lane::walk_lane<Lane, kMode>(A, lds, lane, A.item_begin[item] + lane % kWalkersPerGroup, A.item_end[item]);
-    }
-}
-
-// Walk-family kernels: one wave per workgroup.  With 32-bit factor words a
-// workgroup's LDS block is at least 18.8 KB (factor planes of 33 rows), so at
-// most 8 fit a CU: 2 waves per SIMD.  Telling the compiler so lets it use up
-// to 256 VGPRs (3x3x3 Z2 launch 35.9 -> 33.7 ms, integer ring 29.3 -> 25.3
-// ms); the grid follows the occupancy the runtime queries, registers included.
-#define FGK_WALK_KERNEL __global__ void __launch_bounds__(kThreadsPerGroup) __attribute__((amdgpu_waves_per_eu(1, 2)))
 
 FGK_WALK_KERNEL walk_z2_kernel(WalkArgs A) {
     extern __shared__ uint8_t lds[];
@@ -336,6 +308,7 @@ __global__ void verify_zt_kernel(VerifyArgs V) {
 
 size_t walk_lds_bytes(int variant, int max_m) {
     if (variant == kVarZ2m) return z2m_lds_bytes(max_m);
+    if (variant == kVarZ2s) return z2s_lds_bytes(max_m);
     return variant == kVarZT ? zt_lds_bytes(max_m) : z2_lds_bytes(max_m, variant == kVarZ2h ? 2 : 4);
 }
 
@@ -355,7 +328,9 @@ static const void *walk_kernel(int variant, int mode) {
          reinterpret_cast<const void *>(learn_z2m_kernel), nullptr, nullptr, nullptr,
          reinterpret_cast<const void *>(look_z2m_kernel), reinterpret_cast<const void *>(trace_z2m_kernel)},
     };
-    if (variant < 0 || variant > 3 || mode < 0 || mode > 7) return nullptr;
+    if (mode < 0 || mode > 7) return nullptr;
+    if (variant == kVarZ2s) return solo_walk_kernel(mode);
+    if (variant < 0 || variant > 3) return nullptr;
     return k[variant][mode];
 }
 

@@ -13,6 +13,9 @@ namespace fgk {
 constexpr int kThreadsPerGroup = 64;
 constexpr int kLanesPerWalker = 2;
 constexpr int kWalkersPerGroup = kThreadsPerGroup / kLanesPerWalker;
+// The one-lane layout (class masks only): every lane runs a walker of its
+// own, so a workgroup holds 64 walkers and takes two queue items at a time.
+constexpr int kSoloWalkersPerGroup = kThreadsPerGroup;
 
 // Device-resident walker pool.  Factor planes are structure-of-arrays,
 // word(c, t, w) = words[(c * max_m + t) * walkers + w], so a wave touching
@@ -120,9 +123,10 @@ __host__ __device__ inline int lds_term_stride(int max_m, int word_bytes) {
     return fs;
 }
 
-// LDS bytes per workgroup of the factor planes F (word_bytes per factor).
-__host__ __device__ inline size_t walk_lds_f_bytes(int max_m, int word_bytes) {
-    return ((size_t)kWalkersPerGroup * 3 * lds_term_stride(max_m, word_bytes) * word_bytes + 3) & ~(size_t)3;
+// LDS bytes per workgroup of the factor planes F (word_bytes per factor,
+// wpg walkers per workgroup, one word column per walker).
+__host__ __device__ inline size_t walk_lds_f_bytes(int max_m, int word_bytes, int wpg = kWalkersPerGroup) {
+    return ((size_t)wpg * 3 * lds_term_stride(max_m, word_bytes) * word_bytes + 3) & ~(size_t)3;
 }
 
 // LDS bytes per workgroup of the common planes: factor words F and the
@@ -141,6 +145,14 @@ __host__ __device__ inline int lds_mask_stride(int max_m) { return max_m | 1; }
 __host__ __device__ inline size_t z2m_lds_bytes(int max_m) {
     return walk_lds_f_bytes(max_m, 2) + (size_t)kWalkersPerGroup * 3 * (size_t)h_words(max_m) * 4 +
            (size_t)kWalkersPerGroup * 3 * (size_t)lds_mask_stride(max_m) * 4;
+}
+// One-lane class-mask layout (64 walkers): F holds 32 rows of 16-bit words
+// per component packed two per dword (one dword column per walker), then H
+// and CM as above.
+__host__ __device__ inline size_t z2s_lds_f_bytes() { return (size_t)kSoloWalkersPerGroup * 3 * 16 * 4; }
+__host__ __device__ inline size_t z2s_lds_bytes(int max_m) {
+    return z2s_lds_f_bytes() + (size_t)kSoloWalkersPerGroup * 3 * (size_t)h_words(max_m) * 4 +
+           (size_t)kSoloWalkersPerGroup * 3 * (size_t)lds_mask_stride(max_m) * 4;
 }
 
 // Integer ring: the candidate pairs one tryFlip visits one at a time

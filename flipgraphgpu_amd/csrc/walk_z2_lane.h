@@ -59,9 +59,11 @@ FGK_HD uint32_t gf2_right(uint32_t x, const Gf2Mat &R, int a, int b) {
     return y;
 }
 
-template <typename WordT, bool kMaskT = false>
-struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
-    using B = LaneBaseT<WordT, kMaskT>;
+template <typename WordT, bool kMaskT = false, int kWSv = kWalkersPerGroup>
+struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
+    using B = LaneBaseT<WordT, kMaskT, kWSv>;
+    // the two halves of a flip run on the two lanes of a pair
+    static constexpr bool kPairSplit = LaneBaseT<WordT, kMaskT, kWSv>::kSplit;
     using typename B::Word;
     using typename B::JoinInfo;
     using B::F;
@@ -193,43 +195,50 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
             return;
         }
         const uint32_t fi = f(i, i1);
-#if defined(__HIP_DEVICE_COMPILE__)
-        const int mc = side ? k : j, x = side ? i2 : i1, oc = side ? j : k;
-        const uint32_t nv = side ? B : A, ov = side ? fj2 : fk1;
-        const Half mine = flip_half(mc, x, nv, i, oc, fi, ov, check_reduce);
-        const int ot = pair_exchange(mine.t, side), orule = pair_exchange(mine.rule, side);
-        const int tA = side ? ot : mine.t, rA = side ? orule : mine.rule;
-        const int tB = side ? mine.t : ot, rB = side ? mine.rule : orule;
-        this->mark(4);
-#else
-        const Half ha = flip_half(j, i1, A, i, k, fi, fk1, check_reduce);
-        const Half hb = flip_half(k, i2, B, i, j, fi, fj2, check_reduce);
-        const int tA = ha.t, rA = ha.rule, tB = hb.t, rB = hb.rule;
-#endif
-        if (rA || rB) {
+        if constexpr (kPairSplit) {
+            const int mc = side ? k : j, x = side ? i2 : i1, oc = side ? j : k;
+            const uint32_t nv = side ? B : A, ov = side ? fj2 : fk1;
+            const Half mine = flip_half(mc, x, nv, i, oc, fi, ov, check_reduce);
+            const int ot = pair_exchange(mine.t, side), orule = pair_exchange(mine.rule, side);
+            const int tA = side ? ot : mine.t, rA = side ? orule : mine.rule;
+            const int tB = side ? mine.t : ot, rB = side ? mine.rule : orule;
+            this->mark(4);
+            if (rA || rB) {
+                flip_reduce(i, j, k, i1, i2, A, B, tA, rA, tB, rB);
+                return;
+            }
+            pop_update(fj1, A, fk2, B);
+            const uint32_t before = Sc(mc);
+            leave(mc, x);
+            if (mine.any >= 0) link_join(mc, x, mine.ji);
+            f(mc, x) = nv;
+            addS(oc, pair_exchange((int)(Sc(mc) - before), side));
+            this->mark(5);
+        } else {
+            const Half ha = flip_half(j, i1, A, i, k, fi, fk1, check_reduce);
+            const Half hb = flip_half(k, i2, B, i, j, fi, fj2, check_reduce);
+            if (ha.rule || hb.rule) {
+                flip_reduce(i, j, k, i1, i2, A, B, ha.t, ha.rule, hb.t, hb.rule);
+                return;
+            }
+            pop_update(fj1, A, fk2, B);
+            leave(j, i1);
+            if (ha.any >= 0) link_join(j, i1, ha.ji);
             f(j, i1) = A;
+            leave(k, i2);
+            if (hb.any >= 0) link_join(k, i2, hb.ji);
             f(k, i2) = B;
-            if (rA && tA <= tB) reduce(rA == 1 ? k : i, tA, i1);
-            else reduce(rB == 1 ? j : i, tB, i2);
-            this->mark(8);
-            return;
         }
-        pop_update(fj1, A, fk2, B);
-#if defined(__HIP_DEVICE_COMPILE__)
-        const uint32_t before = Sc(mc);
-        leave(mc, x);
-        if (mine.any >= 0) link_join(mc, x, mine.ji);
-        f(mc, x) = nv;
-        addS(oc, pair_exchange((int)(Sc(mc) - before), side));
-        this->mark(5);
-#else
-        leave(j, i1);
-        if (ha.any >= 0) link_join(j, i1, ha.ji);
+    }
+    // The reduction a flip found (first match: the lowest term, the index1
+    // half before the index2 half at equal terms), after the flip's updates.
+    FGK_HD void flip_reduce(int i, int j, int k, int i1, int i2, uint32_t A, uint32_t Bv, int tA, int rA, int tB,
+                            int rB) {
         f(j, i1) = A;
-        leave(k, i2);
-        if (hb.any >= 0) link_join(k, i2, hb.ji);
-        f(k, i2) = B;
-#endif
+        f(k, i2) = Bv;
+        if (rA && tA <= tB) reduce(rA == 1 ? k : i, tA, i1);
+        else reduce(rB == 1 ? j : i, tB, i2);
+        this->mark(8);
     }
 
     // flip with class masks (every involved factor non-zero).  A half of the
@@ -279,37 +288,36 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
     }
     FGK_HD void flip_masks(int i, int j, int k, int i1, int i2, uint32_t A, uint32_t Bv, uint32_t fj1, uint32_t fk2,
                            bool check_reduce) {
-#if defined(__HIP_DEVICE_COMPILE__)
-        const int mc = side ? k : j, x = side ? i2 : i1, oc = side ? j : k;
-        const uint32_t nv = side ? Bv : A;
-        const MaskHalf mine = mask_half(mc, x, nv, i, oc, check_reduce);
-        const int ot = pair_exchange(mine.t, side), orule = pair_exchange(mine.rule, side);
-        const int tA = side ? ot : mine.t, rA = side ? orule : mine.rule;
-        const int tB = side ? mine.t : ot, rB = side ? mine.rule : orule;
-        this->mark(4);
-#else
-        const MaskHalf ha = mask_half(j, i1, A, i, k, check_reduce);
-        const MaskHalf hb = mask_half(k, i2, Bv, i, j, check_reduce);
-        const int tA = ha.t, rA = ha.rule, tB = hb.t, rB = hb.rule;
-#endif
-        if (rA || rB) {
-            f(j, i1) = A;
-            f(k, i2) = Bv;
-            if (rA && tA <= tB) reduce(rA == 1 ? k : i, tA, i1);
-            else reduce(rB == 1 ? j : i, tB, i2);
-            this->mark(8);
-            return;
+        int dj = 0, dk = 0;
+        if constexpr (kPairSplit) {
+            const int mc = side ? k : j, x = side ? i2 : i1, oc = side ? j : k;
+            const uint32_t nv = side ? Bv : A;
+            const MaskHalf mine = mask_half(mc, x, nv, i, oc, check_reduce);
+            const int ot = pair_exchange(mine.t, side), orule = pair_exchange(mine.rule, side);
+            const int tA = side ? ot : mine.t, rA = side ? orule : mine.rule;
+            const int tB = side ? mine.t : ot, rB = side ? mine.rule : orule;
+            this->mark(4);
+            if (rA || rB) {
+                flip_reduce(i, j, k, i1, i2, A, Bv, tA, rA, tB, rB);
+                return;
+            }
+            pop_update(fj1, A, fk2, Bv);
+            const int d = mask_move(mc, x, mine.old, mine.nw, nv);
+            const int od = pair_exchange(d, side);
+            dj = side ? od : d;
+            dk = side ? d : od;
+            this->mark(5);
+        } else {
+            const MaskHalf ha = mask_half(j, i1, A, i, k, check_reduce);
+            const MaskHalf hb = mask_half(k, i2, Bv, i, j, check_reduce);
+            if (ha.rule || hb.rule) {
+                flip_reduce(i, j, k, i1, i2, A, Bv, ha.t, ha.rule, hb.t, hb.rule);
+                return;
+            }
+            pop_update(fj1, A, fk2, Bv);
+            dj = mask_move(j, i1, ha.old, ha.nw, A);
+            dk = mask_move(k, i2, hb.old, hb.nw, Bv);
         }
-        pop_update(fj1, A, fk2, Bv);
-#if defined(__HIP_DEVICE_COMPILE__)
-        const int d = mask_move(mc, x, mine.old, mine.nw, nv);
-        const int od = pair_exchange(d, side);
-        const int dj = side ? od : d, dk = side ? d : od;
-        this->mark(5);
-#else
-        const int dj = mask_move(j, i1, ha.old, ha.nw, A);
-        const int dk = mask_move(k, i2, hb.old, hb.nw, Bv);
-#endif
         // component i keeps its classes; j and k change by dj and dk
         S0 += (uint32_t)(j == 0 ? dj : (k == 0 ? dk : 0));
         S1 += (uint32_t)(j == 1 ? dj : (k == 1 ? dk : 0));
@@ -339,13 +347,13 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
         const uint32_t A = fj1 ^ fj2, B = fk2 ^ fk1;
         if (!(fj1 && fj2 && fk1 && fk2 && A && B)) return true;
         const uint32_t fi = f(i, i1);
-#if defined(__HIP_DEVICE_COMPILE__)
-        const int mc = side ? k : j, x = side ? i2 : i1, oc = side ? j : k;
-        const int mine = half_reduces(mc, x, side ? B : A, i, oc, fi, side ? fj2 : fk1) ? 1 : 0;
-        return (mine | pair_exchange(mine, side)) != 0;
-#else
-        return half_reduces(j, i1, A, i, k, fi, fk1) || half_reduces(k, i2, B, i, j, fi, fj2);
-#endif
+        if constexpr (kPairSplit) {
+            const int mc = side ? k : j, x = side ? i2 : i1, oc = side ? j : k;
+            const int mine = half_reduces(mc, x, side ? B : A, i, oc, fi, side ? fj2 : fk1) ? 1 : 0;
+            return (mine | pair_exchange(mine, side)) != 0;
+        } else {
+            return half_reduces(j, i1, A, i, k, fi, fk1) || half_reduces(k, i2, B, i, j, fi, fj2);
+        }
     }
     // `count` candidates drawn as tryFlip draws them (an aux word for the
     // pair in fresh-list order, an aux word for the orientation coins); the
@@ -512,9 +520,165 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
         return false;
     }
 
+    // tryFlip of the one-lane layout: the same pair, orientation and flip as
+    // the general path below (flip_masks), with the LDS reads of an iteration
+    // in four dependent batches: (1) the H words of the pair's component and
+    // the packed factor rows of the two components the flip changes, both
+    // known from the draw alone; (2) t1's class mask, which gives t2; (3) the
+    // factors and class masks of index1 and index2 the halves need; (4) the
+    // class stores.  The new classes come from the rows of (1) compared with
+    // the new factors (packed_ne) instead of a member's class mask.
+    // `pre` runs once per call, after the first batch of reads is issued, so
+    // the caller's work that needs no scheme state (the next iteration's
+    // Philox block) fills the wait for them.
+    template <class Pre>
+    FGK_HD bool try_flip_packed(uint32_t r, uint32_t coin, bool check_reduce, Pre &&pre) {
+        const uint32_t total = S0 + S1 + S2;
+        if (!total) {
+            pre();
+            return false;
+        }
+        uint32_t rr = draw_below(r, total);
+        int c = 0;
+        if (rr >= S0) {
+            rr -= S0;
+            c = 1;
+            if (rr >= S1) {
+                rr -= S1;
+                c = 2;
+            }
+        }
+        int j, k;
+        orientation(c, coin, j, k);
+        uint32_t hw8[8], Rj[16], Rk[16];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) hw8[e] = this->hword(c, e);
+#pragma unroll
+        for (int p = 0; p < 16; ++p) {
+            Rj[p] = this->fpair(j, p);
+            Rk[p] = this->fpair(k, p);
+        }
+        pre();
+        int t1;
+        B::select_words(hw8, rr, t1);
+        if (t1 < 0 || t1 >= m) {
+            this->fail(kErrSelect);
+            return false;
+        }
+        // the (rr + 1)-th member above t1
+        uint32_t q = cm(c, t1) & ~((2u << t1) - 1u);
+        for (uint32_t e = 0; e < rr; ++e) q &= q - 1u;
+        if (!q) {
+            this->fail(kErrSelect);
+            return false;
+        }
+        const int t2 = __builtin_ctz(q);
+        this->mark(1);
+        sel_c = c;
+        sel_t1 = t1;
+        sel_t2 = t2;
+        const int i = c;
+        const int i1 = (coin & 2u) ? t2 : t1, i2 = (coin & 2u) ? t1 : t2;
+        this->rec(0, i, j, k, i1, i2, check_reduce ? 1 : 0, 0);
+        const uint32_t fj1 = f(j, i1), fj2 = f(j, i2), fk1 = f(k, i1), fk2 = f(k, i2);
+        const uint32_t oldj = cm(j, i1), ci1 = cm(i, i1), ck1 = cm(k, i1);
+        const uint32_t oldk = cm(k, i2), ci2 = cm(i, i2), cj2 = cm(j, i2);
+        // one round trip for all ten reads: the masks are read before the
+        // zero-factor test, not after it
+        keep_loaded(fj1, fj2, fk1, fk2, oldj, ci1, ck1, oldk, ci2, cj2);
+        const uint32_t A = fj1 ^ fj2, Bv = fk2 ^ fk1;
+        if (!(fj1 && fj2 && fk1 && fk2 && A && Bv)) {
+            flip_literal(i, j, k, i1, i2, check_reduce);
+            this->mark(9);
+            return true;
+        }
+        // new classes: the rows equal to the new factors (rows from the rank
+        // up are zero, new factors are not); neither changed term is in them
+        const uint32_t nwj = ~packed_ne(Rj, A), nwk = ~packed_ne(Rk, Bv);
+        this->mark(2);
+        const uint32_t redj = check_reduce ? (nwj & (ci1 | ck1)) : 0u;
+        const uint32_t redk = check_reduce ? (nwk & (ci2 | cj2)) : 0u;
+        if (redj || redk) {
+            const int tA = redj ? __builtin_ctz(redj) : (1 << 30);
+            const int rA = redj ? (((ci1 >> tA) & 1u) ? 1 : 2) : 0;
+            const int tB = redk ? __builtin_ctz(redk) : (1 << 30);
+            const int rB = redk ? (((ci2 >> tB) & 1u) ? 1 : 2) : 0;
+            flip_reduce(i, j, k, i1, i2, A, Bv, tA, rA, tB, rB);
+            return true;
+        }
+        this->mark(3);
+        pop_update(fj1, A, fk2, Bv);
+        int dj, dk;
+        mask_move2(j, i1, oldj, nwj, A, k, i2, oldk, nwk, Bv, dj, dk);
+        S0 += (uint32_t)(j == 0 ? dj : (k == 0 ? dk : 0));
+        S1 += (uint32_t)(j == 1 ? dj : (k == 1 ? dk : 0));
+        S2 += (uint32_t)(j == 2 ? dj : (k == 2 ? dk : 0));
+        this->mark(5);
+        return true;
+    }
+
+    // Values that must be at hand at this point (GPU: an empty asm that
+    // reads them, so their loads or computation are not sunk past it).
+    template <class... T>
+    FGK_HD static void keep_loaded(T... v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        (asm_use(v), ...);
+#else
+        ((void)v, ...);
+#endif
+    }
+    FGK_HD static void asm_use(uint32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        asm volatile("" ::"v"(v));
+#else
+        (void)v;
+#endif
+    }
+    // mask_move for both halves of a flip in one loop: each trip moves one
+    // member of each half's list (old class without x, new class); a list
+    // already empty repeats its own term x, with a class-mask store the
+    // final one overwrites and an H add of 0.  The wave loops over the
+    // longer list of each lane instead of both lists one after the other.
+    FGK_HD void mask_move2(int j, int xj, uint32_t oldj, uint32_t nwj, uint32_t vj, int k, int xk, uint32_t oldk,
+                           uint32_t nwk, uint32_t vk, int &dj, int &dk) {
+        const uint32_t bj = 1u << xj, belj = bj - 1u, restj = oldj & ~bj, joinj = nwj | bj;
+        const uint32_t bk = 1u << xk, belk = bk - 1u, restk = oldk & ~bk, joink = nwk | bk;
+        uint32_t qj = restj | nwj, qk = restk | nwk;
+        while (qj | qk) {
+            const int yj = qj ? __builtin_ctz(qj) : xj;
+            const int yk = qk ? __builtin_ctz(qk) : xk;
+            const bool nj = (nwj >> yj) & 1u, nk = (nwk >> yk) & 1u;
+            cm(j, yj) = nj ? joinj : restj;
+            cm(k, yk) = nk ? joink : restk;
+            h_add(j, yj, ((belj >> yj) & 1u) ? (nj ? 1 : -1) : 0);
+            h_add(k, yk, ((belk >> yk) & 1u) ? (nk ? 1 : -1) : 0);
+            qj &= qj - 1u;
+            qk &= qk - 1u;
+        }
+        cm(j, xj) = joinj;
+        cm(k, xk) = joink;
+        this->h(j, xj) = (uint8_t)popc(nwj & ~(belj | bj));
+        this->h(k, xk) = (uint8_t)popc(nwk & ~(belk | bk));
+        f(j, xj) = (Word)vj;
+        f(k, xk) = (Word)vk;
+        dj = popc(nwj) - popc(restj);
+        dk = popc(nwk) - popc(restk);
+    }
+
     // tryFlip (scheme_z2.cu:982-1027): r picks the pair in fresh-list order,
-    // coin bit 0 swaps j/k, coin bit 1 swaps the indices.
+    // coin bit 0 swaps j/k, coin bit 1 swaps the indices.  `pre` runs once per
+    // call (see try_flip_packed).
+    template <class Pre>
+    FGK_HD bool try_flip(uint32_t r, uint32_t coin, bool check_reduce, Pre &&pre) {
+        if constexpr (B::kPacked) {
+            return try_flip_packed(r, coin, check_reduce, pre);
+        } else {
+            pre();
+            return try_flip(r, coin, check_reduce);
+        }
+    }
     FGK_HD bool try_flip(uint32_t r, uint32_t coin, bool check_reduce) {
+        if constexpr (B::kPacked) return try_flip_packed(r, coin, check_reduce, [] {});
         const uint32_t total = S0 + S1 + S2;
         if (!total) return false;
         int c, t1, t2, j, k;
@@ -596,6 +760,7 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT> {
 using Z2Lane = Z2LaneT<uint32_t>;    // Z2 factors of up to 32 coefficients
 using Z2Lane16 = Z2LaneT<uint16_t>;  // Z2 factors of up to 16 coefficients
 using Z2LaneM = Z2LaneT<uint16_t, true>;  // up to 16 coefficients and 32 terms, class masks
+using Z2LaneS = Z2LaneT<uint16_t, true, kSoloWalkersPerGroup>;  // the same, one lane per walker
 
 }  // namespace lane
 }  // namespace fgk

@@ -69,7 +69,7 @@ struct ZTLane : LaneBaseT<uint32_t> {
     uint16_t *V;  // lane-offset base of the visited-candidate list
 
     FGK_HD void lane_extra(uint8_t *base, int lane) { V = reinterpret_cast<uint16_t *>(base) + lane; }
-    FGK_HD uint16_t &visited(int q) { return V[q * WS]; }
+    FGK_HD uint16_t &visited(int q) { return V[q * kWS]; }
 
     // fixSigns (scheme_integer.cu:398-425)
     FGK_HD void fix_signs() {

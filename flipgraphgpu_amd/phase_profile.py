@@ -7,7 +7,7 @@ each phase and the cycles per walk iteration.  Marks wait for outstanding LDS
 operations, so this build runs a few percent slower than the product; the
 shares, not the absolute times, are what it is for.
 
-Usage: python -m flipgraphgpu_amd.phase_profile [--layout 0|1|2] [--word-bits 0|16|32]
+Usage: python -m flipgraphgpu_amd.phase_profile [--layout 0|1|2|3] [--word-bits 0|16|32]
 """
 import argparse
 import ctypes
@@ -26,7 +26,8 @@ def main():
     ap.add_argument("--walkers", type=int, default=65536)
     ap.add_argument("--iterations", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--layout", type=int, default=0, help="partner_layout: 0 auto, 1 lists, 2 class masks")
+    ap.add_argument("--layout", type=int, default=0,
+                    help="partner_layout: 0 auto, 1 lists, 2 class masks, 3 class masks with one lane per walker")
     ap.add_argument("--word-bits", type=int, default=0)
     a = ap.parse_args()
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_phase", "libflipgraph.so")
@@ -35,12 +36,14 @@ def main():
     _lib.LIB_PATH = path
     L = _lib.load_library()
     L.fg_phase_read.argtypes = [ctypes.POINTER(ctypes.c_ulonglong)]
+    L.fg_phase_read_solo.argtypes = [ctypes.POINTER(ctypes.c_ulonglong)]
     e = _lib.Engine(a.walkers, 1, max_iterations=a.iterations, plus_iterations=20000, word_bits=a.word_bits,
                     partner_layout=a.layout)
     e.init_naive(*a.n)
     for _ in range(a.warmup):
         e.walk(a.iterations)
     L.fg_phase_reset()
+    L.fg_phase_reset_solo()
     c0 = e.counters()
     e.walk(a.iterations)
     ms = e.last_walk_ms()
@@ -49,6 +52,10 @@ def main():
     e.close()
     buf = (ctypes.c_ulonglong * 16)()
     L.fg_phase_read(buf)
+    solo = (ctypes.c_ulonglong * 16)()
+    L.fg_phase_read_solo(solo)  # the one-lane layout's kernels keep their own sums
+    for q in range(16):
+        buf[q] += solo[q]
     total = sum(buf[:10])
     iters = buf[15]
     print(json.dumps({

@@ -49,7 +49,8 @@ typedef struct {
     double basis_probability;       /* --basis-probability (divided by max iterations; Z2 needs 32-bit words) */
     int32_t partner_layout;     /* Z2 flip-partner bookkeeping: 0 = class masks when the scheme has at most 32
                                    terms and 16-coefficient factors (and word_bits is 0), else partner lists;
-                                   1 = partner lists; 2 = class masks */
+                                   1 = partner lists; 2 = class masks; 3 = class masks with one lane per
+                                   walker (64 walkers per workgroup instead of lane pairs serving 32) */
     int32_t lookahead_period;   /* Z2 reduction-seeking proposals (no reference counterpart; 0 = off): every
                                    lookahead_period-th iteration of a launch first draws lookahead_candidates
                                    pairs with orientation coins from the aux stream and applies the first whose

@@ -85,24 +85,34 @@ extern "C" int emu_walk_records(int ring, fg_walker_record *recs, int count, uin
     for (int q = 0; q < count; ++q)
         for (int c = 0; c < 3; ++c) nn_max = std::max(nn_max, recs[q].n[c] * recs[q].n[(c + 1) % 3]);
     // force_wide: 0 the runtime's automatic choice, 1 32-bit words with partner
-    // lists, 2 16-bit words with partner lists, 3 class masks
+    // lists, 2 16-bit words with partner lists, 3 class masks, 4 class masks
+    // in the one-lane layout (packed rows, 64 walkers per workgroup)
     const bool small = !ring && nn_max <= 16 && mode != fgk::lane::kModeResize && mode != fgk::lane::kModeMeta;
+    const bool solo = small && maxm <= fgk::kMaskMaxTerms && force_wide == 4;
     const bool masks = small && maxm <= fgk::kMaskMaxTerms && (force_wide == 0 || force_wide == 3);
-    const bool half = small && !masks && force_wide != 1;
+    const bool half = small && !masks && !solo && force_wide != 1;
     std::vector<uint8_t> lds(ring ? fgk::zt_lds_bytes(maxm)
-                                  : (masks ? fgk::z2m_lds_bytes(maxm) : fgk::z2_lds_bytes(maxm, half ? 2 : 4)),
+                                  : (solo ? fgk::z2s_lds_bytes(maxm)
+                                          : (masks ? fgk::z2m_lds_bytes(maxm) : fgk::z2_lds_bytes(maxm, half ? 2 : 4))),
                              0xCD);
-    const int G = (W + fgk::kWalkersPerGroup - 1) / fgk::kWalkersPerGroup;
+    const int WPG = solo ? fgk::kSoloWalkersPerGroup : fgk::kWalkersPerGroup;
+    const int G = (W + WPG - 1) / WPG;
     for (int l = 0; l < launches; ++l) {
         if (l > 0) a.forced = nullptr;  // proposals apply to the first launch
         for (int g = 0; g < G; ++g)
-            for (int slot = 0; slot < fgk::kWalkersPerGroup; ++slot) {
+            for (int slot = 0; slot < WPG; ++slot) {
                 // one call per walker: the host side of the lane code does both flip halves
-                const int lane = slot;  // the side-0 lane of the slot's pair
-                const int w = g * fgk::kWalkersPerGroup + slot;
+                const int lane = slot;  // the side-0 lane of the slot's pair (the slot's lane when solo)
+                const int w = g * WPG + slot;
                 using namespace fgk::lane;
                 uint8_t *L = lds.data();
-                if (ring) {
+                if (solo) {
+                    if (mode == kModeMinimize) walk_lane<Z2LaneS, kModeMinimize>(a, L, lane, w, W);
+                    else if (mode == kModeLearn) walk_lane<Z2LaneS, kModeLearn>(a, L, lane, w, W);
+                    else if (mode == kModeTrace) walk_lane<Z2LaneS, kModeTrace>(a, L, lane, w, W);
+                    else if (mode == kModeLook) walk_lane<Z2LaneS, kModeLook>(a, L, lane, w, W);
+                    else walk_lane<Z2LaneS, kModeWalk>(a, L, lane, w, W);
+                } else if (ring) {
                     if (mode == kModeMinimize) walk_lane<ZTLane, kModeMinimize>(a, L, lane, w, W);
                     else if (mode == kModeFlips) walk_lane<ZTLane, kModeFlips>(a, L, lane, w, W);
                     else if (mode == kModeResize) walk_lane<ZTLane, kModeResize>(a, L, lane, w, W);

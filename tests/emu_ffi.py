@@ -29,7 +29,7 @@ def lib():
     return _lib
 
 
-LAYOUTS = {"auto": 0, "wide": 1, "half": 2, "masks": 3}
+LAYOUTS = {"auto": 0, "wide": 1, "half": 2, "masks": 3, "solo": 4}
 
 
 def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=False, wide=False, resize=None,
@@ -38,8 +38,8 @@ def walk(recs, params, seed, launches=1, ring=O.RING_Z2, forced=None, learn=Fals
     pairs `forced` [walkers][4] and the result adds (log_meta, minm).  Z2
     layouts as the runtime picks them (auto: class masks up to 32 terms and
     16 coefficients, else 16-bit words with partner lists up to 16
-    coefficients, else 32-bit words), or forced: "wide", "half", "masks"
-    (wide=True is "wide").  resize = (thr_resize, max_rank, max_coef) runs the
+    coefficients, else 32-bit words), or forced: "wide", "half", "masks",
+    "solo" (class masks, one lane per walker; wide=True is "wide").  resize = (thr_resize, max_rank, max_coef) runs the
     resize mode instead of the walk.  trace_cap > 0 runs the trace mode (one
     launch) and the result adds (ops [walkers][trace_cap][8], counts)."""
     if wide:

@@ -46,6 +46,24 @@ def test_batched_allocation_and_parity():
     assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
 
 
+def test_batched_one_lane_layout():
+    # formats of up to 32 terms in the one-lane class-mask layout: a wave
+    # serves two queue blocks, which may hold different formats
+    W, iters, seed = 500, 1200, 78
+    fmts, weights = [(2, 2, 3), (2, 3, 4), (3, 3, 3)], [1, 2, 4]
+    print(f"[batched-solo] W={W} iters={iters}", flush=True)
+    with fg.Engine(W, seed, max_iterations=iters, plus_iterations=300, expand_probability=10.0,
+                   partner_layout=3) as e:
+        e.init_batched(fmts, weights)
+        start = e.read_walkers()
+        assert e.walk(iters) == 0
+        assert e.walk(iters) == 0
+        gpu = e.read_walkers()
+    p = O.make_params(iterations=iters, plus_iterations=300, p_expand=10.0)
+    ora = O.walk_records(start, p, seed, launches=2, nthreads=8)
+    assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
+
+
 def test_graph_replay_matches_plain_launches():
     W, iters, seed = 512, 800, 5
     out = []
@@ -62,8 +80,8 @@ def test_graph_replay_matches_plain_launches():
 @pytest.mark.parametrize("n,W,use_graph", [((3, 3, 3), 2048, False), ((3, 3, 3), 2048, True),
                                             ((4, 4, 4), 512, True)])
 def test_autotune_keeps_results_exact(n, W, use_graph):
-    """The autotuner times every layout the pool fits (class masks, 16- and
-    32-bit partner lists) and grid with ordinary 100-iteration launches; the
+    """The autotuner times every layout the pool fits (class masks on lane
+    pairs or one lane per walker, 16- and 32-bit partner lists) and grid with ordinary 100-iteration launches; the
     walk then continues in the chosen layout, bit-identical to the oracle
     replaying all those launches."""
     seed = 9
@@ -76,7 +94,7 @@ def test_autotune_keeps_results_exact(n, W, use_graph):
         layout, trials = e.layout_info()
         assert g >= 1 and len(trials) >= 1
         assert layout in set(int(x) for x in trials[:, 0])
-        expect_layouts = {0, 2} | ({3} if n[0] * n[1] * n[2] <= 32 else set())
+        expect_layouts = {0, 2} | ({3, 4} if n[0] * n[1] * n[2] <= 32 else set())
         assert set(int(x) for x in trials[:, 0]) == expect_layouts
         # each trial launched its own grid: every (layout, grid) pair is distinct
         assert len({(int(a), int(b)) for a, b, _ in trials}) == len(trials)

@@ -38,7 +38,7 @@ def test_emulated_kernel_plus_expand_paths(n):
 
 # schemes of up to 32 terms run the class-mask lanes by default; these run
 # the same cases on the partner-list lanes (16- and 32-bit factor words) too
-@pytest.mark.parametrize("layout", ["masks", "half", "wide"])
+@pytest.mark.parametrize("layout", ["masks", "half", "wide", "solo"])
 @pytest.mark.parametrize("n", [(2, 2, 2), (2, 3, 4), (3, 3, 3)])
 def test_emulated_layouts_match_oracle(n, layout):
     _check(n, 64, 2000, seed=17, launches=2, plus=150, p_expand=200.0, layout=layout)
@@ -48,14 +48,14 @@ def test_emulated_kernel_random_iterations():
     _check((3, 3, 3), 64, 1200, seed=99, launches=3, random_iterations=True, p_expand=20.0)
 
 
-@pytest.mark.parametrize("layout", ["masks", "half"])
+@pytest.mark.parametrize("layout", ["masks", "half", "solo"])
 def test_emulated_kernel_without_reductions(layout):
     # complexity-minimizer mode: tryFlip(checkReduce = false)
     ora = _check((3, 3, 3), 64, 2000, seed=3, check_reduce=False, layout=layout)
     assert (ora["m"] >= 27).all() or (ora["m"] < 27).any()
 
 
-@pytest.mark.parametrize("layout", ["masks", "half"])
+@pytest.mark.parametrize("layout", ["masks", "half", "solo"])
 def test_emulated_kernel_reduce_probability(layout):
     _check((3, 3, 3), 64, 1500, seed=11, p_reduce=300.0, plus=200, p_expand=50.0, layout=layout)
 
@@ -85,11 +85,11 @@ def test_emulated_wide_words_match_oracle(n):
     assert first_mismatch(emu, ora) is None, first_mismatch(emu, ora)
 
 
-@pytest.mark.parametrize("layout", ["masks", "half", "wide"])
+@pytest.mark.parametrize("layout", ["masks", "half", "wide", "solo"])
 @pytest.mark.parametrize("n", [(3, 3, 3), (4, 4, 4)])
 def test_emulated_lookahead_matches_oracle(n, layout):
     # reduction-seeking lookahead proposals every 2nd iteration, 3 candidates
-    if n == (4, 4, 4) and layout == "masks":
+    if n == (4, 4, 4) and layout in ("masks", "solo"):
         pytest.skip("class masks hold at most 32 terms")
     start = O.naive_records(*n, 32)
     p = O.make_params(iterations=1500, plus_iterations=300, p_expand=50.0, look_period=2, look_candidates=3)

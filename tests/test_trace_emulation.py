@@ -21,6 +21,7 @@ CASES = [
     ((2, 2, 2), O.RING_ZT, "auto"), ((3, 3, 3), O.RING_ZT, "auto"),
     ("strassen", O.RING_ZT, "auto"), ("strassen", O.RING_Z2, "masks"),
     ("strassen_squared", O.RING_ZT, "auto"), ("strassen_squared", O.RING_Z2, "half"),
+    ((3, 3, 3), O.RING_Z2, "solo"), ("strassen", O.RING_Z2, "solo"),
 ]
 
 

@@ -25,6 +25,7 @@ CASES = [
     ((3, 3, 3), O.RING_ZT, 0, 0), ((4, 4, 4), O.RING_ZT, 0, 0),
     ("strassen", O.RING_ZT, 0, 0), ("strassen", O.RING_Z2, 2, 0),
     ("strassen_squared", O.RING_ZT, 0, 0), ("strassen_squared", O.RING_Z2, 0, 16),
+    ((3, 3, 3), O.RING_Z2, 3, 0), ("strassen", O.RING_Z2, 3, 0),
 ]
 
 

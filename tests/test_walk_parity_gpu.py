@@ -62,7 +62,7 @@ def test_walk_16bit_words_bit_identical(n):
 
 
 @pytest.mark.parametrize("n", [(2, 2, 2), (3, 3, 3), (2, 3, 4)])
-@pytest.mark.parametrize("layout,word_bits", [(1, 0), (1, 32), (2, 0)])
+@pytest.mark.parametrize("layout,word_bits", [(1, 0), (1, 32), (2, 0), (3, 0)])
 def test_walk_partner_layouts_bit_identical(n, layout, word_bits):
     # schemes of up to 32 terms: class masks (the default) and partner lists
     # with 16- or 32-bit words give the same trajectories
@@ -71,7 +71,7 @@ def test_walk_partner_layouts_bit_identical(n, layout, word_bits):
     assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
 
 
-@pytest.mark.parametrize("n,layout", [((3, 3, 3), 2), ((3, 3, 3), 1), ((4, 4, 4), 0)])
+@pytest.mark.parametrize("n,layout", [((3, 3, 3), 2), ((3, 3, 3), 1), ((4, 4, 4), 0), ((3, 3, 3), 3)])
 def test_walk_reduce_probability_bit_identical(n, layout):
     # --reduce-probability > 0: tryReduce after flips (flip_graph.cu:550-551)
     gpu, ora = _run_both(n, 128, 2000, seed=66, launches=2, plus=200, p_expand=50.0, p_reduce=400.0,
@@ -80,7 +80,7 @@ def test_walk_reduce_probability_bit_identical(n, layout):
 
 
 @pytest.mark.parametrize("n,layout,look", [((3, 3, 3), 0, (2, 3)), ((3, 3, 3), 1, (1, 2)), ((4, 4, 4), 0, (2, 3)),
-                                           ((2, 3, 4), 0, (4, 8))])
+                                           ((2, 3, 4), 0, (4, 8)), ((3, 3, 3), 3, (2, 3))])
 def test_walk_lookahead_bit_identical(n, layout, look):
     # reduction-seeking lookahead proposals (period, candidates)
     gpu, ora = _run_both(n, 128, 2000, seed=81, launches=2, plus=200, p_expand=50.0, partner_layout=layout, look=look)
@@ -94,10 +94,21 @@ def test_walk_5x5x5_bit_identical():
     assert (ora["total_flips"] > 0).all()
 
 
-def test_random_iterations_bit_identical():
-    gpu, ora = _run_both((3, 3, 3), 96, 1500, seed=99, launches=3, random_iterations=True, p_expand=20.0)
+@pytest.mark.parametrize("layout", [0, 3])
+def test_random_iterations_bit_identical(layout):
+    # 96 walkers are three queue blocks of 32: the one-lane layout's second
+    # wave serves one block and leaves its upper half-wave idle
+    gpu, ora = _run_both((3, 3, 3), 96, 1500, seed=99, launches=3, random_iterations=True, p_expand=20.0,
+                         partner_layout=layout)
     assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
     assert len(set(int(x) for x in ora["it"])) > 10  # iteration counts really differ per walker
+
+
+@pytest.mark.parametrize("walkers", [1, 33, 64, 200])
+def test_one_lane_layout_ragged_pools(walkers):
+    # pools that end inside a block and inside a wave of the one-lane layout
+    gpu, ora = _run_both((2, 3, 3), walkers, 1200, seed=404, launches=2, plus=100, p_expand=80.0, partner_layout=3)
+    assert first_mismatch(gpu, ora) is None, first_mismatch(gpu, ora)
 
 
 @pytest.mark.parametrize("n,walkers,iterations", [((2, 2, 2), 128, 2500), ((3, 3, 3), 128, 3000),
