@@ -66,6 +66,7 @@ struct fg_scorer {
     long capF = 0;  // candidate tuples the fused buffers hold
     long capL = 0;  // proposing walkers fmap holds
     int lastB = 0, lastK = 0;
+    bool last_fused = false;  // the last round's candidates sit in the fused buffers (else: map / pairs / s)
     // HIP events around the fused kernel of the last round, and its duration
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;
@@ -523,13 +524,14 @@ int propose_fused(fg_scorer *s, const fgrt::PoolView &v, int candidates, int eve
     SCHK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
     s->lastB = L;
     s->lastK = candidates;
+    s->last_fused = true;
     return 0;
 }
 }  // namespace
 
 extern "C" int fg_scorer_last_propose_ms(fg_scorer *s, float *ms) {
     if (!s || !ms) return sfail("fg_scorer_last_propose_ms: bad arguments");
-    *ms = s->lastB ? s->last_ms : 0.0f;
+    *ms = s->lastB && s->last_fused ? s->last_ms : 0.0f;
     return 0;
 }
 
@@ -538,9 +540,12 @@ extern "C" int fg_scorer_last_proposals(fg_scorer *s, int32_t *walkers, int32_t 
     const long n = (long)s->lastB * s->lastK;
     if (n == 0) return 0;
     if (cap < n) return sfail("fg_scorer_last_proposals: buffer too small");
-    if (walkers) SCHK(hipMemcpy(walkers, s->fmap, (size_t)s->lastB * 4, hipMemcpyDeviceToHost));
-    if (pairs) SCHK(hipMemcpy(pairs, s->fpairs, (size_t)n * 12, hipMemcpyDeviceToHost));
-    if (scores) SCHK(hipMemcpy(scores, s->fscores, (size_t)n * 4, hipMemcpyDeviceToHost));
+    const int32_t *wsrc = s->last_fused ? s->fmap : s->map;
+    const int32_t *psrc = s->last_fused ? s->fpairs : s->pairs;
+    const float *ssrc = s->last_fused ? s->fscores : s->s;
+    if (walkers) SCHK(hipMemcpy(walkers, wsrc, (size_t)s->lastB * 4, hipMemcpyDeviceToHost));
+    if (pairs) SCHK(hipMemcpy(pairs, psrc, (size_t)n * 12, hipMemcpyDeviceToHost));
+    if (scores) SCHK(hipMemcpy(scores, ssrc, (size_t)n * 4, hipMemcpyDeviceToHost));
     return (int)s->lastB;
 }
 
@@ -577,6 +582,10 @@ extern "C" int fg_scorer_propose(fg_scorer *s, fg_ctx *ctx, int candidates, int 
         if (featurize(s, v.cur, v.m, s->map, v.W, B) || forward(s, B, B * candidates)) return -1;
         SCHK(fgk::sc::launch_choose(s->s, s->pairs, s->map, B, candidates, v.forced, s->stream));
         SCHK(hipStreamSynchronize(s->stream));
+        // the last chunk's candidates and scores stay readable (fg_scorer_last_proposals)
+        s->lastB = B;
+        s->lastK = candidates;
+        s->last_fused = false;
     }
     return 0;
 }

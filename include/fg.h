@@ -320,11 +320,13 @@ FG_API int fg_scorer_train(fg_scorer *s, fg_ctx *ctx, int steps, int batch, floa
 /* For walkers w with w % every == 0: draw `candidates` equal-factor pairs
  * uniformly, score them and propose the best (fg_set_forced's layout). */
 FG_API int fg_scorer_propose(fg_scorer *s, fg_ctx *ctx, int candidates, int every, uint32_t round);
-/* Candidates and scores of the last proposal round (fused rounds only:
- * schemes of up to 64 terms and 96 factor bits, at most 16 candidates):
- * walkers [rounds' tuples], pairs [tuples][candidates][3], scores
- * [tuples][candidates] (-1e30 for a missing candidate).  Returns the number of
- * tuples (0 when the last round was not fused); any output may be null. */
+/* Candidates and scores of the last proposal round: walkers [tuples], pairs
+ * [tuples][candidates][3] (component -1 for a missing candidate), scores
+ * [tuples][candidates].  A fused round (schemes of up to 64 terms and 96
+ * factor bits, at most 16 candidates) reports every proposing walker (a
+ * missing candidate scores -1e30); a chunked round (larger schemes) reports
+ * its last chunk of at most 4096 walkers.  Returns the number of tuples (0
+ * before any round); any output may be null. */
 FG_API int fg_scorer_last_proposals(fg_scorer *s, int32_t *walkers, int32_t *pairs, float *scores, int64_t cap);
 /* Duration in ms of the fused proposal kernel of the last round (HIP events on
  * the scorer's stream; 0 when the last round was not fused).  No counterpart in

@@ -17,7 +17,7 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12))
 
 
-@pytest.mark.parametrize("count,M,n", [(70, 64, (4, 4, 4)), (37, 27, (3, 3, 3))])
+@pytest.mark.parametrize("count,M,n", [(70, 64, (4, 4, 4)), (37, 27, (3, 3, 3)), (24, 125, (5, 5, 5))])
 def test_scorer_forward_backward_match_oracle(count, M, n):
     print(f"[scorer] eval count={count} M={M} n={n}", flush=True)
     nn = (n[0] * n[1], n[1] * n[2], n[2] * n[0])
@@ -181,16 +181,18 @@ def test_trained_network_beats_untrained_on_heldout():
     assert auc_t >= 0.54 and auc_t >= auc_u + 0.03
 
 
-def test_training_and_proposals():
-    print("[learned] train + propose 4x4x4", flush=True)
-    W = 4096
-    with fg.Engine(W, 3, max_iterations=200, plus_iterations=2000) as e, fg.Scorer(4, 4, 4, 64, seed=2) as sc:
-        e.init_naive(4, 4, 4)
+@pytest.mark.parametrize("n,M,W", [((4, 4, 4), 64, 4096), ((5, 5, 5), 125, 2048)])
+def test_training_and_proposals(n, M, W):
+    # 5x5x5: 25-coefficient factors in 32-bit words, 125 terms, the chunked
+    # proposal path
+    print(f"[learned] train + propose n={n}", flush=True)
+    with fg.Engine(W, 3, max_iterations=200, plus_iterations=2000) as e, fg.Scorer(*n, M, seed=2) as sc:
+        e.init_naive(*n)
         e.log_enable(max_batches=16, every=1, horizon=4)
         for _ in range(12):
             assert e.walk(200) == 0
         info = e.log_info()
-        assert info["rows"] > 20000 and 0 < info["positives"] < info["rows"]
+        assert info["rows"] >= 8 * W and 0 < info["positives"] < info["rows"]
         losses = sc.train(e, steps=150, batch=512, lr=2e-3, seed=5)
         assert np.isfinite(losses).all()
         assert losses[-20:].mean() < losses[:10].mean()
@@ -209,12 +211,13 @@ def test_training_and_proposals():
         assert bad == 0
 
 
-@pytest.mark.parametrize("n,M", [((4, 4, 4), 64), ((3, 3, 3), 27)])
+@pytest.mark.parametrize("n,M", [((4, 4, 4), 64), ((3, 3, 3), 27), ((5, 5, 5), 125)])
 def test_fused_proposals_match_oracle_scores(n, M):
-    # the fused proposal kernel (whole network in one workgroup per walker)
-    # scores every candidate as the float64 oracle does at the same bf16
-    # rounding points, and proposes the first best-scoring candidate
-    print(f"[learned] fused proposals n={n}", flush=True)
+    # the fused proposal kernel (whole network in one workgroup per walker;
+    # schemes of up to 64 terms) or the chunked path (5x5x5: 125 terms, 75
+    # factor bits) scores every candidate as the float64 oracle does at the
+    # same bf16 rounding points, and proposes the first best-scoring candidate
+    print(f"[learned] proposals n={n}", flush=True)
     W, K, every = 512, 8, 2
     nn = (n[0] * n[1], n[1] * n[2], n[2] * n[0])
     with fg.Engine(W, 9, max_iterations=300, plus_iterations=300, max_rank=M, word_bits=32) as e, \
