@@ -152,7 +152,16 @@ struct LaneBaseT {
     // Address of row t of component c in a walker's F column based at Fb.
     FGK_HD static Word *fat(Word *Fb, int fs, int c, int t) {
         if constexpr (kPacked) return Fb + (c * kPackedPairs + 15 - (t & 15)) * (2 * kWS) + (t >> 4);
-        else return Fb + (c * fs + t) * kWS;
+        else return Fb + (mul24(c, fs) + t) * kWS;
+    }
+    // c * x for the small non-negative plane indices (one full-rate
+    // v_mul_u32_u24 instead of a quarter-rate 32-bit multiply).
+    FGK_HD static int mul24(int c, int x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return (int)__umul24((unsigned)c, (unsigned)x);
+#else
+        return c * x;
+#endif
     }
     FGK_HD static const Word *fat(const Word *Fb, int fs, int c, int t) { return fat(const_cast<Word *>(Fb), fs, c, t); }
     Word *F;      // walker-slot-offset plane bases
@@ -232,11 +241,17 @@ struct LaneBaseT {
     FGK_HD uint32_t fpair(int c, int p) const {
         return reinterpret_cast<const uint32_t *>(F)[(c * kPackedPairs + p) * kWS];
     }
+    // Packed layout: component c's column, and the offset of row t in it
+    // (in words), so fcol(c)[packed_off(t)] is f(c, t).
+    FGK_HD const Word *fcol(int c) const { return F + c * kPackedPairs * 2 * kWS; }
+    FGK_HD static int packed_off(int t) { return (15 - (t & 15)) * (2 * kWS) + (t >> 4); }
     // H packs terms 4q..4q+3 of a walker into one dword: byte (t & 3) of word q.
-    FGK_HD uint8_t &h(int c, int t) { return H[((c * hw + (t >> 2)) * kWS) * 4 + (t & 3)]; }
-    FGK_HD uint32_t &hword(int c, int q) { return reinterpret_cast<uint32_t *>(H)[(c * hw + q) * kWS]; }
-    FGK_HD uint8_t &nx(int c, int t) { return N[((c * hw + (t >> 2)) * kWS) * 4 + (t & 3)]; }
-    FGK_HD uint32_t &cm(int c, int t) { return CM[(c * cs + t) * kWS]; }
+    // H words per component (a constant 8 in the one-lane layout, schemes of at most 32 terms)
+    FGK_HD int hwc() const { return kPacked ? 8 : hw; }
+    FGK_HD uint8_t &h(int c, int t) { return H[((mul24(c, hwc()) + (t >> 2)) * kWS) * 4 + (t & 3)]; }
+    FGK_HD uint32_t &hword(int c, int q) { return reinterpret_cast<uint32_t *>(H)[(mul24(c, hwc()) + q) * kWS]; }
+    FGK_HD uint8_t &nx(int c, int t) { return N[((mul24(c, hw) + (t >> 2)) * kWS) * 4 + (t & 3)]; }
+    FGK_HD uint32_t &cm(int c, int t) { return CM[(mul24(c, cs) + t) * kWS]; }
     // H[c][t] += d (d = +-1) as one LDS add on t's dword that returns nothing,
     // so the wave does not wait for it (the byte never wraps: H counts are
     // below 256 and a decremented count is at least 1).
@@ -551,7 +566,12 @@ struct LaneBaseT {
         r -= below;
         const uint32_t c0 = word & 0xFFu, c1 = c0 + ((word >> 8) & 0xFFu), c2 = c1 + ((word >> 16) & 0xFFu);
         const int b = (r >= c0 ? 1 : 0) + (r >= c1 ? 1 : 0) + (r >= c2 ? 1 : 0);
-        r -= b == 0 ? 0u : (b == 1 ? c0 : (b == 2 ? c1 : c2));
+        // the running sums c0 <= c1 <= c2 below r: the last one is subtracted
+        // (selects, not a branch on b)
+        uint32_t sub = r >= c0 ? c0 : 0u;
+        sub = r >= c1 ? c1 : sub;
+        sub = r >= c2 ? c2 : sub;
+        r -= sub;
         t1 = idx * 4 + b;
     }
 
@@ -640,6 +660,8 @@ struct LaneBaseT {
         if (t != m) copy_term(t, m);
     }
 };
+
+FGK_HD uint32_t min_u32(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
 // x in a vector register (an opaque copy on the GPU), so values derived
 // from it stay in vector registers too.
@@ -985,12 +1007,22 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     // and swap them, so each lane runs one Philox per two iterations.
     uint32_t nb0 = 0, nb1 = 0, nb2 = 0, nb3 = 0;
     bool have_next = false;
+    // one-lane layout, plain and trace walks: every iteration's block is
+    // computed during the previous flip (the first one here)
+    constexpr bool kPrefetch = Lane::kPacked && (kMode == kModeWalk || kMode == kModeTrace);
+    if constexpr (kPrefetch)
+        philox10((uint32_t)it, (uint32_t)(it >> 32), (uint32_t)w, 0u, s.k0, s.k1, nb0, nb1, nb2, nb3);
     s.mark_start();
     for (int iter = 0; iter < trips; ++iter) {
         bool flipped = false, copy = false;
         uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
         if (iter < iters) {
-            if constexpr (!Lane::kSplit) {
+            if constexpr (kPrefetch) {
+                b0 = nb0;
+                b1 = nb1;
+                b2 = nb2;
+                b3 = nb3;
+            } else if constexpr (!Lane::kSplit) {
                 // one-lane layout: the block may have been computed during
                 // the previous iteration's flip
                 if (have_next) {
@@ -1050,49 +1082,49 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
             } else if constexpr (kMode == kModeLook) {
                 if (iter % A.look_period == A.look_period - 1) ok = s.try_lookahead(A.look_candidates, check_reduce);
                 if (!ok) ok = s.try_flip(b0, b1, check_reduce);
-            } else if constexpr (Lane::kPacked) {
+            } else if constexpr (kPrefetch) {
                 // the next iteration's block (the counter is already advanced)
                 // fills the flip's first wait; the block is used only if the
                 // launch goes on
                 ok = s.try_flip(b0, b1, check_reduce, [&]() {
                     philox10((uint32_t)it, (uint32_t)(it >> 32), (uint32_t)w, 0u, s.k0, s.k1, nb0, nb1, nb2, nb3);
                     Lane::keep_loaded(nb0, nb1, nb2, nb3);  // computed here, not sunk to its use
-                    have_next = true;
                 });
             } else {
                 ok = s.try_flip(b0, b1, check_reduce);
             }
-            if (!ok) {  // flip_graph.cu:535-537
-                s.try_expand(1 + (int)(b1 % 2u));
-            } else {
-                flipped = true;
-                ++total_flips;
-                sum_m += (uint64_t)rank;
-                if (s.m < rank) flips_count = 0;
-                ++flips_count;
-                if (s.m < best_rank || (s.m == best_rank && b2 < thr_best_eq)) {  // :545-548
-                    best_rank = s.m;
-                    best_m = s.m;
-                    copy = true;
-                }
-            }
+            if (!ok) s.try_expand(1 + (int)(b1 % 2u));  // flip_graph.cu:535-537
+            // the bookkeeping of a flip, written as selects (no branches)
+            flipped = ok;
+            total_flips += ok ? 1u : 0u;
+            sum_m += ok ? (uint64_t)rank : 0u;
+            flips_count = ok ? (s.m < rank ? 0 : flips_count) + 1 : flips_count;
+            copy = ok && (s.m < best_rank || (s.m == best_rank && b2 < thr_best_eq));  // :545-548
+            best_rank = copy ? s.m : best_rank;
+            best_m = copy ? s.m : best_m;
         }
         copy_best(s, A, copy, lane, w);
         s.mark(6);
-        if (flipped) {
-            if (A.use_reduce && s.aux32() < thr_reduce) {  // :550-551
+        // the switches are tested first, so a walk without these edges
+        // branches on uniform values only
+        if (A.use_reduce) {
+            if (flipped && s.aux32() < thr_reduce) {  // :550-551
                 s.rec(6, 0, 0, 0, 0, 0, 0, 0);
                 s.try_reduce();
             }
-            if (A.use_expand && b3 < thr_expand && s.m <= best_m + 2) {   // :553-556
+        }
+        if (A.use_expand) {
+            if (flipped && b3 < thr_expand && s.m <= best_m + 2) {  // :553-556
                 if (s.try_expand(1 + (int)(s.aux32() % 2u))) flips_count = 0;
             }
-            if constexpr (kMode == kModeMeta) {
+        }
+        if constexpr (kMode == kModeMeta) {
+            if (flipped) {
                 if (A.use_sandwich && s.aux32() < A.thr_sandwich) s.sandwiching();  // :558-559
                 if (A.use_basis && s.aux32() < A.thr_basis) rz_swap_basis(s);       // :561-562
             }
-            if (flips_count >= plus_iterations && s.try_plus()) flips_count = 0;  // :564-565
         }
+        if (flipped && flips_count >= plus_iterations && s.try_plus()) flips_count = 0;  // :564-565
         if (A.min_window <= 0 || iter < A.min_window) min_m = s.m < min_m ? s.m : min_m;
         s.mark(7);
     }

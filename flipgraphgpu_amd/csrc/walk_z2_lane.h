@@ -561,14 +561,13 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
         pre();
         int t1;
         B::select_words(hw8, rr, t1);
-        if (t1 < 0 || t1 >= m) {
-            this->fail(kErrSelect);
-            return false;
-        }
-        // the (rr + 1)-th member above t1
-        uint32_t q = cm(c, t1) & ~((2u << t1) - 1u);
-        for (uint32_t e = 0; e < rr; ++e) q &= q - 1u;
-        if (!q) {
+        // the (rr + 1)-th member above t1 (t1 clamped into the plane for the
+        // read; an out-of-range t1 or an empty q is reported below)
+        const int t1c = t1 < m ? t1 : 0;
+        const uint32_t cmt1 = cm(c, t1c);
+        uint32_t q = cmt1 & ~((2u << t1c) - 1u);
+        for (uint32_t e = 0; e < rr && q; ++e) q &= q - 1u;
+        if (t1 >= m || !q) {
             this->fail(kErrSelect);
             return false;
         }
@@ -578,16 +577,25 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
         sel_t1 = t1;
         sel_t2 = t2;
         const int i = c;
-        const int i1 = (coin & 2u) ? t2 : t1, i2 = (coin & 2u) ? t1 : t2;
+        const bool sw = (coin & 2u) != 0;
+        const int i1 = sw ? t2 : t1, i2 = sw ? t1 : t2;
         this->rec(0, i, j, k, i1, i2, check_reduce ? 1 : 0, 0);
-        const uint32_t fj1 = f(j, i1), fj2 = f(j, i2), fk1 = f(k, i1), fk2 = f(k, i2);
-        const uint32_t oldj = cm(j, i1), ci1 = cm(i, i1), ck1 = cm(k, i1);
-        const uint32_t oldk = cm(k, i2), ci2 = cm(i, i2), cj2 = cm(j, i2);
-        // one round trip for all ten reads: the masks are read before the
+        // the reads of index1 and index2: their factor offsets in a packed
+        // component column and their class-mask rows, computed once
+        const int o1 = B::packed_off(i1), o2 = B::packed_off(i2);
+        const Word *Fj = this->fcol(j), *Fk = this->fcol(k);
+        const uint32_t fj1 = Fj[o1], fj2 = Fj[o2], fk1 = Fk[o1], fk2 = Fk[o2];
+        const uint32_t *Cj = &cm(j, 0), *Ck = &cm(k, 0);
+        const uint32_t oldj = Cj[i1 * B::kWS], ck1 = Ck[i1 * B::kWS];
+        const uint32_t oldk = Ck[i2 * B::kWS], cj2 = Cj[i2 * B::kWS];
+        // index1 and index2 share their component-i class: t1's mask
+        const uint32_t ci1 = cmt1, ci2 = cmt1;
+        // one round trip for all eight reads: the masks are read before the
         // zero-factor test, not after it
-        keep_loaded(fj1, fj2, fk1, fk2, oldj, ci1, ck1, oldk, ci2, cj2);
+        keep_loaded(fj1, fj2, fk1, fk2, oldj, ck1, oldk, cj2);
         const uint32_t A = fj1 ^ fj2, Bv = fk2 ^ fk1;
-        if (!(fj1 && fj2 && fk1 && fk2 && A && Bv)) {
+        const uint32_t low = min_u32(min_u32(min_u32(fj1, fj2), min_u32(fk1, fk2)), min_u32(A, Bv));
+        if (low == 0u) {
             flip_literal(i, j, k, i1, i2, check_reduce);
             this->mark(9);
             return true;
