@@ -60,6 +60,7 @@ def parse():
     tt = ap.add_argument_group("time to target (learned vs uniform proposals, 4x4x4 Z2)")
     tt.add_argument("--ttt", action="store_true", help="run the time-to-target study instead of the throughput bench")
     tt.add_argument("--seeds", type=int, default=10)
+    tt.add_argument("--ttt-n", type=int, default=4, help="cube size n of the n x n x n Z2 format (4: rank 64 -> 49, 5: rank 125 -> 100)")
     tt.add_argument("--ttt-walkers", type=int, default=131072)
     tt.add_argument("--ttt-launch", type=int, default=1000, help="walk iterations per launch (one proposal round each)")
     tt.add_argument("--ttt-target", type=int, default=49)
@@ -180,14 +181,14 @@ def auc(scores, labels):
     return float((r[pos].sum() - P * (P + 1) / 2) / (P * N))
 
 
-def descent_log(seed, W, L, warm, launches, plus):
-    """A uniform 4x4x4 walk's trajectory log: `warm` launches of descent from the
-    naive scheme first, then `launches` logged launches, each walker's first
+def descent_log(seed, W, L, warm, launches, plus, n=4):
+    """A uniform n x n x n walk's trajectory log: `warm` launches of descent from
+    the naive scheme first, then `launches` logged launches, each walker's first
     flip labelled 1 when its rank falls within that launch."""
     import flipgraphgpu_amd as fg
 
     e = fg.Engine(W, seed, max_iterations=L, plus_iterations=plus, expand_probability=0.0)
-    e.init_naive(4, 4, 4)
+    e.init_naive(n, n, n)
     for _ in range(warm):
         if e.walk(L):
             raise RuntimeError("invalid schemes while logging")
@@ -206,11 +207,12 @@ def heldout_ranking(a, sc, L):
 
     import flipgraphgpu_amd as fg
 
-    e = descent_log(2000, 4096, L, a.log_warm, 4, a.ttt_plus)
+    c, M = a.ttt_n, a.ttt_n ** 3
+    e = descent_log(2000, 4096, L, a.log_warm, 4, a.ttt_plus, c)
     n = e.log_info()["rows"]
-    words, m, pairs, y = e.log_read(0, n, 64)
+    words, m, pairs, y = e.log_read(0, n, M)
     e.close()
-    sc0 = fg.Scorer(4, 4, 4, 64, seed=7)
+    sc0 = fg.Scorer(c, c, c, M, seed=7)
 
     def scores(s):
         out = []
@@ -225,7 +227,7 @@ def heldout_ranking(a, sc, L):
 
 
 def time_to_target(a, seeds=None, train_steps=None, emit=True):
-    """Median wall time over seeds to a verified n=4 Z2 scheme of rank <= target,
+    """Median wall time over seeds to a verified n x n x n Z2 scheme (n = --ttt-n) of rank <= target,
     uniform walks vs learned + uniform proposals.  The scorer is trained once,
     on trajectories logged by uniform walks of the same shape (seed 1000);
     its training time is reported separately.  A run ends at the first launch
@@ -239,15 +241,16 @@ def time_to_target(a, seeds=None, train_steps=None, emit=True):
     import flipgraphgpu_amd as fg
 
     W, L = a.ttt_walkers, a.ttt_launch
+    nc, M = a.ttt_n, a.ttt_n ** 3
 
     def progress(msg):
         print(msg, file=sys.stderr, flush=True)
 
     def run(seed, scorer):
         e = fg.Engine(W, seed, max_iterations=L, plus_iterations=a.ttt_plus, expand_probability=0.0)
-        e.init_naive(4, 4, 4)
+        e.init_naive(nc, nc, nc)
         t0 = time.perf_counter()
-        launches, best = 0, 64
+        launches, best = 0, M
         while time.perf_counter() - t0 < a.ttt_timeout:
             if scorer is None:
                 bad = e.walk(L)
@@ -268,11 +271,11 @@ def time_to_target(a, seeds=None, train_steps=None, emit=True):
     # training data: uniform walks with the trajectory log, from the descent
     # phase (after log_warm launches), labelled by the flip's own launch
     t0 = time.perf_counter()
-    e = descent_log(1000, a.log_walkers, L, a.log_warm, a.log_launches, a.ttt_plus)
+    e = descent_log(1000, a.log_walkers, L, a.log_warm, a.log_launches, a.ttt_plus, nc)
     info = e.log_info()
     t_log = time.perf_counter() - t0
     progress(f"[ttt] logged {info['rows']} flips ({info['positives']} positive) in {t_log:.1f} s")
-    sc = fg.Scorer(4, 4, 4, 64, seed=7)
+    sc = fg.Scorer(nc, nc, nc, M, seed=7)
     t0 = time.perf_counter()
     losses = sc.train(e, steps=train_steps, batch=a.train_batch, lr=a.lr, seed=11)
     t_train = time.perf_counter() - t0
@@ -295,11 +298,11 @@ def time_to_target(a, seeds=None, train_steps=None, emit=True):
 
     mu, ml = med("uniform"), med("learned")
     out = {
-        "metric": f"time to verified 4x4x4 Z2 rank-{a.ttt_target} (median over {seeds} seeds)",
+        "metric": f"time to verified {nc}x{nc}x{nc} Z2 rank-{a.ttt_target} (median over {seeds} seeds)",
         "value": ml, "unit": "s", "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
         "uniform_median_s": mu, "learned_median_s": ml,
         "speedup_learned_vs_uniform": (mu / ml) if ml > 0 else None,
-        "config": {"workload": f"4x4x4 Z2 from the naive rank-64 scheme, {W} walkers, {L} iterations per launch, "
+        "config": {"workload": f"{nc}x{nc}x{nc} Z2 from the naive rank-{M} scheme, {W} walkers, {L} iterations per launch, "
                                f"plus-iterations {a.ttt_plus}; learned arm: every {a.learned_every}th walker's first "
                                f"flip of each launch is the scorer's best of {a.candidates} uniform candidates",
                    "walkers": W, "launch_iterations": L},
