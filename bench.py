@@ -553,6 +553,7 @@ def workload_descent(a):
                       expand_probability=a.expand_probability)
         e.init_naive(4, 4, 4)
         c0 = e.counters()
+        cw = c0
         t0 = time.perf_counter()
         traj = []
         for q in range(launches):
@@ -560,8 +561,15 @@ def workload_descent(a):
                 raise RuntimeError("invalid schemes")
             if (q + 1) % 20 == 0:
                 s = e.summary()
+                c = e.counters()
+                # share of iterations whose tryFlip found a flip (the rest had
+                # no admissible candidate and took the expand fallback)
+                rate = (c["flips"] - cw["flips"]) / max(1, c["iterations"] - cw["iterations"])
+                cw = c
+                ranks, counts = np.unique(s["m"], return_counts=True)
                 traj.append({"launches": q + 1, "best_rank": int(s["best_m"].min()),
-                             "mean_rank": round(float(np.mean(s["m"])), 4)})
+                             "mean_rank": round(float(np.mean(s["m"])), 4), "flip_success": round(float(rate), 4),
+                             "rank_histogram": {int(r): int(k) for r, k in zip(ranks, counts) if k >= 64}})
                 print(f"[descent] {name} {traj[-1]}", file=sys.stderr, flush=True)
         dt = time.perf_counter() - t0
         out["rings"][name] = {"seconds": round(dt, 2), "flips_per_s": (e.counters()["flips"] - c0["flips"]) / dt,
