@@ -752,20 +752,33 @@ FGK_HD void copy_best(Lane &s, const WalkArgs &A, bool want, int lane, int w) {
         const int src = __builtin_ctzll(mask);
         mask &= mask - 1;
         const int ms = __builtin_amdgcn_readlane(s.m, src);
-        const int f0 = __builtin_amdgcn_readlane(s.n0, src), f1 = __builtin_amdgcn_readlane(s.n1, src),
-                  f2 = __builtin_amdgcn_readlane(s.n2, src);
         // the source lane's walker (the one-lane layout takes two queue
         // items per wave, so the walkers of a wave need not be consecutive)
         const int ws = __builtin_amdgcn_readlane(w, src);
         const int slot = lane % Lane::kWS, src_slot = src % Lane::kWS;
-        if (A.bfmt && lane == 0) {
-            int32_t *bf = A.bfmt + 4 * (size_t)ws;
-            bf[0] = f0;
-            bf[1] = f1;
-            bf[2] = f2;
+        if (A.bfmt) {
+            const int f0 = __builtin_amdgcn_readlane(s.n0, src), f1 = __builtin_amdgcn_readlane(s.n1, src),
+                      f2 = __builtin_amdgcn_readlane(s.n2, src);
+            if (lane == 0) {
+                int32_t *bf = A.bfmt + 4 * (size_t)ws;
+                bf[0] = f0;
+                bf[1] = f1;
+                bf[2] = f2;
+            }
         }
         const typename Lane::Word *Fs = s.F - Lane::fslot(slot) + Lane::fslot(src_slot);
         uint32_t *dst = A.best + (size_t)ws * 3 * A.max_m;
+        if constexpr (Lane::kPacked) {
+            // at most 32 terms: lane t copies row t of the three components
+            if (lane < ms) {
+                const int o = Lane::packed_off(lane);
+                const uint32_t x0 = Fs[o], x1 = Fs[o + 32 * Lane::kWS], x2 = Fs[o + 64 * Lane::kWS];
+                dst[lane] = x0;
+                dst[A.max_m + lane] = x1;
+                dst[2 * A.max_m + lane] = x2;
+            }
+            continue;
+        }
         // the three components' reads go out together: one LDS round trip per pass
         for (int t = lane; t < ms; t += kThreadsPerGroup) {
             const uint32_t x0 = *Lane::fat(Fs, s.fs, 0, t), x1 = *Lane::fat(Fs, s.fs, 1, t),

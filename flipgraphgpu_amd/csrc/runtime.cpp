@@ -83,6 +83,7 @@ struct fg_ctx {
     bool minimize = false;  // pool set up by fg_init_minimizer
     bool flips = false;     // pool set up by fg_init_flips
     int variant = 0;        // walk kernel variant (kernels.h: kVarZ2 / kVarZT / kVarZ2h / kVarZ2m / kVarZ2s)
+    bool bfmt_same = true;  // every walker's best scheme has the walker's current format
     int nn_max = 0;         // widest factor of the pool's formats
     bool masks_fit = false; // the class-mask layout can hold the pool
     bool meta = false;      // sandwiching / basis edges configured
@@ -443,6 +444,7 @@ int init_pool(fg_ctx *ctx, const std::vector<fg::Scheme> &starts, const std::vec
     HIPCHK(hipMemcpy(ctx->aux_idx, aidx.data(), W * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->fmt, fmt.data(), (size_t)16 * W, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->bfmt, fmt.data(), (size_t)16 * W, hipMemcpyHostToDevice));
+    ctx->bfmt_same = true;
     HIPCHK(hipMemcpy(ctx->it, z64.data(), W * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->aux, z64.data(), W * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->total_flips, z64.data(), W * 8, hipMemcpyHostToDevice));
@@ -777,7 +779,14 @@ int walk_launch(fg_ctx *ctx, int iterations, bool use_forced, double resize) {
     a.thr_sandwich = threshold(ctx->cfg.sandwiching_probability, ctx->cfg.max_iterations);
     a.use_basis = ctx->cfg.basis_probability > 0;
     a.thr_basis = threshold(ctx->cfg.basis_probability, ctx->cfg.max_iterations);
-    a.bfmt = ctx->bfmt;
+    // best-scheme formats: walk launches that cannot change a format skip
+    // their copies while every walker's best has its current format
+    a.bfmt = (ctx->bfmt_same && !(resize > 0)) ? nullptr : ctx->bfmt;
+    if (resize > 0 && ctx->bfmt_same) {
+        // a resize changes walkers' formats: from here on the copies write them
+        ctx->bfmt_same = false;
+        ctx->drop_graph();
+    }
     if (resize > 0) {
         // resizeKernel: MAX_RANK and MAX_MATRIX_ELEMENTS become the pool's
         // term capacity and the coefficients a factor word holds
@@ -1212,6 +1221,12 @@ extern "C" int fg_write_walkers(fg_ctx *ctx, int begin, int count, const fg_walk
     HIPCHK(hipMemcpy(ctx->aux_idx, ai.data(), W * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->fmt, fmt.data(), (size_t)16 * W, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->bfmt, bfmt.data(), (size_t)16 * W, hipMemcpyHostToDevice));
+    {
+        bool same = true;
+        for (size_t q = 0; q < (size_t)4 * W && same; ++q) same = (q % 4 == 3) || fmt[q] == bfmt[q];
+        if (same != ctx->bfmt_same) ctx->drop_graph();  // a captured launch holds the old argument
+        ctx->bfmt_same = same;
+    }
     HIPCHK(hipMemcpy(ctx->it, it.data(), W * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->aux, aux.data(), W * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->total_flips, tf.data(), W * 8, hipMemcpyHostToDevice));
