@@ -142,7 +142,7 @@ int main(int argc, char *argv[]) {
     parser.add("--max-iterations", ArgType::Natural, "INT", "number of flips per iterations", "10000");
     parser.add("--path", ArgType::String, "PATH", "path to save schemes", "schemes");
     parser.add("--input-path", ArgType::String, "PATH", "path to init scheme(s)", "");
-    parser.add("--block-size", ArgType::Natural, "INT", "number of walkers per workgroup request (the kernel uses 64)", "32");
+    parser.add("--block-size", ArgType::Natural, "INT", "walkers per workgroup (accepted; the runtime picks 32 or 64 with the LDS layout)", "32");
     parser.add("--target-complexity", ArgType::Natural, "INT", "target complexity", "0");
     parser.add("--max-no-improvements", ArgType::Natural, "INT", "max iterations without improvements", "3");
     parser.add("--seed", ArgType::Natural, "INT", "random seed", "0");

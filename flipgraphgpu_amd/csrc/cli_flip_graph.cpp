@@ -161,7 +161,7 @@ int main(int argc, char *argv[]) {
     parser.add("--max-iterations", ArgType::Natural, "INT", "number of flips per iterations", "10000");
     parser.add("--path", ArgType::String, "PATH", "path to save schemes", "schemes");
     parser.add("--input-path", ArgType::String, "PATH", "path to init schemes", "null");
-    parser.add("--block-size", ArgType::Natural, "INT", "number of walkers per workgroup request (the kernel uses 64)", "32");
+    parser.add("--block-size", ArgType::Natural, "INT", "walkers per workgroup (accepted; the runtime picks 32 or 64 with the LDS layout)", "32");
     parser.add("--expand-probability", ArgType::Real, "REAL", "expand edge probability (divided by max iterations)", "0.01");
     parser.add("--reduce-probability", ArgType::Real, "REAL", "reduce edge probability (divided by max iterations)", "0.0");
     parser.add("--sandwiching-probability", ArgType::Real, "REAL", "sandwiching edge probability (divided by max iterations)", "0.0");
