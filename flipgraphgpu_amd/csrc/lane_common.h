@@ -98,6 +98,48 @@ constexpr uint32_t kPW0 = 0x9E3779B9u, kPW1 = 0xBB67AE85u;
 constexpr int kMaxRank = 350;        // src/config.cuh:10
 constexpr int kPlusMaxTries = 1000;  // tryPlus / trySplitExisted rejection bound
 
+// a ^ b ^ c (GPU: one v_bitop3_b32 with the three-input XOR table 0x96,
+// which the compiler leaves as two XORs here).
+FGK_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+#else
+    return a ^ b ^ c;
+#endif
+}
+
+// Philox4x32-10 computed a few rounds at a time: start() sets the counter
+// and key, rounds<R0, R1>() applies rounds R0..R1-1 (round r keyed by
+// key + r * (kPW0, kPW1)); after rounds 0..9 x0..x3 equal philox10's output.
+struct PhiloxRounds {
+    uint32_t x0, x1, x2, x3, k0, k1;
+    FGK_HD void start(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t key0, uint32_t key1) {
+        x0 = c0;
+        x1 = c1;
+        x2 = c2;
+        x3 = c3;
+        k0 = key0;
+        k1 = key1;
+    }
+    template <int R0, int R1>
+    FGK_HD void rounds() {
+#pragma unroll
+        for (int r = R0; r < R1; ++r) {
+            const uint32_t rk0 = k0 + (uint32_t)r * kPW0, rk1 = k1 + (uint32_t)r * kPW1;
+            const uint64_t p0 = (uint64_t)kPM0 * x0;
+            const uint64_t p1 = (uint64_t)kPM1 * x2;
+            const uint32_t y0 = xor3((uint32_t)(p1 >> 32), x1, rk0);
+            const uint32_t y2 = xor3((uint32_t)(p0 >> 32), x3, rk1);
+            x0 = y0;
+            x1 = (uint32_t)p1;
+            x2 = y2;
+            x3 = (uint32_t)p0;
+        }
+    }
+};
+
 FGK_HD void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t &o0,
                      uint32_t &o1, uint32_t &o2, uint32_t &o3) {
 #pragma unroll
@@ -108,8 +150,8 @@ FGK_HD void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_
         }
         const uint64_t p0 = (uint64_t)kPM0 * c0;
         const uint64_t p1 = (uint64_t)kPM1 * c2;
-        const uint32_t y0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t y2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t y0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+        const uint32_t y2 = xor3((uint32_t)(p0 >> 32), c3, k1);
         c0 = y0;
         c1 = (uint32_t)p1;
         c2 = y2;
@@ -158,9 +200,23 @@ struct LaneBaseT {
     // v_mul_u32_u24 instead of a quarter-rate 32-bit multiply).
     FGK_HD static int mul24(int c, int x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-        return (int)__umul24((unsigned)c, (unsigned)x);
+        // the compiler may turn __umul24 of a value it cannot bound into a
+        // quarter-rate v_mul_lo_u32, so a product of two variables is
+        // written as the instruction itself
+        if (__builtin_constant_p(c) || __builtin_constant_p(x)) return (int)__umul24((unsigned)c, (unsigned)x);
+        int r;
+        asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(c), "v"(x));
+        return r;
 #else
         return c * x;
+#endif
+    }
+    // a * b for small signed values (|a|, |b| < 2^23: one v_mul_i32_i24)
+    FGK_HD static int smul24(int a, int b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __mul24(a, b);
+#else
+        return a * b;
 #endif
     }
     FGK_HD static const Word *fat(const Word *Fb, int fs, int c, int t) { return fat(const_cast<Word *>(Fb), fs, c, t); }
@@ -255,13 +311,21 @@ struct LaneBaseT {
     // H[c][t] += d (d = +-1) as one LDS add on t's dword that returns nothing,
     // so the wave does not wait for it (the byte never wraps: H counts are
     // below 256 and a decremented count is at least 1).
-    FGK_HD void h_add(int c, int t, int d) {
-        uint32_t *p = &hword(c, t >> 2);
-        const uint32_t v = (uint32_t)d << (8 * (t & 3));
+    FGK_HD void h_add(int c, int t, int d) { word_add(&hword(c, t >> 2), (uint32_t)d << (8 * (t & 3))); }
+    // *p += v as an LDS add that returns nothing (GPU)
+    FGK_HD static void word_add(uint32_t *p, uint32_t v) {
 #if defined(__HIP_DEVICE_COMPILE__)
         atomicAdd(p, v);
 #else
         *p += v;
+#endif
+    }
+    // all ones when bit y of x is set, else 0 (one v_bfe_i32)
+    FGK_HD static uint32_t sbit(uint32_t x, int y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return (uint32_t)__builtin_amdgcn_sbfe((int)x, (unsigned)y, 1u);
+#else
+        return 0u - ((x >> y) & 1u);
 #endif
     }
     FGK_HD void fail(int32_t code) { fgk_atomic_or(err, code); }
@@ -663,6 +727,14 @@ struct LaneBaseT {
 
 FGK_HD uint32_t min_u32(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
+// No instruction is moved across this point by the GPU compiler's
+// scheduler (nothing on the host).
+FGK_HD void sched_fence() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
 // x in a vector register (an opaque copy on the GPU), so values derived
 // from it stay in vector registers too.
 FGK_HD uint32_t vgpr_copy(uint32_t x) {
@@ -688,15 +760,27 @@ FGK_HD uint32_t nonzero_halves(uint32_t x) {
 #endif
 }
 
+// (acc << 1) | x as one v_lshl_or_b32 (the compiler otherwise spreads a
+// shift-or chain into a shift per element and three-input ors).
+FGK_HD uint32_t shl1_or(uint32_t acc, uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t r;
+    asm("v_lshl_or_b32 %0, %1, 1, %2" : "=v"(r) : "v"(acc), "v"(x));
+    return r;
+#else
+    return (acc << 1) | x;
+#endif
+}
+
 // Rows of a packed component (dword p: row 15 - p low, row 31 - p high)
 // that differ from v: bit t set when row t != v.  Shifting the two half
 // flags of dword p in at bits 0 and 16 leaves row 15 - p at bit 15 - p and
 // row 31 - p at bit 31 - p after the 16 dwords.
 FGK_HD uint32_t packed_ne(const uint32_t (&R)[16], uint32_t v) {
     const uint32_t vv = v | (v << 16);
-    uint32_t acc = 0;
+    uint32_t acc = nonzero_halves(R[0] ^ vv);
 #pragma unroll
-    for (int p = 0; p < 16; ++p) acc = (acc << 1) | nonzero_halves(R[p] ^ vv);
+    for (int p = 1; p < 16; ++p) acc = shl1_or(acc, nonzero_halves(R[p] ^ vv));
     return acc;
 }
 
@@ -748,6 +832,42 @@ template <class Lane>
 FGK_HD void copy_best(Lane &s, const WalkArgs &A, bool want, int lane, int w) {
 #if defined(__HIP_DEVICE_COMPILE__)
     uint64_t mask = __ballot(want && s.side == 0);
+    if constexpr (Lane::kPacked) {
+        // at most 32 terms: two walkers per pass, lanes 0-31 copy the first
+        // and lanes 32-63 the second (row t on lane t of its half); a pass
+        // with one walker left copies it on both halves
+        const int half = lane >> 5, t = lane & 31;
+        while (mask) {
+            const int sa = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const int sb = mask ? __builtin_ctzll(mask) : sa;
+            mask &= mask - 1;
+            const int msa = __builtin_amdgcn_readlane(s.m, sa), msb = __builtin_amdgcn_readlane(s.m, sb);
+            const int wsa = __builtin_amdgcn_readlane(w, sa), wsb = __builtin_amdgcn_readlane(w, sb);
+            const int src = half ? sb : sa, ms = half ? msb : msa, ws = half ? wsb : wsa;
+            if (A.bfmt) {
+                const int f0 = half ? __builtin_amdgcn_readlane(s.n0, sb) : __builtin_amdgcn_readlane(s.n0, sa);
+                const int f1 = half ? __builtin_amdgcn_readlane(s.n1, sb) : __builtin_amdgcn_readlane(s.n1, sa);
+                const int f2 = half ? __builtin_amdgcn_readlane(s.n2, sb) : __builtin_amdgcn_readlane(s.n2, sa);
+                if (t == 0) {
+                    int32_t *bf = A.bfmt + 4 * (size_t)ws;
+                    bf[0] = f0;
+                    bf[1] = f1;
+                    bf[2] = f2;
+                }
+            }
+            if (t < ms) {
+                const typename Lane::Word *Fs = s.F - Lane::fslot(lane % Lane::kWS) + Lane::fslot(src);
+                uint32_t *dst = A.best + (size_t)ws * 3 * A.max_m;
+                const int o = Lane::packed_off(t);
+                const uint32_t x0 = Fs[o], x1 = Fs[o + 32 * Lane::kWS], x2 = Fs[o + 64 * Lane::kWS];
+                dst[t] = x0;
+                dst[A.max_m + t] = x1;
+                dst[2 * A.max_m + t] = x2;
+            }
+        }
+        return;
+    }
     while (mask) {
         const int src = __builtin_ctzll(mask);
         mask &= mask - 1;
@@ -768,17 +888,6 @@ FGK_HD void copy_best(Lane &s, const WalkArgs &A, bool want, int lane, int w) {
         }
         const typename Lane::Word *Fs = s.F - Lane::fslot(slot) + Lane::fslot(src_slot);
         uint32_t *dst = A.best + (size_t)ws * 3 * A.max_m;
-        if constexpr (Lane::kPacked) {
-            // at most 32 terms: lane t copies row t of the three components
-            if (lane < ms) {
-                const int o = Lane::packed_off(lane);
-                const uint32_t x0 = Fs[o], x1 = Fs[o + 32 * Lane::kWS], x2 = Fs[o + 64 * Lane::kWS];
-                dst[lane] = x0;
-                dst[A.max_m + lane] = x1;
-                dst[2 * A.max_m + lane] = x2;
-            }
-            continue;
-        }
         // the three components' reads go out together: one LDS round trip per pass
         for (int t = lane; t < ms; t += kThreadsPerGroup) {
             const uint32_t x0 = *Lane::fat(Fs, s.fs, 0, t), x1 = *Lane::fat(Fs, s.fs, 1, t),
@@ -1020,6 +1129,7 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     // and swap them, so each lane runs one Philox per two iterations.
     uint32_t nb0 = 0, nb1 = 0, nb2 = 0, nb3 = 0;
     bool have_next = false;
+    PhiloxRounds px;
     // one-lane layout, plain and trace walks: every iteration's block is
     // computed during the previous flip (the first one here)
     constexpr bool kPrefetch = Lane::kPacked && (kMode == kModeWalk || kMode == kModeTrace);
@@ -1097,11 +1207,24 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
                 if (!ok) ok = s.try_flip(b0, b1, check_reduce);
             } else if constexpr (kPrefetch) {
                 // the next iteration's block (the counter is already advanced)
-                // fills the flip's first wait; the block is used only if the
-                // launch goes on
-                ok = s.try_flip(b0, b1, check_reduce, [&]() {
-                    philox10((uint32_t)it, (uint32_t)(it >> 32), (uint32_t)w, 0u, s.k0, s.k1, nb0, nb1, nb2, nb3);
-                    Lane::keep_loaded(nb0, nb1, nb2, nb3);  // computed here, not sunk to its use
+                // fills the flip's three read waits (rounds 0-3, 4-6, 7-9);
+                // the block is used only if the launch goes on
+                ok = s.try_flip(b0, b1, check_reduce, [&](auto phase) {
+                    constexpr int P = decltype(phase)::value;
+                    if constexpr (P == 0) {
+                        px.start((uint32_t)it, (uint32_t)(it >> 32), (uint32_t)w, 0u, s.k0, s.k1);
+                        px.template rounds<0, 4>();
+                    } else if constexpr (P == 1) {
+                        px.template rounds<4, 7>();
+                    } else {
+                        px.template rounds<7, 10>();
+                        nb0 = px.x0;
+                        nb1 = px.x1;
+                        nb2 = px.x2;
+                        nb3 = px.x3;
+                        Lane::keep_loaded(nb0, nb1, nb2, nb3);  // computed here, not sunk to its use
+                    }
+                    Lane::keep_loaded(px.x0, px.x1, px.x2, px.x3);
                 });
             } else {
                 ok = s.try_flip(b0, b1, check_reduce);

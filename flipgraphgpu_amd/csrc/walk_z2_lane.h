@@ -528,14 +528,16 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
     // factors and class masks of index1 and index2 the halves need; (4) the
     // class stores.  The new classes come from the rows of (1) compared with
     // the new factors (packed_ne) instead of a member's class mask.
-    // `pre` runs once per call, after the first batch of reads is issued, so
-    // the caller's work that needs no scheme state (the next iteration's
-    // Philox block) fills the wait for them.
+    // `pre(phase)` is called three times per call, with phase 0, 1 and 2 as
+    // std::integral_constant, once after each of the three batches of reads
+    // is issued (on every return path, so all three always run), so the
+    // caller's work that needs no scheme state (the next iteration's Philox
+    // block, in three parts) fills the waits for them.
     template <class Pre>
     FGK_HD bool try_flip_packed(uint32_t r, uint32_t coin, bool check_reduce, Pre &&pre) {
         const uint32_t total = S0 + S1 + S2;
         if (!total) {
-            pre();
+            pre_all(pre);
             return false;
         }
         uint32_t rr = draw_below(r, total);
@@ -558,17 +560,31 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
             Rj[p] = this->fpair(j, p);
             Rk[p] = this->fpair(k, p);
         }
-        pre();
+        // every read of the first batch is issued before `pre` and the
+        // selection (the scheduler would otherwise sink the row reads below
+        // the selection, behind the H words' wait)
+        sched_fence();
+        pre(std::integral_constant<int, 0>{});
+        sched_fence();
         int t1;
         B::select_words(hw8, rr, t1);
         // the (rr + 1)-th member above t1 (t1 clamped into the plane for the
         // read; an out-of-range t1 or an empty q is reported below)
         const int t1c = t1 < m ? t1 : 0;
         const uint32_t cmt1 = cm(c, t1c);
+        sched_fence();
+        pre(std::integral_constant<int, 1>{});
+        sched_fence();
+        // clear the rr lowest members: two steps as selects, more (rare)
+        // in a loop (clearing from an empty q leaves it empty)
         uint32_t q = cmt1 & ~((2u << t1c) - 1u);
-        for (uint32_t e = 0; e < rr && q; ++e) q &= q - 1u;
+        const uint32_t q1 = q & (q - 1u), q2 = q1 & (q1 - 1u);
+        q = rr == 0u ? q : (rr == 1u ? q1 : q2);
+        if (rr > 2u)
+            for (uint32_t e = 2; e < rr && q; ++e) q &= q - 1u;
         if (t1 >= m || !q) {
             this->fail(kErrSelect);
+            pre(std::integral_constant<int, 2>{});
             return false;
         }
         const int t2 = __builtin_ctz(q);
@@ -590,6 +606,9 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
         const uint32_t oldk = Ck[i2 * B::kWS], cj2 = Cj[i2 * B::kWS];
         // index1 and index2 share their component-i class: t1's mask
         const uint32_t ci1 = cmt1, ci2 = cmt1;
+        sched_fence();
+        pre(std::integral_constant<int, 2>{});
+        sched_fence();
         // one round trip for all eight reads: the masks are read before the
         // zero-factor test, not after it
         keep_loaded(fj1, fj2, fk1, fk2, oldj, ck1, oldk, cj2);
@@ -618,9 +637,18 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
         pop_update(fj1, A, fk2, Bv);
         int dj, dk;
         mask_move2(j, i1, oldj, nwj, A, k, i2, oldk, nwk, Bv, dj, dk);
-        S0 += (uint32_t)(j == 0 ? dj : (k == 0 ? dk : 0));
-        S1 += (uint32_t)(j == 1 ? dj : (k == 1 ? dk : 0));
-        S2 += (uint32_t)(j == 2 ? dj : (k == 2 ? dk : 0));
+        // the pair-count changes of components j and k: d_lo goes to the
+        // lower of the two components other than c (j unless coin bit 0
+        // swapped j and k), d_hi to the higher; the choices are bit masks
+        // (0 or all ones) from the bits of c and the coin, not selects on
+        // compare results
+        const uint32_t ms = 0u - (coin & 1u), cu = (uint32_t)c;
+        const uint32_t ujd = (uint32_t)dj, ukd = (uint32_t)dk;
+        const uint32_t d_lo = (ukd & ms) | (ujd & ~ms), d_hi = (ujd & ms) | (ukd & ~ms);
+        const uint32_t m_ne0 = 0u - ((cu | (cu >> 1)) & 1u), m_is2 = 0u - (cu >> 1);
+        S0 += d_lo & m_ne0;
+        S1 += (d_lo & ~m_ne0) | (d_hi & m_is2);
+        S2 += d_hi & ~m_is2;
         this->mark(5);
         return true;
     }
@@ -652,14 +680,22 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
         const uint32_t bj = 1u << xj, belj = bj - 1u, restj = oldj & ~bj, joinj = nwj | bj;
         const uint32_t bk = 1u << xk, belk = bk - 1u, restk = oldk & ~bk, joink = nwk | bk;
         uint32_t qj = restj | nwj, qk = restk | nwk;
+        // per-component bases, so a member's class-mask and H addresses are
+        // one shift-add each
+        uint32_t *Cj = &cm(j, 0), *Ck = &cm(k, 0);
+        uint32_t *Hj = &this->hword(j, 0), *Hk = &this->hword(k, 0);
         while (qj | qk) {
             const int yj = qj ? __builtin_ctz(qj) : xj;
             const int yk = qk ? __builtin_ctz(qk) : xk;
-            const bool nj = (nwj >> yj) & 1u, nk = (nwk >> yk) & 1u;
-            cm(j, yj) = nj ? joinj : restj;
-            cm(k, yk) = nk ? joink : restk;
-            h_add(j, yj, ((belj >> yj) & 1u) ? (nj ? 1 : -1) : 0);
-            h_add(k, yk, ((belk >> yk) & 1u) ? (nk ? 1 : -1) : 0);
+            // all-ones when the member is in the new class / below x
+            const uint32_t mnj = B::sbit(nwj, yj), mnk = B::sbit(nwk, yk);
+            const uint32_t mbj = B::sbit(belj, yj), mbk = B::sbit(belk, yk);
+            Cj[yj * B::kWS] = (mnj & joinj) | (~mnj & restj);
+            Ck[yk * B::kWS] = (mnk & joink) | (~mnk & restk);
+            // H change of the member: +1 joining below x, -1 leaving below
+            // x, 0 above x
+            B::word_add(Hj + (yj >> 2) * B::kWS, ((~mnj | 1u) & mbj) << (8 * (yj & 3)));
+            B::word_add(Hk + (yk >> 2) * B::kWS, ((~mnk | 1u) & mbk) << (8 * (yk & 3)));
             qj &= qj - 1u;
             qk &= qk - 1u;
         }
@@ -681,12 +717,19 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
         if constexpr (B::kPacked) {
             return try_flip_packed(r, coin, check_reduce, pre);
         } else {
-            pre();
+            pre_all(pre);
             return try_flip(r, coin, check_reduce);
         }
     }
+    // the three phases of `pre` at once
+    template <class Pre>
+    FGK_HD static void pre_all(Pre &&pre) {
+        pre(std::integral_constant<int, 0>{});
+        pre(std::integral_constant<int, 1>{});
+        pre(std::integral_constant<int, 2>{});
+    }
     FGK_HD bool try_flip(uint32_t r, uint32_t coin, bool check_reduce) {
-        if constexpr (B::kPacked) return try_flip_packed(r, coin, check_reduce, [] {});
+        if constexpr (B::kPacked) return try_flip_packed(r, coin, check_reduce, [](auto) {});
         const uint32_t total = S0 + S1 + S2;
         if (!total) return false;
         int c, t1, t2, j, k;
