@@ -110,6 +110,43 @@ FGK_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
 #endif
 }
 
+// Per 16-bit half of x: 1 when the half is non-zero, else 0 (one
+// v_pk_min_u16 on the GPU; written out, since the compiler turns a packed
+// minimum into two compares and selects per dword).
+FGK_HD uint32_t nonzero_halves(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t r;
+    asm("v_pk_min_u16 %0, %1, %2" : "=v"(r) : "v"(x), "s"(0x00010001u));
+    return r;
+#else
+    return ((x & 0xFFFFu) ? 1u : 0u) | ((x >> 16) ? 0x10000u : 0u);
+#endif
+}
+
+// (acc << 1) | x as one v_lshl_or_b32 (the compiler otherwise spreads a
+// shift-or chain into a shift per element and three-input ors).
+FGK_HD uint32_t shl1_or(uint32_t acc, uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t r;
+    asm("v_lshl_or_b32 %0, %1, 1, %2" : "=v"(r) : "v"(acc), "v"(x));
+    return r;
+#else
+    return (acc << 1) | x;
+#endif
+}
+
+// Rows of a packed component (dword p: row 15 - p low, row 31 - p high)
+// that differ from v: bit t set when row t != v.  Shifting the two half
+// flags of dword p in at bits 0 and 16 leaves row 15 - p at bit 15 - p and
+// row 31 - p at bit 31 - p after the 16 dwords.
+FGK_HD uint32_t packed_ne(const uint32_t (&R)[16], uint32_t v) {
+    const uint32_t vv = v | (v << 16);
+    uint32_t acc = nonzero_halves(R[0] ^ vv);
+#pragma unroll
+    for (int p = 1; p < 16; ++p) acc = shl1_or(acc, nonzero_halves(R[p] ^ vv));
+    return acc;
+}
+
 // Philox4x32-10 computed a few rounds at a time: start() sets the counter
 // and key, rounds<R0, R1>() applies rounds R0..R1-1 (round r keyed by
 // key + r * (kPW0, kPW1)); after rounds 0..9 x0..x3 equal philox10's output.
@@ -409,15 +446,28 @@ struct LaneBaseT {
         uint32_t v[R];
 #pragma unroll
         for (int e = 0; e < R; ++e) v[e] = f(c, e);
+        // the one-lane layout also keeps the rows as packed pairs, so a class
+        // is one packed compare (packed_ne) instead of R compares
+        uint32_t P[kPackedPairs];
+        if constexpr (kPacked && R == 32) {
+#pragma unroll
+            for (int p = 0; p < kPackedPairs; ++p) P[p] = fpair(c, p);
+        }
         const M live = m >= R ? ~M(0) : ((M(1) << m) - M(1));
         M done = ~live;
         uint32_t S = 0;
         while (~done) {
             const int t = ctz_m(~done);
-            const uint32_t vt = f(c, t);
+            // the leader's factor from the registers (a select tree, not an
+            // LDS round trip per class)
+            const uint32_t vt = pick_row<R>(v, t);
             M mask = 0;
+            if constexpr (kPacked && R == 32) {
+                mask = (M)~packed_ne(P, vt);
+            } else {
 #pragma unroll
-            for (int e = 0; e < R; ++e) mask |= M(v[e] == vt ? 1u : 0u) << e;
+                for (int e = 0; e < R; ++e) mask |= M(v[e] == vt ? 1u : 0u) << e;
+            }
             mask &= live;
             done |= mask;
             const uint32_t k = popc_m(mask);
@@ -431,6 +481,25 @@ struct LaneBaseT {
             }
         }
         return S;
+    }
+    // v[t] for a lane-varying t < R as a binary select tree over the
+    // registers (GPU; plain indexing on the host)
+    template <int R>
+    FGK_HD static uint32_t pick_row(const uint32_t (&v)[R], int t) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        uint32_t w[R];
+#pragma unroll
+        for (int e = 0; e < R; ++e) w[e] = v[e];
+#pragma unroll
+        for (int half = R / 2, b = 0; half >= 1; half /= 2, ++b) {
+            const bool hi = ((t >> b) & 1) != 0;
+#pragma unroll
+            for (int e = 0; e < half; ++e) w[e] = hi ? w[2 * e + 1] : w[2 * e];
+        }
+        return w[0];
+#else
+        return v[t];
+#endif
     }
     FGK_HD static int ctz_m(uint32_t x) { return __builtin_ctz(x); }
     FGK_HD static int ctz_m(uint64_t x) { return __builtin_ctzll(x); }
@@ -745,43 +814,6 @@ FGK_HD uint32_t vgpr_copy(uint32_t x) {
 #else
     return x;
 #endif
-}
-
-// Per 16-bit half of x: 1 when the half is non-zero, else 0 (one
-// v_pk_min_u16 on the GPU; written out, since the compiler turns a packed
-// minimum into two compares and selects per dword).
-FGK_HD uint32_t nonzero_halves(uint32_t x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    uint32_t r;
-    asm("v_pk_min_u16 %0, %1, %2" : "=v"(r) : "v"(x), "s"(0x00010001u));
-    return r;
-#else
-    return ((x & 0xFFFFu) ? 1u : 0u) | ((x >> 16) ? 0x10000u : 0u);
-#endif
-}
-
-// (acc << 1) | x as one v_lshl_or_b32 (the compiler otherwise spreads a
-// shift-or chain into a shift per element and three-input ors).
-FGK_HD uint32_t shl1_or(uint32_t acc, uint32_t x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    uint32_t r;
-    asm("v_lshl_or_b32 %0, %1, 1, %2" : "=v"(r) : "v"(acc), "v"(x));
-    return r;
-#else
-    return (acc << 1) | x;
-#endif
-}
-
-// Rows of a packed component (dword p: row 15 - p low, row 31 - p high)
-// that differ from v: bit t set when row t != v.  Shifting the two half
-// flags of dword p in at bits 0 and 16 leaves row 15 - p at bit 15 - p and
-// row 31 - p at bit 31 - p after the 16 dwords.
-FGK_HD uint32_t packed_ne(const uint32_t (&R)[16], uint32_t v) {
-    const uint32_t vv = v | (v << 16);
-    uint32_t acc = nonzero_halves(R[0] ^ vv);
-#pragma unroll
-    for (int p = 1; p < 16; ++p) acc = shl1_or(acc, nonzero_halves(R[p] ^ vv));
-    return acc;
 }
 
 // A uniform draw in [0, size): floor(x * size / 2^32) (one v_mul_hi_u32; the
