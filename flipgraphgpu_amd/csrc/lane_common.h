@@ -279,6 +279,17 @@ struct LaneBaseT {
     uint32_t vmask; // coefficient bits of a factor word (all for Z2, bits 0-15 for the integer ring)
     int pop;        // coefficient count of the scheme, kept for getComplexity
     bool track_pop; // keep pop current through flips (the minimizer reads it every flip)
+    // One-lane layout: rows changed since CM, H and S0..S2 were last exact
+    // (bit t of dirtyC: row t of component C), set by the operators that
+    // change a few rows (a reduction after a flip, plus, split) before they
+    // bring the classes up to date; zero otherwise.
+    uint32_t dirty0, dirty1, dirty2;
+    FGK_HD void mark_row(int c, int t) {
+        const uint32_t b = 1u << t;
+        dirty0 |= c == 0 ? b : 0u;
+        dirty1 |= c == 1 ? b : 0u;
+        dirty2 |= c == 2 ? b : 0u;
+    }
     int sel_c, sel_t1, sel_t2;  // pair of the last tryFlip (component, index1 < index2)
     // Flip-trace launches only (null in every other kernel, where the
     // recording folds away): the operators applied to this walker's scheme,
@@ -458,9 +469,12 @@ struct LaneBaseT {
         uint32_t S = 0;
         while (~done) {
             const int t = ctz_m(~done);
-            // the leader's factor from the registers (a select tree, not an
-            // LDS round trip per class)
-            const uint32_t vt = pick_row<R>(v, t);
+            // the one-lane layout (one wave per SIMD, nothing to hide an LDS
+            // round trip behind) takes the leader's factor from the
+            // registers by a select tree; the lane pairs read it
+            uint32_t vt;
+            if constexpr (kPacked) vt = pick_row<R>(v, t);
+            else vt = f(c, t);
             M mask = 0;
             if constexpr (kPacked && R == 32) {
                 mask = (M)~packed_ne(P, vt);
@@ -1027,6 +1041,7 @@ FGK_HD void walk_lane(const WalkArgs &A, uint8_t *lds, int lane, int w, int w_en
     s.k0 = Lane::kPacked ? vgpr_copy(A.key0) : A.key0;
     s.k1 = Lane::kPacked ? vgpr_copy(A.key1) : A.key1;
     s.wid = (uint32_t)w;
+    s.dirty0 = s.dirty1 = s.dirty2 = 0u;
     s.m = 0;
     s.n0 = s.n1 = s.n2 = 1;
     s.visit_cap = A.zt_visit_cap;

@@ -96,6 +96,7 @@ constexpr int32_t kErrSelect = 1;   // pair draw found no term (partner counts i
 constexpr int32_t kErrWalk = 2;     // class list walk exceeded the term count
 constexpr int32_t kErrCapacity = 4; // a term would exceed the LDS capacity
 constexpr int32_t kErrArgs = 8;     // the argument block's layout differs from the kernel's
+constexpr int32_t kErrCommit = 16;  // host check: an incremental class update differs from a rebuild
 
 // Padding of the packed H / N planes: a multiple of 32 terms (8 words of 4
 // terms) so the selection scan reads whole 8-word blocks.

@@ -83,6 +83,10 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
     using B::nonzero_term;
     using B::remove_at;
     using B::rebuild;
+    using B::dirty0;
+    using B::dirty1;
+    using B::dirty2;
+    using B::mark_row;
     using B::class_walk;
     using B::leave;
     using B::link_join;
@@ -117,13 +121,122 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
             }
         }
     }
-    // reduce (scheme_z2.cu:631-641)
-    FGK_HD void reduce(int c, int a, int b) {
+    // reduce (scheme_z2.cu:631-641).  `marked`: every row changed since the
+    // classes were last exact is marked in dirty0..2 (a flip's two rows,
+    // marked by flip_reduce), so the one-lane layout may update the classes
+    // instead of rebuilding them; other callers rebuild.
+    FGK_HD void reduce(int c, int a, int b, bool marked = false) {
         f(c, a) ^= f(c, b);
         const bool zero = f(c, a) == 0;
+        if constexpr (B::kPacked) {
+            if (marked && !zero && !this->track_pop) {
+                // term b is replaced by the last term: rows b and the old
+                // last change in every component, row a in component c
+                const int old_m = m;
+                remove_at(b);
+                mark_row(c, a);
+                const uint32_t moved = (1u << b) | (1u << (old_m - 1));
+                dirty0 |= moved;
+                dirty1 |= moved;
+                dirty2 |= moved;
+                commit_rows(old_m);
+                return;
+            }
+            dirty0 = dirty1 = dirty2 = 0u;
+        }
         remove_at(b);
         if (zero) remove_zeroes();
         rebuild();
+    }
+
+    // ---- one-lane layout: class bookkeeping after a few row changes ----
+    // The classes of the scheme after its rows marked in dirty0..2 changed
+    // and its rank went from old_m to m, equal to what rebuild() computes.
+    // A row y that is neither a changed row D nor in a changed row's old or
+    // new class keeps its class and H.  For the others: a changed live row's
+    // class E is the packed compare of its new factor; an unchanged row's
+    // class is its old class without D plus the changed rows whose E holds
+    // it; H(y) = members above y; S changes by the H changes.  Falls back to
+    // rebuild() when a component has more than four changed rows.
+    FGK_HD void commit_rows(int old_m) {
+        const uint32_t live = m >= 32 ? 0xFFFFFFFFu : ((1u << m) - 1u);
+        const uint32_t olive = old_m >= 32 ? 0xFFFFFFFFu : ((1u << old_m) - 1u);
+        if (popc(dirty0) > 4 || popc(dirty1) > 4 || popc(dirty2) > 4) {
+            dirty0 = dirty1 = dirty2 = 0u;
+            rebuild();
+            return;
+        }
+        // rows past the rank are zero (the row compares rely on it)
+        for (int t = m; t < old_m; ++t) {
+            f(0, t) = 0;
+            f(1, t) = 0;
+            f(2, t) = 0;
+        }
+        for (int c = 0; c < 3; ++c) {
+            const uint32_t D = (c == 0 ? dirty0 : (c == 1 ? dirty1 : dirty2)) & (live | olive);
+            if (!D) continue;
+            uint32_t P[B::kPackedPairs];
+#pragma unroll
+            for (int p = 0; p < B::kPackedPairs; ++p) P[p] = this->fpair(c, p);
+            // the changed rows (up to four), their new classes and old masks
+            int dk[4];
+            uint32_t vk[4], ok[4], Ek[4];
+            uint32_t q = D;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                dk[e] = q ? __builtin_ctz(q) : -1;
+                q &= q - 1u;
+                const int d = dk[e] < 0 ? 0 : dk[e];
+                vk[e] = f(c, d);
+                ok[e] = cm(c, d);
+            }
+            uint32_t A = D;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const bool has = dk[e] >= 0;
+                Ek[e] = has && dk[e] < m ? (~packed_ne(P, vk[e]) & live) : 0u;
+                A |= Ek[e] | (has && dk[e] < old_m ? (ok[e] & olive) : 0u);
+            }
+            int dS = 0;
+            for (uint32_t r = A; r; r &= r - 1u) {
+                const int y = __builtin_ctz(r);
+                const uint32_t above = ~((2u << y) - 1u);
+                const uint32_t oldmask = y < old_m ? (cm(c, y) & olive) : 0u;
+                uint32_t nw;
+                if ((D >> y) & 1u) {
+                    nw = 0u;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) nw |= dk[e] == y ? Ek[e] : 0u;
+                } else {
+                    nw = oldmask & ~D;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) nw |= dk[e] >= 0 && ((Ek[e] >> y) & 1u) ? (1u << dk[e]) : 0u;
+                }
+                nw = y < m ? nw : 0u;
+                const int hold = popc(oldmask & above), hnew = popc(nw & above);
+                cm(c, y) = nw;
+                this->h(c, y) = (uint8_t)hnew;
+                dS += hnew - hold;
+            }
+            addS(c, dS);
+        }
+        dirty0 = dirty1 = dirty2 = 0u;
+#if defined(FGK_CHECK_COMMIT) && !defined(__HIP_DEVICE_COMPILE__)
+        // host emulation built with FGK_CHECK_COMMIT: the update must equal
+        // a full rebuild (masks, H and pair counts of every live row)
+        uint32_t cmv[3][32], hv[3][32];
+        const uint32_t Sv[3] = {S0, S1, S2};
+        for (int c = 0; c < 3; ++c)
+            for (int t = 0; t < m; ++t) {
+                cmv[c][t] = cm(c, t);
+                hv[c][t] = this->h(c, t);
+            }
+        rebuild();
+        bool same = Sv[0] == S0 && Sv[1] == S1 && Sv[2] == S2;
+        for (int c = 0; c < 3; ++c)
+            for (int t = 0; t < m; ++t) same = same && cmv[c][t] == cm(c, t) && hv[c][t] == this->h(c, t);
+        if (!same) this->fail(kErrCommit);
+#endif
     }
 
     // flip (scheme_z2.cu:531-577) taken literally; used when a zero factor is
@@ -236,8 +349,12 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
                             int rB) {
         f(j, i1) = A;
         f(k, i2) = Bv;
-        if (rA && tA <= tB) reduce(rA == 1 ? k : i, tA, i1);
-        else reduce(rB == 1 ? j : i, tB, i2);
+        if constexpr (B::kPacked) {
+            mark_row(j, i1);
+            mark_row(k, i2);
+        }
+        if (rA && tA <= tB) reduce(rA == 1 ? k : i, tA, i1, true);
+        else reduce(rB == 1 ? j : i, tB, i2, true);
         this->mark(8);
     }
 
@@ -401,6 +518,20 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
         }
         f(i, m) = u; f(j, m) = v; f(k, m) = w;
         ++m;
+        if constexpr (B::kPacked) {
+            if (a && b && c && !this->track_pop) {
+                // the two changed rows and the new term's row in every component
+                mark_row(variant == 0 ? j : (variant == 1 ? k : i), i1);
+                mark_row(variant == 0 ? i : (variant == 1 ? j : k), i2);
+                const uint32_t nb = 1u << (m - 1);
+                dirty0 |= nb;
+                dirty1 |= nb;
+                dirty2 |= nb;
+                commit_rows(m - 1);
+                return;
+            }
+            dirty0 = dirty1 = dirty2 = 0u;
+        }
         if (!a || !b || !c) remove_zeroes();
         rebuild();
     }
@@ -414,6 +545,18 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
         f(j, m) = f(j, index);
         f(k, m) = f(k, index);
         ++m;
+        if constexpr (B::kPacked) {
+            if (x1 && x2 && !this->track_pop) {
+                mark_row(i, index);
+                const uint32_t nb = 1u << (m - 1);
+                dirty0 |= nb;
+                dirty1 |= nb;
+                dirty2 |= nb;
+                commit_rows(m - 1);
+                return;
+            }
+            dirty0 = dirty1 = dirty2 = 0u;
+        }
         rebuild();
     }
 
