@@ -237,13 +237,19 @@ struct LaneBaseT {
     // v_mul_u32_u24 instead of a quarter-rate 32-bit multiply).
     FGK_HD static int mul24(int c, int x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-        // the compiler may turn __umul24 of a value it cannot bound into a
-        // quarter-rate v_mul_lo_u32, so a product of two variables is
-        // written as the instruction itself
-        if (__builtin_constant_p(c) || __builtin_constant_p(x)) return (int)__umul24((unsigned)c, (unsigned)x);
-        int r;
-        asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(c), "v"(x));
-        return r;
+        // in the one-lane layout the compiler turned __umul24 of a value it
+        // cannot bound into a quarter-rate v_mul_lo_u32 on the flip's path,
+        // so there a product of two variables is written as the instruction
+        // itself (the lane-pair layouts keep __umul24, which the compiler
+        // folds into their address arithmetic)
+        if constexpr (kPacked) {
+            if (!__builtin_constant_p(c) && !__builtin_constant_p(x)) {
+                int r;
+                asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(c), "v"(x));
+                return r;
+            }
+        }
+        return (int)__umul24((unsigned)c, (unsigned)x);
 #else
         return c * x;
 #endif
