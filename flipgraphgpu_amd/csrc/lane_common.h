@@ -287,8 +287,9 @@ struct LaneBaseT {
     bool track_pop; // keep pop current through flips (the minimizer reads it every flip)
     // One-lane layout: rows changed since CM, H and S0..S2 were last exact
     // (bit t of dirtyC: row t of component C), set by the operators that
-    // change a few rows (a reduction after a flip, plus, split) before they
-    // bring the classes up to date; zero otherwise.
+    // change a few rows (a reduction after a flip, a flip that leaves a zero
+    // factor, plus, split) before they bring the classes up to date; zero
+    // otherwise.
     uint32_t dirty0, dirty1, dirty2;
     FGK_HD void mark_row(int c, int t) {
         const uint32_t b = 1u << t;

@@ -121,6 +121,38 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
             }
         }
     }
+    // remove_zeroes for the one-lane layout (at most 32 terms), with every
+    // row it moves or drops marked in dirty0..2: the terms with a zero factor
+    // come from one packed compare per component, then the same trailing
+    // trim and swaps, including the unchecked swapped-in term.
+    FGK_HD void remove_zero_rows() {
+        auto lowmask = [](int n) { return n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u); };
+        uint32_t Z = 0u;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            uint32_t P[B::kPackedPairs];
+#pragma unroll
+            for (int p = 0; p < B::kPackedPairs; ++p) P[p] = this->fpair(c, p);
+            Z |= ~packed_ne(P, 0u);
+        }
+        Z &= lowmask(m);
+        uint32_t moved = 0u;
+        while (m > 0 && ((Z >> (m - 1)) & 1u)) {
+            --m;
+            moved |= 1u << m;
+        }
+        for (uint32_t q = Z & lowmask(m); q; q &= lowmask(m)) {
+            const int t = __builtin_ctz(q);
+            q &= q - 1u;
+            --m;
+            copy_term(t, m);
+            moved |= (1u << t) | (1u << m);
+        }
+        dirty0 |= moved;
+        dirty1 |= moved;
+        dirty2 |= moved;
+    }
+
     // reduce (scheme_z2.cu:631-641).  `marked`: every row changed since the
     // classes were last exact is marked in dirty0..2 (a flip's two rows,
     // marked by flip_reduce), so the one-lane layout may update the classes
@@ -245,6 +277,19 @@ struct Z2LaneT : LaneBaseT<WordT, kMaskT, kWSv> {
         f(j, i1) ^= f(j, i2);
         f(k, i2) ^= f(k, i1);
         if (!f(j, i1) || !f(k, i2)) {
+            if constexpr (B::kPacked) {
+                if (!this->track_pop) {
+                    // the two changed rows and the rows removeZeroes moves
+                    // or drops: the classes are updated, not rebuilt
+                    mark_row(j, i1);
+                    mark_row(k, i2);
+                    const int old_m = m;
+                    remove_zero_rows();
+                    commit_rows(old_m);
+                    return;
+                }
+                dirty0 = dirty1 = dirty2 = 0u;
+            }
             remove_zeroes();
             rebuild();
             return;
