@@ -440,8 +440,6 @@ constexpr int kPQs = 2 * kPD + 8;                // LDS row stride of the pair-h
 static_assert(kProposeMaxTerms * (kPXs + kPEs) * 2 <= kPSmem, "feature + activation tiles must fit the G tile");
 static_assert(16 * kPQs * 2 <= kPSmem && kProposeMaxK <= 16, "pair-head rows must fit the G tile");
 
-__device__ __forceinline__ float bf16r(float x) { return (float)(bf16)x; }
-
 // acc[rt][ct] += A[64 x K] (LDS, row stride lda) * Bt^T, for this wave's 32
 // columns; Bt is the transposed weight [128][ldb] in global memory.
 __device__ __forceinline__ void propose_mma(floatx4 (&acc)[4][2], const bf16 *A, int lda, const bf16 *Bt, int ldb,
