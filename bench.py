@@ -11,7 +11,8 @@ roofline: the walk kernel is bound by VALU integer throughput.  Algorithmic
 work per applied flip at rank m is W = 2 + 2m word operations (the two XOR
 updates and the equality test of the two new factors against all m terms);
 achieved = sum(W) / (sum of walk-kernel durations from HIP events), peak =
-CUs * clock * 64 lanes.
+CUs * clock * 128 lanes (gfx950 SIMDs execute two wave64 VALU instructions
+per 4 cycles, from two waves), checked on the device by fg_valu_probe.
 cpu_baseline: the oracle's restatement of the reference walk (history-
 ordered FlipSet, as the reference keeps it), built -O3 -march=native on this
 host, on every CPU this process may use; median of 5 bounded runs of the same
@@ -634,7 +635,11 @@ def main():
     _, best_rank, _ = e.best()
     e.close()
 
-    peak = info["cu"] * info["clock_khz"] * 1e3 * 64
+    # gfx950 VALU peak: a SIMD executes two wave64 instructions per 4 cycles
+    # (from two waves), 128 lanes per CU-clock, the rate behind the published
+    # FP32 vector figure; fg_valu_probe measures it on this device
+    peak = info["cu"] * info["clock_khz"] * 1e3 * 128
+    probe_ops, probe_ms = fg.valu_probe()
     traffic_src = "profiles/walk_z2_3x3x3_hbm_traffic.md"
     default_shape = n == (3, 3, 3) and a.walkers == 65536 and a.iterations == 10000 and not zt
     traffic = profiled_traffic(os.path.join(ROOT, traffic_src)) if default_shape else None
@@ -674,6 +679,9 @@ def main():
             "traffic_source": traffic_src if traffic is not None else None,
             "kernel_ms_per_step": kernel_ms / a.steps,
             "ops_per_flip": ops / flips if flips else None,
+            "peak_measured": probe_ops / 1e12,
+            "peak_measured_by": f"fg_valu_probe: independent v_xor_b32 chains, 16384 x 256 threads, {probe_ms:.3f} ms",
+            "peak_one_wave_per_simd": peak / 2e12,
             "device": info,
         },
     }

@@ -166,6 +166,15 @@ def device_info():
     return {"cu": cu.value, "clock_khz": clk.value, "lds_per_block": lds.value, "arch": name.value.decode()}
 
 
+def valu_probe(blocks=16384, trips=100):
+    """Measured 32-bit integer VALU rate of the current GPU (fg_valu_probe):
+    (lane operations per second, kernel ms of the fastest timed launch)."""
+    L = load_library()
+    ops, ms = ctypes.c_double(), ctypes.c_double()
+    _check(L.fg_valu_probe(int(blocks), int(trips), ctypes.byref(ops), ctypes.byref(ms)), "fg_valu_probe")
+    return ops.value, ms.value
+
+
 class Engine:
     """One walker pool on the current GPU (fg_ctx)."""
 

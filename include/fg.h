@@ -381,6 +381,15 @@ FG_API int fg_reducer_words(fg_reducer *r, uint64_t *out);
 
 /* Device properties used for roofline reporting. */
 FG_API int fg_device_info(int *cu_count, int *clock_khz, int *lds_per_block, char *name, int name_cap);
+/* The device's 32-bit integer VALU rate, measured: `blocks` workgroups of
+ * 256 threads whose lanes each issue trips x 256 independent v_xor_b32;
+ * writes the lane operations per second of the fastest of three timed
+ * launches (and its kernel time in ms when kernel_ms is not null).  No
+ * reference counterpart: it checks on the device at hand the peak the walk's
+ * roofline fraction is quoted against (CUs x clock x 128 lanes on gfx950,
+ * whose SIMDs execute two wave64 VALU instructions per 4 cycles from two
+ * waves). */
+FG_API int fg_valu_probe(int blocks, int trips, double *lane_ops_per_s, double *kernel_ms);
 
 #ifdef __cplusplus
 }
